@@ -1,0 +1,277 @@
+/*
+ * sph_capi.h -- C-ABI of the MI355X-native DFSPH time step and neighbourhood
+ * search (libsph_amd.so, gfx950).
+ *
+ * Two handle types replace the two C++ interfaces of SPlisHSPlasH v2.16.0 on
+ * the hot path:
+ *
+ *   sph_ns_t     -> SPH::NeighborhoodSearch, the typedef to
+ *                   CompactNSearch::NeighborhoodSearch
+ *                   (SPlisHSPlasH/NeighborhoodSearch.h:8-14). Member calls
+ *                   used by the reference are listed per entry point.
+ *   sph_dfsph_t  -> SPH::TimeStepDFSPH (SPlisHSPlasH/DFSPH/TimeStepDFSPH.h:24-76)
+ *                   together with the per-particle state it steps
+ *                   (FluidModel.h:113-124, SimulationDataDFSPH.h:96-211) and
+ *                   the boundary models it reads (BoundaryModel_Akinci2012,
+ *                   BoundaryModel_Bender2019).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: SPH_OK (0) or a negative
+ *     SPH_ERR_* code; sph_last_error() gives a message for the calling thread.
+ *     No C++ exception crosses this boundary.
+ *   - Host buffers stay owned by the caller; device memory is owned by the
+ *     library. Each handle owns one HIP stream; a handle is not thread-safe.
+ *   - Positions are 3 contiguous reals per point (the layout of
+ *     std::vector<Vector3r> with Eigen::DontAlign). dtype selects float
+ *     (SPH_F32) or double (SPH_F64) input; device arithmetic is FP32 with FP64
+ *     reductions.
+ *   - There is no CPU fallback: if no gfx950 device is present every
+ *     compute entry point returns SPH_ERR_DEVICE.
+ */
+#ifndef SPH_CAPI_H
+#define SPH_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_INVALID  (-1) /* bad argument or handle                     */
+#define SPH_ERR_HIP      (-2) /* a HIP runtime call failed                  */
+#define SPH_ERR_OOM      (-3) /* device allocation failed                   */
+#define SPH_ERR_STATE    (-4) /* call not valid in the handle's state       */
+#define SPH_ERR_DEVICE   (-5) /* no usable gfx950 device                    */
+#define SPH_ERR_NOTFOUND (-6) /* unknown field name                         */
+
+#define SPH_F32 0
+#define SPH_F64 1
+
+/* Reference enum values (Simulation.h BoundaryHandlingMethods,
+ * NonPressureForceRegistration.cpp:52-58). */
+#define SPH_BOUNDARY_NONE       (-1)
+#define SPH_BOUNDARY_AKINCI2012  0
+#define SPH_BOUNDARY_BENDER2019  2
+#define SPH_VISCOSITY_NONE       0
+#define SPH_VISCOSITY_WEILER2018 6
+
+const char *sph_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int sph_version(void);
+/* Number of visible HIP devices (0 if none); does not fail without a GPU. */
+int sph_device_count(int *out);
+/* Waits for all work of this process on the current device. */
+int sph_device_synchronize(void);
+
+/* ------------------------------------------------------------------------ */
+/* Neighbourhood search (the CompactNSearch members the reference calls)    */
+/* ------------------------------------------------------------------------ */
+typedef struct sph_ns *sph_ns_t;
+
+/* NeighborhoodSearch(Real r, bool erase_empty_cells) -- Simulation.cpp:149 */
+int sph_ns_create(double radius, int erase_empty_cells, sph_ns_t *out);
+int sph_ns_destroy(sph_ns_t ns);
+/* set_radius(Real) -- Simulation.cpp:151 */
+int sph_ns_set_radius(sph_ns_t ns, double radius);
+int sph_ns_get_radius(sph_ns_t ns, double *out);
+/* reset() -- SimulatorBase.cpp:850: forget all point sets. */
+int sph_ns_reset(sph_ns_t ns);
+/* add_point_set(x, n, is_dynamic, search_neighbors, find_neighbors, user_data)
+ * -- FluidModel.cpp:323, BoundaryModel_Akinci2012.cpp:109. The library keeps
+ * the caller's pointer (as CompactNSearch does) and reads it on
+ * update_point_sets()/find_neighbors(points_changed=1). */
+int sph_ns_add_point_set(sph_ns_t ns, const void *x, size_t n, int dtype,
+                         int is_dynamic, int search_neighbors,
+                         int find_neighbors, void *user_data,
+                         unsigned *out_index);
+/* resize_point_set(i, x, n) -- FluidModel.cpp:216,737 */
+int sph_ns_resize_point_set(sph_ns_t ns, unsigned set, const void *x,
+                            size_t n);
+/* update_point_sets() -- host-to-device copy of every dynamic set. */
+int sph_ns_update_point_sets(sph_ns_t ns);
+/* find_neighbors(bool points_changed=true) -- Simulation.cpp:617 */
+int sph_ns_find_neighbors(sph_ns_t ns, int points_changed);
+/* find_neighbors(Real const* x, vector<vector<unsigned>>&) -- point query
+ * (Tools/FoamGenerator/main.cpp:1452). Writes the neighbours of x in `set`
+ * into out[0..cap) in ascending order and the total count to *count. */
+int sph_ns_query_point(sph_ns_t ns, const double x[3], unsigned set,
+                       unsigned *out, unsigned cap, unsigned *count);
+/* z_sort() -- Simulation.cpp:626: computes the Morton-order permutation of
+ * every dynamic set; sort_field then applies it. */
+int sph_ns_z_sort(sph_ns_t ns);
+/* PointSet::sort_field<T>(T*) -- permutes a caller array of n elements of
+ * elem_bytes bytes in place with the last z_sort() permutation. */
+int sph_ns_sort_field(sph_ns_t ns, unsigned set, void *data,
+                      size_t elem_bytes);
+/* set_active(bool) / set_active(i, search, find) / set_active(i, j, bool)
+ * -- Simulation.cpp:713-754 */
+int sph_ns_set_active_all(sph_ns_t ns, int active);
+int sph_ns_set_active_set(sph_ns_t ns, unsigned set, int search_neighbors,
+                          int find_neighbors);
+int sph_ns_set_active_pair(sph_ns_t ns, unsigned i, unsigned j, int active);
+int sph_ns_is_active_pair(sph_ns_t ns, unsigned i, unsigned j, int *out);
+/* n_point_sets(), point_set(i).n_points(), get_user_data() */
+int sph_ns_n_point_sets(sph_ns_t ns, unsigned *out);
+int sph_ns_n_points(sph_ns_t ns, unsigned set, size_t *out);
+int sph_ns_user_data(sph_ns_t ns, unsigned set, void **out);
+/* PointSet::n_neighbors(j, i), neighbor(j, i, k), neighbor_list(j, i)
+ * -- Simulation.h:456-473. Indices refer to the caller's point order. The
+ * list pointer stays valid until the next find_neighbors/z_sort/reset. */
+int sph_ns_n_neighbors(sph_ns_t ns, unsigned set_i, unsigned set_j,
+                       unsigned i, unsigned *out);
+int sph_ns_neighbor(sph_ns_t ns, unsigned set_i, unsigned set_j, unsigned i,
+                    unsigned k, unsigned *out);
+int sph_ns_neighbor_list(sph_ns_t ns, unsigned set_i, unsigned set_j,
+                         unsigned i, const unsigned **out, unsigned *count);
+/* Bulk CSR view of all lists of (set_i -> set_j): offsets has n_points+1
+ * entries. Same lifetime as sph_ns_neighbor_list. */
+int sph_ns_neighbor_csr(sph_ns_t ns, unsigned set_i, unsigned set_j,
+                        const unsigned **offsets, const unsigned **indices);
+
+/* ------------------------------------------------------------------------ */
+/* DFSPH time step                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct sph_dfsph *sph_dfsph_t;
+
+typedef struct sph_dfsph_params {
+    double particle_radius;     /* Simulation particleRadius; h = 4r        */
+    double density0;            /* FluidModel density0                      */
+    double gravity[3];          /* Simulation gravitation                   */
+    double time_step_size;      /* TimeManager initial h                    */
+    int    cfl_method;          /* 0 off, 1 CFL, 2 CFL + iteration rule     */
+    double cfl_factor;
+    double cfl_min_time_step_size;
+    double cfl_max_time_step_size;
+    int    min_iterations;      /* DFSPH.minIterations                      */
+    int    max_iterations;      /* DFSPH.maxIterations                      */
+    double max_error;           /* DFSPH.maxError, percent                  */
+    int    max_iterations_v;    /* DFSPH.maxIterationsV                     */
+    double max_error_v;         /* DFSPH.maxErrorV, percent                 */
+    int    enable_divergence_solver;
+    double xsph;                /* XSPH.xsph                                */
+    double xsph_boundary;       /* XSPH.xsphBoundary                        */
+    int    viscosity_method;    /* SPH_VISCOSITY_*                          */
+    double viscosity;           /* Weiler2018 viscosity (kinematic)         */
+    double viscosity_boundary;  /* Weiler2018 viscosityBoundary             */
+    int    visco_max_iter;
+    double visco_max_error;
+    int    boundary_handling_method; /* SPH_BOUNDARY_*                      */
+    int    reserved[8];
+} sph_dfsph_params_t;
+
+/* Defaults of the reference constructors (Simulation.cpp:66-89,
+ * TimeStepDFSPH.cpp:29-41, TimeManager.cpp:12, XSPH.cpp:20,
+ * Viscosity_Weiler2018.cpp:21-29). */
+void sph_dfsph_default_params(sph_dfsph_params_t *p);
+
+typedef struct sph_dfsph_counters {
+    double time;             /* TimeManager time after the last step       */
+    double time_step_size;   /* TimeManager h after the last step          */
+    double avg_density_err;  /* last pressure iteration, kg/m^3            */
+    double avg_divergence_err; /* last divergence iteration                */
+    int    iterations;       /* "DFSPH - iterations" (m_iterations)         */
+    int    iterations_v;     /* "DFSPH - iterationsV" (m_iterationsV)       */
+    int    visco_iterations; /* "Visco iterations" (Eigen CG iters)         */
+    int    steps;            /* steps taken since creation                  */
+    long long total_iterations;
+    long long total_iterations_v;
+    long long total_visco_iterations;
+    int    max_fluid_neighbors;
+    int    max_boundary_neighbors;
+    int    flags;            /* bit0: NaN/Inf seen in a reduction           */
+    int    reserved[7];
+} sph_dfsph_counters_t;
+
+/* Sub-steps of TimeStepDFSPH::step() (TimeStepDFSPH.cpp:117-249), callable
+ * one at a time so a caller can check each against its own reference. */
+enum sph_dfsph_stage {
+    SPH_STAGE_NEIGHBORHOOD_SEARCH = 0, /* Simulation::performNeighborhoodSearch */
+    SPH_STAGE_VOLUME_MAPS        = 1, /* TimeStep::computeVolumeAndBoundaryX  */
+    SPH_STAGE_DENSITIES          = 2, /* TimeStep::computeDensities           */
+    SPH_STAGE_DFSPH_FACTOR       = 3, /* computeDFSPHFactor                   */
+    SPH_STAGE_DIVERGENCE_SETUP   = 4, /* divergenceSolve set-up loop          */
+    SPH_STAGE_DIVERGENCE_ITER    = 5, /* one divergenceSolveIteration         */
+    SPH_STAGE_DIVERGENCE_FINISH  = 6, /* divergenceSolve finalisation         */
+    SPH_STAGE_DIVERGENCE_SOLVE   = 7, /* whole divergenceSolve                */
+    SPH_STAGE_CLEAR_ACCEL        = 8, /* TimeStep::clearAccelerations         */
+    SPH_STAGE_VISCOSITY          = 9, /* Viscosity_Weiler2018::step           */
+    SPH_STAGE_XSPH               = 10,/* XSPH::step                           */
+    SPH_STAGE_CFL                = 11,/* Simulation::updateTimeStepSize       */
+    SPH_STAGE_UPDATE_VELOCITY    = 12,/* v += h_old a                         */
+    SPH_STAGE_PRESSURE_SETUP     = 13,/* pressureSolve set-up loop            */
+    SPH_STAGE_PRESSURE_ITER      = 14,/* one pressureSolveIteration           */
+    SPH_STAGE_PRESSURE_FINISH    = 15,/* pressureSolve finalisation           */
+    SPH_STAGE_PRESSURE_SOLVE     = 16,/* whole pressureSolve                  */
+    SPH_STAGE_UPDATE_POSITION    = 17,/* x += h_old v and t += h_old          */
+    SPH_STAGE_COUNT              = 18
+};
+
+int sph_dfsph_create(const sph_dfsph_params_t *params, sph_dfsph_t *out);
+int sph_dfsph_destroy(sph_dfsph_t s);
+/* FluidModel::initModel (FluidModel.cpp:285-327): n particles, positions and
+ * velocities as 3n floats. Particle i gets id i. */
+int sph_dfsph_set_fluid(sph_dfsph_t s, const float *x, const float *v,
+                        size_t n);
+/* BoundaryModel_Akinci2012::initModel (BoundaryModel_Akinci2012.cpp:77-110)
+ * for a static body: n boundary particles, 3n floats. */
+int sph_dfsph_add_boundary_particles(sph_dfsph_t s, const float *x, size_t n);
+/* BoundaryModel_Bender2019 volume map of an axis-aligned box wall
+ * (SimulatorBase::initVolumeMap, SimulatorBase.cpp:2566-2704): the signed
+ * distance of the box surface (inverted when `invert`, i.e. fluid inside),
+ * minus `thickness`, discretised on resolution[0..2] cubic cells over the
+ * box grown by 8h + thickness, and the volume field integrated from it. */
+int sph_dfsph_set_volume_map_box(sph_dfsph_t s, const double box_min[3],
+                                 const double box_max[3], int invert,
+                                 double thickness,
+                                 const unsigned resolution[3]);
+/* Deferred initialisation (SimulatorBase::deferredInit,
+ * StaticBoundarySimulator.cpp:181-199): Akinci boundary volumes
+ * (Simulation::updateBoundaryVolume) and device buffers. Called by the first
+ * step if not called before. */
+int sph_dfsph_init(sph_dfsph_t s);
+/* n_steps calls of TimeStepDFSPH::step(). */
+int sph_dfsph_step(sph_dfsph_t s, int n_steps);
+/* One sub-step, see enum sph_dfsph_stage. */
+int sph_dfsph_stage(sph_dfsph_t s, int stage);
+int sph_dfsph_num_particles(sph_dfsph_t s, size_t *out);
+/* Field access by the reference's field names (TimeStepDFSPH.cpp:49-53,
+ * FluidModel fields, Viscosity_Weiler2018.cpp:33): "position", "velocity",
+ * "acceleration", "density", "factor", "advected density", "p / rho^2",
+ * "p_v / rho^2", "pressure acceleration", "velocity difference", "id",
+ * "boundary volume", "boundary x", "boundary particle volume",
+ * "fluid neighbor count", "boundary neighbor count". Values are float32
+ * (uint32 for ids and counts) in particle-id order; bytes must equal
+ * n * components * 4. */
+int sph_dfsph_get_field(sph_dfsph_t s, const char *name, void *host,
+                        size_t bytes);
+int sph_dfsph_set_field(sph_dfsph_t s, const char *name, const void *host,
+                        size_t bytes);
+/* Neighbour lists of the last search, particle-id indexed CSR (fluid->fluid
+ * when boundary == 0, fluid->boundary particles when boundary == 1).
+ * offsets: n+1 entries; indices: capacity entries; *nnz receives the total. */
+int sph_dfsph_get_neighbors(sph_dfsph_t s, int boundary, unsigned *offsets,
+                            unsigned *indices, size_t capacity, size_t *nnz);
+int sph_dfsph_get_counters(sph_dfsph_t s, sph_dfsph_counters_t *out);
+int sph_dfsph_set_time(sph_dfsph_t s, double time, double time_step_size);
+/* Per-kernel GPU timing with HIP events on the handle's stream. When
+ * enabled, sph_dfsph_kernel_stats reports, for kernel k < *count, its name,
+ * launch count and summed milliseconds. */
+int sph_dfsph_enable_timing(sph_dfsph_t s, int enable);
+int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name,
+                           long long *launches, double *total_ms,
+                           int *count);
+int sph_dfsph_reset_kernel_stats(sph_dfsph_t s);
+int sph_dfsph_synchronize(sph_dfsph_t s);
+
+/* Cubic-spline table kernel (PrecomputedKernel<CubicKernel,10000>,
+ * SPHKernels.h:614-691) evaluated on the device for n FP32 offsets r (3n
+ * floats): W into w[n], gradW into g[3n]. For kernel checks. */
+int sph_kernel_eval(double support_radius, const float *r, size_t n,
+                    float *w, float *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_CAPI_H */

@@ -1,0 +1,265 @@
+"""ctypes binding of the FP64 oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE: the oracle is the checker for the HIP path. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from splishsplash_amd.params import DFSPHCounters, DFSPHParams, STAGES, default_params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_up = C.POINTER(C.c_uint)
+_ip = C.POINTER(C.c_int)
+_szp = C.POINTER(C.c_size_t)
+
+
+def build():
+    """Compile the oracle (g++, OpenMP)."""
+    subprocess.run(["make", "-C", HERE, "-B", "build/liboracle.so"], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sigs = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_create": (C.c_int, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
+            "orc_destroy": (C.c_int, [C.c_void_p]),
+            "orc_set_fluid": (C.c_int, [C.c_void_p, _dp, _dp, C.c_size_t]),
+            "orc_add_boundary_particles": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
+            "orc_set_volume_map_box": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _up]),
+            "orc_init": (C.c_int, [C.c_void_p]),
+            "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
+            "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
+            "orc_num_particles": (C.c_int, [C.c_void_p, _szp]),
+            "orc_get_field": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.c_size_t]),
+            "orc_set_field": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.c_size_t]),
+            "orc_get_counters": (C.c_int, [C.c_void_p, C.POINTER(DFSPHCounters)]),
+            "orc_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+            "orc_get_neighbors": (C.c_int, [C.c_void_p, C.c_int, _up, _up, C.c_size_t, _szp]),
+            "orc_ns_f32": (C.c_int, [_fp, C.c_size_t, _fp, C.c_size_t, C.c_float, C.c_int, C.c_int,
+                                     _up, _up, C.c_size_t, _szp]),
+            "orc_ns_f64": (C.c_int, [_dp, C.c_size_t, _dp, C.c_size_t, C.c_double, C.c_int, C.c_int,
+                                     _up, _up, C.c_size_t, _szp]),
+            "orc_kernel_table": (C.c_int, [C.c_double, _dp, _dp, _dp]),
+            "orc_kernel_eval": (C.c_int, [C.c_double, C.c_int, _dp, C.c_size_t, _dp, _dp]),
+            "orc_gauss_legendre": (C.c_int, [C.c_int, _dp, _dp]),
+            "orc_serendipity": (C.c_int, [_dp, _dp, _dp]),
+            "orc_serendipity_node": (C.c_int, [C.c_int, _dp]),
+            "orc_box_sdf": (C.c_int, [_dp, _dp, C.c_int, C.c_double, _dp, C.c_size_t, _dp]),
+            "orc_volume_map_eval": (C.c_int, [_dp, _dp, C.c_int, C.c_double, _up, C.c_double, C.c_int,
+                                              C.c_int, _dp, C.c_size_t, _dp, _dp, _ip]),
+            "orc_volume_map_domain": (C.c_int, [_dp, _dp, C.c_double, _up, C.c_double, _dp, _dp, _dp]),
+            "orc_cg_dense": (C.c_int, [_dp, C.c_size_t, _dp, _dp, C.c_int, C.c_double, _ip, _dp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def neighbors(xq, xs=None, radius=0.1, brute=False, dtype=np.float32):
+    """CSR (offsets, indices) of the neighbours of each xq point among xs
+    (same set with self excluded when xs is None)."""
+    same = xs is None
+    xq = np.ascontiguousarray(xq, dtype=dtype).reshape(-1, 3)
+    xs = xq if same else np.ascontiguousarray(xs, dtype=dtype).reshape(-1, 3)
+    nq, ns = len(xq), len(xs)
+    off = np.zeros(nq + 1, np.uint32)
+    nnz = C.c_size_t(0)
+    f, pt, rt = (lib().orc_ns_f32, _fp, C.c_float) if dtype == np.float32 else (lib().orc_ns_f64, _dp, C.c_double)
+    _chk(f(_ptr(xq, pt), nq, _ptr(xs, pt), ns, rt(radius), int(same), int(brute),
+           _ptr(off, _up), None, 0, C.byref(nnz)))
+    idx = np.zeros(max(nnz.value, 1), np.uint32)
+    _chk(f(_ptr(xq, pt), nq, _ptr(xs, pt), ns, rt(radius), int(same), int(brute),
+           _ptr(off, _up), _ptr(idx, _up), idx.size, C.byref(nnz)))
+    return off, idx[: nnz.value]
+
+
+def kernel_table(h):
+    T = np.zeros(10000)
+    G = np.zeros(10001)
+    s = np.zeros(4)
+    _chk(lib().orc_kernel_table(h, _ptr(T, _dp), _ptr(G, _dp), _ptr(s, _dp)))
+    return T, G, dict(invStep=s[0], W_zero=s[1], k=s[2], l=s[3])
+
+
+def kernel_eval(h, r, precomputed=True):
+    r = _d(r).reshape(-1, 3)
+    n = len(r)
+    w = np.zeros(n)
+    g = np.zeros((n, 3))
+    _chk(lib().orc_kernel_eval(h, int(precomputed), _ptr(r, _dp), n, _ptr(w, _dp), _ptr(g, _dp)))
+    return w, g
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _chk(lib().orc_gauss_legendre(n, _ptr(x, _dp), _ptr(w, _dp)))
+    return x, w
+
+
+def serendipity(xi):
+    xi = _d(xi)
+    N = np.zeros(32)
+    dN = np.zeros((32, 3))
+    _chk(lib().orc_serendipity(_ptr(xi, _dp), _ptr(N, _dp), _ptr(dN, _dp)))
+    return N, dN
+
+
+def serendipity_nodes():
+    out = np.zeros((32, 3))
+    for k in range(32):
+        p = np.zeros(3)
+        _chk(lib().orc_serendipity_node(k, _ptr(p, _dp)))
+        out[k] = p
+    return out
+
+
+def box_sdf(lo, hi, x, invert=True, thickness=0.0):
+    x = _d(x).reshape(-1, 3)
+    out = np.zeros(len(x))
+    _chk(lib().orc_box_sdf(_ptr(_d(lo), _dp), _ptr(_d(hi), _dp), int(invert), thickness,
+                           _ptr(x, _dp), len(x), _ptr(out, _dp)))
+    return out
+
+
+def volume_map_eval(lo, hi, res, support_radius, x, field, interpolate, invert=True, thickness=0.0):
+    """Node values (interpolate=False) or interpolated values and gradients
+    of the box volume map at points x."""
+    x = _d(x).reshape(-1, 3)
+    n = len(x)
+    val = np.zeros(n)
+    grad = np.zeros((n, 3))
+    ok = np.zeros(n, np.int32)
+    r = np.ascontiguousarray(res, dtype=np.uint32)
+    _chk(lib().orc_volume_map_eval(_ptr(_d(lo), _dp), _ptr(_d(hi), _dp), int(invert), thickness,
+                                   _ptr(r, _up), support_radius, field, int(interpolate),
+                                   _ptr(x, _dp), n, _ptr(val, _dp), _ptr(grad, _dp), _ptr(ok, _ip)))
+    return val, grad, ok.astype(bool)
+
+
+def volume_map_domain(lo, hi, res, support_radius, thickness=0.0):
+    a, b, c = np.zeros(3), np.zeros(3), np.zeros(3)
+    r = np.ascontiguousarray(res, dtype=np.uint32)
+    _chk(lib().orc_volume_map_domain(_ptr(_d(lo), _dp), _ptr(_d(hi), _dp), thickness, _ptr(r, _up),
+                                     support_radius, _ptr(a, _dp), _ptr(b, _dp), _ptr(c, _dp)))
+    return a, b, c
+
+
+def cg_dense(A, b, x0, max_iter, tol):
+    A = _d(A)
+    n = len(b)
+    x = _d(x0).copy()
+    it = C.c_int(0)
+    err = C.c_double(0)
+    _chk(lib().orc_cg_dense(_ptr(A, _dp), n, _ptr(_d(b), _dp), _ptr(x, _dp), max_iter, tol,
+                            C.byref(it), C.byref(err)))
+    return x, it.value, err.value
+
+
+class OracleDFSPH:
+    """FP64 TimeStepDFSPH restatement (mirrors splishsplash_amd.DFSPH)."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        _chk(lib().orc_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_fluid(self, x, v=None):
+        x = _d(x).reshape(-1, 3)
+        v = np.zeros_like(x) if v is None else _d(v).reshape(-1, 3)
+        self.n = len(x)
+        _chk(lib().orc_set_fluid(self.h, _ptr(x, _dp), _ptr(v, _dp), self.n))
+
+    def add_boundary_particles(self, x):
+        x = _d(x).reshape(-1, 3)
+        _chk(lib().orc_add_boundary_particles(self.h, _ptr(x, _dp), len(x)))
+
+    def set_volume_map_box(self, lo, hi, res, invert=True, thickness=0.0):
+        r = np.ascontiguousarray(res, dtype=np.uint32)
+        _chk(lib().orc_set_volume_map_box(self.h, _ptr(_d(lo), _dp), _ptr(_d(hi), _dp), int(invert),
+                                          thickness, _ptr(r, _up)))
+
+    def init(self):
+        _chk(lib().orc_init(self.h))
+
+    def step(self, n=1):
+        _chk(lib().orc_step(self.h, n))
+
+    def stage(self, name):
+        _chk(lib().orc_stage(self.h, STAGES[name] if isinstance(name, str) else int(name)))
+
+    def get(self, name, comps=None):
+        if comps is None:
+            from splishsplash_amd.params import FIELD_COMPONENTS
+            comps = FIELD_COMPONENTS.get(name, 1)
+        count = self.n if name != "boundary particle volume" else None
+        if count is None:
+            raise ValueError("use get_boundary_volumes")
+        out = np.zeros(self.n * comps)
+        _chk(lib().orc_get_field(self.h, name.encode(), _ptr(out, _dp), out.size))
+        return out.reshape(self.n, comps) if comps > 1 else out
+
+    def get_boundary_volumes(self, nb):
+        out = np.zeros(nb)
+        _chk(lib().orc_get_field(self.h, b"boundary particle volume", _ptr(out, _dp), nb))
+        return out
+
+    def set(self, name, values):
+        a = _d(values).ravel()
+        _chk(lib().orc_set_field(self.h, name.encode(), _ptr(a, _dp), a.size))
+
+    def counters(self):
+        c = DFSPHCounters()
+        _chk(lib().orc_get_counters(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def set_time(self, t, dt):
+        _chk(lib().orc_set_time(self.h, t, dt))
+
+    def neighbors(self, boundary=False):
+        off = np.zeros(self.n + 1, np.uint32)
+        nnz = C.c_size_t(0)
+        _chk(lib().orc_get_neighbors(self.h, int(boundary), _ptr(off, _up), None, 0, C.byref(nnz)))
+        idx = np.zeros(max(nnz.value, 1), np.uint32)
+        _chk(lib().orc_get_neighbors(self.h, int(boundary), _ptr(off, _up), _ptr(idx, _up), idx.size,
+                                     C.byref(nnz)))
+        return off, idx[: nnz.value]

@@ -1,0 +1,1589 @@
+// sph_oracle.cpp -- TEST INFRASTRUCTURE, not part of the product.
+//
+// FP64 CPU restatement of SPlisHSPlasH v2.16.0's scalar DFSPH path (the
+// branches compiled with USE_DOUBLE, where USE_AVX is forced off,
+// CMake/Common.cmake:20). The HIP path in splishsplash_amd/ is checked
+// against it; nothing in the product links it. Built with -ffp-contract=off
+// so every expression rounds as written.
+//
+// Parity status per function (how each is pinned is in tests/):
+//   Cubic / PreCubic        pinned: golden table from the compiled reference
+//                           SPHKernels.cpp + closed forms (KernelTests.cpp).
+//   ns_grid / ns_brute      pinned: brute force, symmetry, lattice counts.
+//                           Inclusion rule of CompactNSearch itself: parity
+//                           unpinned (library absent from the reference tree).
+//   gauss_legendre          pinned: golden abscissae/weights of
+//                           GaussQuadrature.cpp (compiled probe).
+//   serendipity             pinned: interpolation, partition of unity, cubic
+//                           reproduction; Discregrid itself absent: parity to
+//                           its node numbering unpinned.
+//   VolumeMap               pinned: analytic half-space volume integral.
+//   conjugate_gradient      pinned: Eigen CG + BlockJacobiPreconditioner3D
+//                           compiled probe on a fixed SPD system.
+//   Sim stages              pinned: closed forms (lattice density, zero
+//                           gradient sums), dense Jacobi, momentum balance.
+#include "sph_oracle.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+namespace orc {
+
+static const double PI = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ kernels
+// SPHKernels.h:24-34
+void Cubic::set(double h) {
+    radius = h;
+    const double h3 = radius * radius * radius;
+    k = 8.0 / (PI * h3);
+    l = 48.0 / (PI * h3);
+    W0 = W(0.0);
+}
+
+// SPHKernels.h:36-54
+double Cubic::W(double r) const {
+    double res = 0.0;
+    const double q = r / radius;
+    if (q <= 1.0) {
+        if (q <= 0.5) {
+            const double q2 = q * q;
+            const double q3 = q2 * q;
+            res = k * (6.0 * q3 - 6.0 * q2 + 1.0);
+        } else {
+            res = k * (2.0 * std::pow(1.0 - q, 3.0));
+        }
+    }
+    return res;
+}
+
+// SPHKernels.h:61-85
+void Cubic::gradW(const double *r, double *out) const {
+    const double rl = std::sqrt(sqn3(r));
+    const double q = rl / radius;
+    if ((rl > 1.0e-9) && (q <= 1.0)) {
+        double gq[3];
+        for (int c = 0; c < 3; c++) gq[c] = (r[c] / rl) / radius;
+        double s;
+        if (q <= 0.5) {
+            s = l * q * (3.0 * q - 2.0);
+        } else {
+            const double f = 1.0 - q;
+            s = l * (-f * f);
+        }
+        for (int c = 0; c < 3; c++) out[c] = s * gq[c];
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+    }
+}
+
+// SPHKernels.h:626-647
+void PreCubic::set(double h) {
+    radius = h;
+    radius2 = radius * radius;
+    cubic.set(h);
+    const double stepSize = radius / (double)(RES - 1);
+    invStep = 1.0 / stepSize;
+    T.assign(RES, 0.0);
+    G.assign(RES + 1, 0.0);
+    for (unsigned i = 0; i < RES; i++) {
+        const double posX = stepSize * (double)i;
+        T[i] = cubic.W(posX);
+        if (posX > 1.0e-9) {
+            double rv[3] = {posX, 0.0, 0.0}, g[3];
+            cubic.gradW(rv, g);
+            G[i] = g[0] / posX;
+        } else {
+            G[i] = 0.0;
+        }
+    }
+    G[RES] = 0.0;
+    W0 = cubic.W(0.0);
+}
+
+// SPHKernels.h:650-661
+double PreCubic::W(const double *r) const {
+    double res = 0.0;
+    const double r2 = sqn3(r);
+    if (r2 <= radius2) {
+        const double rl = std::sqrt(r2);
+        const unsigned pos = std::min<unsigned>((unsigned)(rl * invStep), RES - 2u);
+        res = 0.5 * (T[pos] + T[pos + 1]);
+    }
+    return res;
+}
+
+// SPHKernels.h:673-685
+void PreCubic::gradW(const double *r, double *out) const {
+    const double rl = std::sqrt(sqn3(r));
+    if (rl <= radius) {
+        const unsigned pos = std::min<unsigned>((unsigned)(rl * invStep), RES - 2u);
+        const double g = 0.5 * (G[pos] + G[pos + 1]);
+        for (int c = 0; c < 3; c++) out[c] = g * r[c];
+    } else {
+        out[0] = out[1] = out[2] = 0.0;
+    }
+}
+
+// ------------------------------------------------------- neighbour search
+namespace {
+inline uint64_t cell_key(int64_t cx, int64_t cy, int64_t cz) {
+    const int64_t B = 1 << 20;
+    return ((uint64_t)(cx + B) << 42) | ((uint64_t)(cy + B) << 21) | (uint64_t)(cz + B);
+}
+template <class R> inline int64_t cell_of(R x, R inv) { return (int64_t)std::floor((double)x * (double)inv); }
+template <class R> inline bool inside(const R *a, const R *b, R r2) {
+    const R d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    return sqn3(d) < r2;
+}
+}  // namespace
+
+template <class R>
+void ns_grid(const R *xq, size_t nq, const R *xs, size_t ns, R radius, bool same_set, CSR &out) {
+    const R r2 = radius * radius;
+    // Cell size = radius, as CompactNSearch's grid (cell edge = search radius).
+    const double inv = 1.0 / (double)radius;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    grid.reserve(ns / 4 + 16);
+    for (size_t j = 0; j < ns; j++) {
+        const int64_t c[3] = {cell_of(xs[3 * j], (R)inv), cell_of(xs[3 * j + 1], (R)inv), cell_of(xs[3 * j + 2], (R)inv)};
+        grid[cell_key(c[0], c[1], c[2])].push_back((uint32_t)j);
+    }
+    std::vector<std::vector<uint32_t>> lists(nq);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nq; i++) {
+        const R *xi = xq + 3 * i;
+        const int64_t c[3] = {cell_of(xi[0], (R)inv), cell_of(xi[1], (R)inv), cell_of(xi[2], (R)inv)};
+        auto &L = lists[i];
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    auto it = grid.find(cell_key(c[0] + dx, c[1] + dy, c[2] + dz));
+                    if (it == grid.end()) continue;
+                    for (uint32_t j : it->second) {
+                        if (same_set && j == (uint32_t)i) continue;
+                        if (inside(xi, xs + 3 * (size_t)j, r2)) L.push_back(j);
+                    }
+                }
+        std::sort(L.begin(), L.end());
+    }
+    out.off.assign(nq + 1, 0);
+    for (size_t i = 0; i < nq; i++) out.off[i + 1] = out.off[i] + (uint32_t)lists[i].size();
+    out.idx.resize(out.off[nq]);
+    for (size_t i = 0; i < nq; i++) std::copy(lists[i].begin(), lists[i].end(), out.idx.begin() + out.off[i]);
+}
+
+template <class R>
+void ns_brute(const R *xq, size_t nq, const R *xs, size_t ns, R radius, bool same_set, CSR &out) {
+    const R r2 = radius * radius;
+    std::vector<std::vector<uint32_t>> lists(nq);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nq; i++)
+        for (size_t j = 0; j < ns; j++) {
+            if (same_set && j == (size_t)i) continue;
+            if (inside(xq + 3 * i, xs + 3 * j, r2)) lists[i].push_back((uint32_t)j);
+        }
+    out.off.assign(nq + 1, 0);
+    for (size_t i = 0; i < nq; i++) out.off[i + 1] = out.off[i] + (uint32_t)lists[i].size();
+    out.idx.resize(out.off[nq]);
+    for (size_t i = 0; i < nq; i++) std::copy(lists[i].begin(), lists[i].end(), out.idx.begin() + out.off[i]);
+}
+
+template void ns_grid<float>(const float *, size_t, const float *, size_t, float, bool, CSR &);
+template void ns_grid<double>(const double *, size_t, const double *, size_t, double, bool, CSR &);
+template void ns_brute<float>(const float *, size_t, const float *, size_t, float, bool, CSR &);
+template void ns_brute<double>(const double *, size_t, const double *, size_t, double, bool, CSR &);
+
+// ------------------------------------------------------ Gauss-Legendre
+void gauss_legendre(int n, double *x, double *w) {
+    // Extended precision so the rounded abscissae and weights are exact to
+    // the last bit of a double for n <= 64.
+    typedef long double LD;
+    const int m = (n + 1) / 2;
+    const LD pi = 3.141592653589793238462643383279502884L;
+    for (int i = 1; i <= m; i++) {
+        LD z = std::cos(pi * (i - 0.25L) / (n + 0.5L)), pp = 0;
+        for (int it = 0; it < 100; it++) {
+            LD p1 = 1.0L, p2 = 0.0L;
+            for (int j = 1; j <= n; j++) {
+                const LD p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0L);
+            const LD z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= 1e-19L) break;
+        }
+        LD p1 = 1.0L, p2 = 0.0L;
+        for (int j = 1; j <= n; j++) {  // derivative at the converged root
+            const LD p3 = p2;
+            p2 = p1;
+            p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0L);
+        x[i - 1] = (double)-z;
+        x[n - i] = (double)z;
+        w[i - 1] = w[n - i] = (double)(2.0L / ((1.0L - z) * (1.0L + z) * pp * pp));
+    }
+}
+
+// ------------------------------------------------------------ box SDF
+double BoxSDF::eval(const double *p) const {
+    bool in = true;
+    double dmin = std::numeric_limits<double>::max(), q2 = 0.0;
+    for (int c = 0; c < 3; c++) {
+        if (p[c] < lo[c] || p[c] > hi[c]) in = false;
+        dmin = std::min(dmin, std::min(p[c] - lo[c], hi[c] - p[c]));
+        const double q = std::max(std::max(lo[c] - p[c], 0.0), p[c] - hi[c]);
+        q2 += q * q;
+    }
+    const double dmesh = in ? -dmin : std::sqrt(q2);
+    const double sign = invert ? -1.0 : 1.0;
+    return sign * (dmesh - thickness);
+}
+
+// -------------------------------------------------- serendipity element
+void serendipity_node(int n, double *xi) {
+    if (n < 8) {
+        for (int c = 0; c < 3; c++) xi[c] = ((n >> c) & 1) ? 1.0 : -1.0;
+        return;
+    }
+    const int e = (n - 8) / 8;      // edge direction 0,1,2
+    const int k = (n - 8) % 8;      // edge (2 bits) and node t
+    const int t = k & 1, b = k >> 1;
+    const int o1 = (e + 1) % 3, o2 = (e + 2) % 3;
+    // the two fixed coordinates, in ascending axis order
+    const int a0 = std::min(o1, o2), a1 = std::max(o1, o2);
+    xi[e] = t ? 1.0 / 3.0 : -1.0 / 3.0;
+    xi[a0] = (b & 1) ? 1.0 : -1.0;
+    xi[a1] = (b & 2) ? 1.0 : -1.0;
+}
+
+void serendipity(const double *xi, double *N, double *dN) {
+    const double x = xi[0], y = xi[1], z = xi[2];
+    const double r2 = x * x + y * y + z * z;
+    for (int n = 0; n < 32; n++) {
+        double p[3];
+        serendipity_node(n, p);
+        if (n < 8) {
+            const double fx = 1.0 + x * p[0], fy = 1.0 + y * p[1], fz = 1.0 + z * p[2];
+            const double g = 9.0 * r2 - 19.0;
+            N[n] = (1.0 / 64.0) * fx * fy * fz * g;
+            if (dN) {
+                dN[3 * n + 0] = (1.0 / 64.0) * (p[0] * fy * fz * g + fx * fy * fz * 18.0 * x);
+                dN[3 * n + 1] = (1.0 / 64.0) * (fx * p[1] * fz * g + fx * fy * fz * 18.0 * y);
+                dN[3 * n + 2] = (1.0 / 64.0) * (fx * fy * p[2] * g + fx * fy * fz * 18.0 * z);
+            }
+        } else {
+            const int e = (n - 8) / 8;
+            const double s[3] = {x, y, z};
+            const int o1 = (e + 1) % 3, o2 = (e + 2) % 3;
+            const double u = s[e], fa = 1.0 - u * u, fb = 1.0 + 9.0 * u * p[e];
+            const double f1 = 1.0 + s[o1] * p[o1], f2 = 1.0 + s[o2] * p[o2];
+            N[n] = (9.0 / 64.0) * fa * fb * f1 * f2;
+            if (dN) {
+                double d[3];
+                d[e] = (9.0 / 64.0) * (-2.0 * u * fb + fa * 9.0 * p[e]) * f1 * f2;
+                d[o1] = (9.0 / 64.0) * fa * fb * p[o1] * f2;
+                d[o2] = (9.0 / 64.0) * fa * fb * f1 * p[o2];
+                for (int c = 0; c < 3; c++) dN[3 * n + c] = d[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- volume map
+// Domain = box grown by 8h + thickness (SimulatorBase.cpp:2596-2600); cell
+// size = domain diagonal / resolution.
+void VolumeMap::init(const BoxSDF &s, double h, const unsigned *r) {
+    sdf = s;
+    supportRadius = h;
+    cubic.set(h);
+    gauss_legendre(16, gx, gw);  // p = 30 -> 16 points (GaussQuadrature.cpp:102-103)
+    const double pad = 8.0 * h + s.thickness;
+    for (int c = 0; c < 3; c++) {
+        lo[c] = s.lo[c] - pad;
+        hi[c] = s.hi[c] + pad;
+        res[c] = r[c];
+        cs[c] = (hi[c] - lo[c]) / (double)res[c];
+        c0[c] = 2.0 / cs[c];
+    }
+    cache.clear();
+}
+
+bool VolumeMap::locate(const double *x, uint64_t &cell, double *xi) const {
+    for (int c = 0; c < 3; c++)
+        if (!(x[c] >= lo[c] && x[c] <= hi[c])) return false;
+    unsigned mi[3];
+    for (int c = 0; c < 3; c++) {
+        mi[c] = (unsigned)((x[c] - lo[c]) * (1.0 / cs[c]));
+        if (mi[c] >= res[c]) mi[c] = res[c] - 1;
+    }
+    cell = ((uint64_t)mi[2] * res[1] + mi[1]) * res[0] + mi[0];
+    for (int c = 0; c < 3; c++) {
+        const double smin = lo[c] + cs[c] * mi[c], smax = smin + cs[c];
+        const double denom = smax - smin;
+        xi[c] = (2.0 / denom) * x[c] - (smax + smin) / denom;
+    }
+    return true;
+}
+
+double VolumeMap::node_sdf(const double *x) const { return sdf.eval(x); }
+
+// volume_func of SimulatorBase.cpp:2627-2658 with the analytic box distance
+// in place of the interpolated field 0 inside the integrand.
+double VolumeMap::node_volume(const double *x) const {
+    const double h = supportRadius;
+    const double dist_x = sdf.eval(x);
+    if (dist_x > 2.0 * h) return 0.0;
+    double res = 0.0;
+    for (int i = 0; i < 16; i++) {
+        const double wi = gw[i];
+        for (int j = 0; j < 16; j++) {
+            const double wij = wi * gw[j];
+            for (int k = 0; k < 16; k++) {
+                const double wijk = wij * gw[k];
+                const double d[3] = {h * gx[i], h * gx[j], h * gx[k]};
+                double f = 0.0;
+                if (!(sqn3(d) > h * h)) {
+                    const double y[3] = {x[0] + d[0], x[1] + d[1], x[2] + d[2]};
+                    bool indom = true;
+                    for (int c = 0; c < 3; c++)
+                        if (!(y[c] >= lo[c] && y[c] <= hi[c])) indom = false;
+                    if (indom) {
+                        const double dist = sdf.eval(y);
+                        if (dist <= 0.0)
+                            f = 1.0;
+                        else if (dist < h)
+                            f = cubic.W(dist) / cubic.W0;
+                    }
+                }
+                res += wijk * f;
+            }
+        }
+    }
+    res *= h * h * h;
+    return 0.8 * res;
+}
+
+const std::vector<double> &VolumeMap::cell_values(uint64_t cell) {
+    auto it = cache.find(cell);
+    if (it != cache.end()) return it->second;
+    std::vector<double> v(64);
+    const unsigned i = (unsigned)(cell % res[0]), j = (unsigned)((cell / res[0]) % res[1]),
+                   k = (unsigned)(cell / ((uint64_t)res[0] * res[1]));
+    const double smin[3] = {lo[0] + cs[0] * i, lo[1] + cs[1] * j, lo[2] + cs[2] * k};
+    for (int n = 0; n < 32; n++) {
+        double p[3], y[3];
+        serendipity_node(n, p);
+        for (int c = 0; c < 3; c++) y[c] = smin[c] + 0.5 * (p[c] + 1.0) * cs[c];
+        v[n] = node_sdf(y);
+        v[32 + n] = node_volume(y);
+    }
+    return cache.emplace(cell, std::move(v)).first->second;
+}
+
+void VolumeMap::prefetch(const std::vector<uint64_t> &cells) {
+    std::vector<uint64_t> todo;
+    for (uint64_t c : cells)
+        if (!cache.count(c)) todo.push_back(c);
+    std::sort(todo.begin(), todo.end());
+    todo.erase(std::unique(todo.begin(), todo.end()), todo.end());
+    std::vector<std::vector<double>> vals(todo.size());
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long long t = 0; t < (long long)todo.size(); t++) {
+        const uint64_t cell = todo[t];
+        std::vector<double> v(64);
+        const unsigned i = (unsigned)(cell % res[0]), j = (unsigned)((cell / res[0]) % res[1]),
+                       k = (unsigned)(cell / ((uint64_t)res[0] * res[1]));
+        const double smin[3] = {lo[0] + cs[0] * i, lo[1] + cs[1] * j, lo[2] + cs[2] * k};
+        for (int n = 0; n < 32; n++) {
+            double p[3], y[3];
+            serendipity_node(n, p);
+            for (int c = 0; c < 3; c++) y[c] = smin[c] + 0.5 * (p[c] + 1.0) * cs[c];
+            v[n] = node_sdf(y);
+            v[32 + n] = node_volume(y);
+        }
+        vals[t] = std::move(v);
+    }
+    for (size_t t = 0; t < todo.size(); t++) cache.emplace(todo[t], std::move(vals[t]));
+}
+
+bool VolumeMap::interpolate(int field, const double *x, double *value, double *grad) {
+    uint64_t cell;
+    double xi[3];
+    if (!locate(x, cell, xi)) return false;
+    const std::vector<double> &cv = cell_values(cell);
+    double N[32], dN[96];
+    serendipity(xi, N, grad ? dN : nullptr);
+    double phi = 0.0, g[3] = {0, 0, 0};
+    for (int n = 0; n < 32; n++) {
+        const double c = cv[32 * field + n];
+        phi += c * N[n];
+        if (grad)
+            for (int d = 0; d < 3; d++) g[d] += c * dN[3 * n + d];
+    }
+    *value = phi;
+    if (grad)
+        for (int d = 0; d < 3; d++) grad[d] = g[d] * c0[d];
+    return true;
+}
+
+// ----------------------------------------------------------------- CG
+// 3x3 inverse as adjugate / determinant (the cofactor form Eigen uses for
+// fixed 3x3 matrices, Eigen/src/LU/InverseImpl.h).
+void inverse3(const double *m, double *inv) {
+    // m row-major
+    const double c00 = m[4] * m[8] - m[5] * m[7];
+    const double c01 = m[5] * m[6] - m[3] * m[8];
+    const double c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const double id = 1.0 / det;
+    inv[0] = c00 * id;
+    inv[3] = c01 * id;
+    inv[6] = c02 * id;
+    inv[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    inv[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    inv[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    inv[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    inv[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    inv[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+
+static double vdot(const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+    return s;
+}
+
+// ConjugateGradient.h:28-91
+template <class MatVec, class Precond>
+CGResult conjugate_gradient(size_t n, MatVec &&A, Precond &&M, const std::vector<double> &b,
+                            std::vector<double> &x, int maxIters, double tol) {
+    std::vector<double> residual(n), tmp(n), p(n), z(n);
+    A(x, tmp);
+    for (size_t i = 0; i < n; i++) residual[i] = b[i] - tmp[i];
+    const double rhsNorm2 = vdot(b, b);
+    if (rhsNorm2 == 0) {
+        std::fill(x.begin(), x.end(), 0.0);
+        return {0, 0.0};
+    }
+    const double considerAsZero = std::numeric_limits<double>::min();
+    const double threshold = std::max(tol * tol * rhsNorm2, considerAsZero);
+    double residualNorm2 = vdot(residual, residual);
+    if (residualNorm2 < threshold) return {0, std::sqrt(residualNorm2 / rhsNorm2)};
+    M(residual, p);
+    double absNew = vdot(residual, p);
+    int i = 0;
+    while (i < maxIters) {
+        A(p, tmp);
+        const double alpha = absNew / vdot(p, tmp);
+        for (size_t k = 0; k < n; k++) x[k] += alpha * p[k];
+        for (size_t k = 0; k < n; k++) residual[k] -= alpha * tmp[k];
+        residualNorm2 = vdot(residual, residual);
+        if (residualNorm2 < threshold) break;
+        M(residual, z);
+        const double absOld = absNew;
+        absNew = vdot(residual, z);
+        const double beta = absNew / absOld;
+        for (size_t k = 0; k < n; k++) p[k] = z[k] + beta * p[k];
+        i++;
+    }
+    return {i, std::sqrt(residualNorm2 / rhsNorm2)};
+}
+
+// ------------------------------------------------------------------ Sim
+Sim::Sim(const sph_dfsph_params_t &p) : P(p) {
+    r = P.particle_radius;
+    hs = 4.0 * r;                                 // Simulation.cpp:283
+    const double diam = 2.0 * r;
+    V = 0.8 * diam * diam * diam;                 // FluidModel.cpp:234-242
+    mass = V * P.density0;
+    K.set(hs);
+    dt = P.time_step_size;
+}
+
+void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
+    n = nn;
+    x.assign(xx, xx + 3 * n);
+    v.assign(vv, vv + 3 * n);
+    a.assign(3 * n, 0.0);
+    rho.assign(n, 0.0);
+    factor.assign(n, 0.0);
+    rho_adv.assign(n, 0.0);
+    p.assign(n, 0.0);
+    pv.assign(n, 0.0);
+    ap.assign(3 * n, 0.0);
+    vdiff.assign(3 * n, 0.0);
+    vmV.assign(n, 0.0);
+    vmX.assign(3 * n, 0.0);
+    initialized = false;
+}
+
+void Sim::add_boundary(const double *xx, size_t nn) {
+    bx.insert(bx.end(), xx, xx + 3 * nn);
+    nb += nn;
+    bV.assign(nb, 0.0);
+    initialized = false;
+}
+
+void Sim::set_volume_map(const BoxSDF &s, const unsigned *res) {
+    vm.reset(new VolumeMap());
+    vm->init(s, hs, res);
+}
+
+// Simulation::updateBoundaryVolume + BoundaryModel_Akinci2012::computeBoundaryVolume
+// (Simulation.cpp:696-756, BoundaryModel_Akinci2012.cpp:48-75).
+void Sim::init() {
+    if (nb > 0) {
+        CSR bb;
+        ns_grid<double>(bx.data(), nb, bx.data(), nb, hs, true, bb);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)nb; i++) {
+            double delta = K.W0;
+            for (uint32_t k = bb.off[i]; k < bb.off[i + 1]; k++) {
+                const uint32_t j = bb.idx[k];
+                const double d[3] = {bx[3 * i] - bx[3 * j], bx[3 * i + 1] - bx[3 * j + 1], bx[3 * i + 2] - bx[3 * j + 2]};
+                delta += K.W(d);
+            }
+            bV[i] = 1.0 / delta;
+        }
+    }
+    initialized = true;
+}
+
+// Simulation::performNeighborhoodSearch (Simulation.cpp:606-619); also caches
+// h at the start of TimeStepDFSPH::step (TimeStepDFSPH.cpp:121).
+void Sim::neighborhood_search() {
+    if (!initialized) init();
+    h_step = dt;
+    ns_grid<double>(x.data(), n, x.data(), n, hs, true, nf);
+    if (has_boundary_particles())
+        ns_grid<double>(x.data(), n, bx.data(), nb, hs, false, nbd);
+    else {
+        nbd.off.assign(n + 1, 0);
+        nbd.idx.clear();
+    }
+}
+
+// TimeStep::computeVolumeAndBoundaryX (TimeStep.cpp:177-346), static box.
+void Sim::volume_maps() {
+    if (!has_volume_map()) return;
+    std::vector<uint64_t> cells;
+    cells.reserve(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t c;
+        double xi[3];
+        if (vm->locate(&x[3 * i], c, xi)) cells.push_back(c);
+    }
+    vm->prefetch(cells);
+    const double supportRadius = hs, particleRadius = r;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        double *xi = &x[3 * i];
+        double *xb = &vmX[3 * i];
+        xb[0] = xb[1] = xb[2] = 0.0;
+        vmV[i] = 0.0;
+        double normal[3] = {0, 0, 0};
+        double dist = std::numeric_limits<double>::max();
+        double val;
+        const bool chk = vm->interpolate(0, xi, &val, normal);
+        if (chk) dist = val;
+        bool animate = false;
+        if ((dist > 0.0) && (dist < supportRadius)) {
+            double volume;
+            vm->interpolate(1, xi, &volume, nullptr);
+            if ((volume > 0.0) && (volume != std::numeric_limits<double>::max())) {
+                vmV[i] = volume;
+                const double nl = std::sqrt(sqn3(normal));
+                if (nl > 1.0e-9) {
+                    for (int c = 0; c < 3; c++) normal[c] /= nl;
+                    const double d = std::max(dist + 0.5 * particleRadius, 2.0 * particleRadius);
+                    for (int c = 0; c < 3; c++) xb[c] = xi[c] - d * normal[c];
+                } else {
+                    vmV[i] = 0.0;
+                }
+            } else {
+                vmV[i] = 0.0;
+            }
+        } else if (dist <= 0.0) {
+            animate = true;
+            vmV[i] = 0.0;
+        } else {
+            vmV[i] = 0.0;
+        }
+        if (animate) {
+            if (dist != std::numeric_limits<double>::max()) {
+                const double nl = std::sqrt(sqn3(normal));
+                if (nl > 1.0e-5) {
+                    for (int c = 0; c < 3; c++) normal[c] /= nl;
+                    double delta = 2.0 * particleRadius - dist;
+                    delta = std::min(delta, 0.1 * particleRadius);
+                    for (int c = 0; c < 3; c++) xi[c] = xi[c] + delta * normal[c];
+                    v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0;
+                }
+            }
+            vmV[i] = 0.0;
+        }
+    }
+}
+
+// TimeStep::computeDensities, scalar branch (TimeStep.cpp:116-169).
+void Sim::densities() {
+    const double density0 = P.density0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i];
+        double density = V * K.W0;
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const double *xj = &x[3 * (size_t)nf.idx[k]];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            density += V * K.W(d);
+        }
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double *xj = &bx[3 * (size_t)j];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                density += bV[j] * K.W(d);
+            }
+        } else if (has_volume_map()) {
+            const double Vj = vmV[i];
+            if (Vj > 0.0) {
+                const double *xj = &vmX[3 * i];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                density += Vj * K.W(d);
+            }
+        }
+        rho[i] = density * density0;
+    }
+}
+
+// computeDFSPHFactor, scalar branch (TimeStepDFSPH.cpp:1106-1186).
+void Sim::dfsph_factor() {
+    const double eps = 1.0e-5;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i];
+        double sum_grad_p_k = 0.0;
+        double gpi[3] = {0, 0, 0};
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const double *xj = &x[3 * (size_t)nf.idx[k]];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            K.gradW(d, g);
+            const double gpj[3] = {-V * g[0], -V * g[1], -V * g[2]};
+            sum_grad_p_k += sqn3(gpj);
+            for (int c = 0; c < 3; c++) gpi[c] -= gpj[c];
+        }
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double *xj = &bx[3 * (size_t)j];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) gpi[c] -= -bV[j] * g[c];
+            }
+        } else if (has_volume_map()) {
+            const double Vj = vmV[i];
+            if (Vj > 0.0) {
+                const double *xj = &vmX[3 * i];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) gpi[c] -= -Vj * g[c];
+            }
+        }
+        sum_grad_p_k += sqn3(gpi);
+        factor[i] = (sum_grad_p_k > eps) ? 1.0 / sum_grad_p_k : 0.0;
+    }
+}
+
+// Sum over the fluid and boundary neighbours of (v_i - v_j) . gradW, the
+// common body of computeDensityAdv / computeDensityChange
+// (TimeStepDFSPH.cpp:1191-1295). Boundary velocities are zero (static).
+static inline double div_sum(const Sim &S, size_t i) {
+    const double *xi = &S.x[3 * i], *vi = &S.v[3 * i];
+    double delta = 0.0;
+    for (uint32_t k = S.nf.off[i]; k < S.nf.off[i + 1]; k++) {
+        const size_t j = S.nf.idx[k];
+        const double *xj = &S.x[3 * j], *vj = &S.v[3 * j];
+        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        double g[3];
+        S.K.gradW(d, g);
+        const double dv[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+        delta += dot3(dv, g);
+    }
+    delta *= S.V;
+    if (S.has_boundary_particles()) {
+        for (uint32_t k = S.nbd.off[i]; k < S.nbd.off[i + 1]; k++) {
+            const uint32_t j = S.nbd.idx[k];
+            const double *xj = &S.bx[3 * (size_t)j];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            S.K.gradW(d, g);
+            delta += S.bV[j] * dot3(vi, g);
+        }
+    } else if (S.has_volume_map()) {
+        const double Vj = S.vmV[i];
+        if (Vj > 0.0) {
+            const double *xj = &S.vmX[3 * i];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            S.K.gradW(d, g);
+            delta += Vj * dot3(vi, g);
+        }
+    }
+    return delta;
+}
+
+static inline unsigned num_neighbors(const Sim &S, size_t i) {
+    return S.nf.count(i) + (S.has_boundary_particles() ? S.nbd.count(i) : 0u);
+}
+
+// divergenceSolve set-up loop (TimeStepDFSPH.cpp:398-463).
+void Sim::divergence_setup() {
+    const double h = dt, invH = 1.0 / h;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        rho_adv[i] = div_sum(*this, i);
+        double densityAdv = std::max(rho_adv[i], 0.0);
+        if (num_neighbors(*this, i) < 20) densityAdv = 0.0;
+        factor[i] *= invH;
+        if (densityAdv > 0.0)
+            pv[i] = 0.5 * std::min(pv[i], 0.5) * invH;
+        else
+            pv[i] = 0.0;
+    }
+}
+
+// computePressureAccel (TimeStepDFSPH.cpp:1299-1367).
+void Sim::pressure_accel(const std::vector<double> &prho2, size_t i, double *ai) const {
+    const double eps = 1.0e-5;
+    ai[0] = ai[1] = ai[2] = 0.0;
+    const double p_i = prho2[i];
+    const double *xi = &x[3 * i];
+    for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+        const size_t j = nf.idx[k];
+        const double pSum = p_i + 1.0 * prho2[j];
+        if (std::fabs(pSum) > eps) {
+            const double *xj = &x[3 * j];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            K.gradW(d, g);
+            for (int c = 0; c < 3; c++) ai[c] += pSum * (-V * g[c]);
+        }
+    }
+    if (std::fabs(p_i) > eps) {
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double *xj = &bx[3 * (size_t)j];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-bV[j] * g[c]);
+            }
+        } else if (has_volume_map()) {
+            const double Vj = vmV[i];
+            if (Vj > 0.0) {
+                const double *xj = &vmX[3 * i];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-Vj * g[c]);
+            }
+        }
+    }
+}
+
+// compute_aij_pj (TimeStepDFSPH.cpp:1370-1420).
+double Sim::aij_pj(size_t i) const {
+    const double *xi = &x[3 * i], *ai = &ap[3 * i];
+    double s = 0.0;
+    for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+        const size_t j = nf.idx[k];
+        const double *xj = &x[3 * j], *aj = &ap[3 * j];
+        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        double g[3];
+        K.gradW(d, g);
+        const double da[3] = {ai[0] - aj[0], ai[1] - aj[1], ai[2] - aj[2]};
+        s += dot3(da, g);
+    }
+    s *= V;
+    if (has_boundary_particles()) {
+        for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+            const uint32_t j = nbd.idx[k];
+            const double *xj = &bx[3 * (size_t)j];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            K.gradW(d, g);
+            s += bV[j] * dot3(ai, g);
+        }
+    } else if (has_volume_map()) {
+        const double Vj = vmV[i];
+        if (Vj > 0.0) {
+            const double *xj = &vmX[3 * i];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            K.gradW(d, g);
+            s += Vj * dot3(ai, g);
+        }
+    }
+    return s;
+}
+
+// divergenceSolveIteration (TimeStepDFSPH.cpp:621-706). Returns the average
+// error; per-particle terms are summed in index order.
+double Sim::divergence_iteration() {
+    const double h = dt, density0 = P.density0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) pressure_accel(pv, i, &ap[3 * i]);
+    std::vector<double> err(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        double aij = aij_pj(i);
+        aij *= h;
+        const double s_i = -rho_adv[i];
+        double residuum = std::min(s_i - aij, 0.0);
+        if (num_neighbors(*this, i) < 20) residuum = 0.0;
+        pv[i] = std::max(pv[i] - 0.5 * (s_i - aij) * factor[i], 0.0);
+        err[i] = -density0 * residuum;
+    }
+    double e = 0.0;
+    for (size_t i = 0; i < n; i++) e += err[i];
+    return n ? e / (double)n : 0.0;
+}
+
+// divergenceSolve finalisation (TimeStepDFSPH.cpp:500-540).
+void Sim::divergence_finish() {
+    const double h = dt;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        pressure_accel(pv, i, &ap[3 * i]);
+        for (int c = 0; c < 3; c++) v[3 * i + c] += h * ap[3 * i + c];
+        factor[i] *= h;
+    }
+    for (size_t i = 0; i < n; i++) pv[i] *= h;
+}
+
+// TimeStepDFSPH::divergenceSolve (TimeStepDFSPH.cpp:386-541).
+void Sim::divergence_solve() {
+    const double h = dt;
+    divergence_setup();
+    iterationsV = 0;
+    bool chk = false;
+    double avg = 0.0;
+    while ((!chk || (iterationsV < 1)) && (iterationsV < P.max_iterations_v)) {
+        avg = divergence_iteration();
+        const double eta = (1.0 / h) * P.max_error_v * 0.01 * P.density0;
+        chk = (avg <= eta);
+        iterationsV++;
+    }
+    lastErrV = avg;
+    totItV += iterationsV;
+    divergence_finish();
+}
+
+// TimeStep::clearAccelerations (TimeStep.cpp:35-50).
+void Sim::clear_accelerations() {
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) a[3 * i + c] = P.gravity[c];
+}
+
+// Orthonormal tangents, MathFunctions::getOrthogonalVectors (MathFunctions.cpp:234-247).
+static void orthogonal_vectors(const double *vec, double *x, double *y) {
+    double v[3] = {1, 0, 0};
+    if (std::fabs(dot3(v, vec)) > 0.999) {
+        v[0] = 0;
+        v[1] = 1;
+    }
+    x[0] = vec[1] * v[2] - vec[2] * v[1];
+    x[1] = vec[2] * v[0] - vec[0] * v[2];
+    x[2] = vec[0] * v[1] - vec[1] * v[0];
+    y[0] = vec[1] * x[2] - vec[2] * x[1];
+    y[1] = vec[2] * x[0] - vec[0] * x[2];
+    y[2] = vec[0] * x[1] - vec[1] * x[0];
+    const double nx = std::sqrt(sqn3(x)), ny = std::sqrt(sqn3(y));
+    for (int c = 0; c < 3; c++) {
+        x[c] /= nx;
+        y[c] /= ny;
+    }
+}
+
+// matrixVecProd, scalar branch (Viscosity_Weiler2018.cpp:271-441). Boundary
+// velocities are zero (static bodies), so the Bender2019 boundary term uses
+// v_i only.
+void Sim::weiler_matvec(const std::vector<double> &vec, std::vector<double> &result) const {
+    const double h = hs, h2 = h * h, dtl = dt;
+    const double density0 = P.density0, mu = P.viscosity * density0, mub = P.viscosity_boundary * density0;
+    const double d = 10.0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i];
+        double ai[3] = {0, 0, 0};
+        const double density_i = rho[i];
+        const double *vi = &vec[3 * i];
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double density_j = rho[j];
+            const double *xj = &x[3 * j];
+            const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double g[3];
+            K.gradW(xixj, g);
+            const double *vj = &vec[3 * j];
+            const double dv[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+            const double s = d * mu * (mass / density_j) * dot3(dv, xixj) / (sqn3(xixj) + 0.01 * h2);
+            for (int c = 0; c < 3; c++) ai[c] += s * g[c];
+        }
+        if (mub != 0.0) {
+            if (has_boundary_particles()) {
+                for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                    const uint32_t j = nbd.idx[k];
+                    const double *xj = &bx[3 * (size_t)j];
+                    const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double g[3];
+                    K.gradW(xixj, g);
+                    const double s = d * mub * (density0 * bV[j] / density_i) * dot3(vi, xixj) / (sqn3(xixj) + 0.01 * h2);
+                    for (int c = 0; c < 3; c++) ai[c] += s * g[c];
+                }
+            } else if (has_volume_map() && vmV[i] > 0.0) {
+                const double Vj = vmV[i];
+                const double *xj = &vmX[3 * i];
+                const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
+                const double nl = std::sqrt(sqn3(normal));
+                if (nl > 0.0001) {
+                    for (int c = 0; c < 3; c++) normal[c] /= nl;
+                    double t1[3], t2[3];
+                    orthogonal_vectors(normal, t1, t2);
+                    const double dist = 0.5 * h;  // m_tangentialDistanceFactor * h
+                    const double vol = 0.25 * Vj;
+                    for (int sidx = 0; sidx < 4; sidx++) {
+                        const double *t = (sidx < 2) ? t1 : t2;
+                        const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
+                        double xs[3], xix[3];
+                        for (int c = 0; c < 3; c++) {
+                            xs[c] = xj[c] + sg * t[c] * dist;
+                            xix[c] = xi[c] - xs[c];
+                        }
+                        double g[3];
+                        K.gradW(xix, g);
+                        const double s = d * mub * vol * dot3(vi, xix) / (sqn3(xix) + 0.01 * h2);
+                        for (int c = 0; c < 3; c++) ai[c] += s * g[c];
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < 3; c++) result[3 * i + c] = vec[3 * i + c] - dtl / density_i * ai[c];
+    }
+}
+
+// diagonalMatrixElement, block form (Viscosity_Weiler2018.cpp:605-742).
+void Sim::weiler_diag(size_t i, double *m) const {
+    const double h = hs, h2 = h * h, dtl = dt;
+    const double density0 = P.density0, mu = P.viscosity * density0, mub = P.viscosity_boundary * density0;
+    const double d = 10.0;
+    const double density_i = rho[i];
+    for (int c = 0; c < 9; c++) m[c] = 0.0;
+    const double *xi = &x[3 * i];
+    for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+        const size_t j = nf.idx[k];
+        const double density_j = rho[j];
+        const double *xj = &x[3 * j];
+        const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        double g[3];
+        K.gradW(xixj, g);
+        const double s = d * mu * (mass / density_j) / (sqn3(xixj) + 0.01 * h2);
+        for (int r0 = 0; r0 < 3; r0++)
+            for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xixj[c]);
+    }
+    if (mub != 0.0) {
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double *xj = &bx[3 * (size_t)j];
+                const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(xixj, g);
+                const double s = d * mub * (density0 * bV[j] / density_i) / (sqn3(xixj) + 0.01 * h2);
+                for (int r0 = 0; r0 < 3; r0++)
+                    for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xixj[c]);
+            }
+        } else if (has_volume_map() && vmV[i] > 0.0) {
+            const double Vj = vmV[i];
+            const double *xj = &vmX[3 * i];
+            const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
+            const double nl = std::sqrt(sqn3(normal));
+            if (nl > 0.0001) {
+                for (int c = 0; c < 3; c++) normal[c] /= nl;
+                double t1[3], t2[3];
+                orthogonal_vectors(normal, t1, t2);
+                const double dist = 0.5 * h;
+                const double vol = 0.25 * Vj;
+                for (int sidx = 0; sidx < 4; sidx++) {
+                    const double *t = (sidx < 2) ? t1 : t2;
+                    const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
+                    double xs[3], xix[3];
+                    for (int c = 0; c < 3; c++) {
+                        xs[c] = xj[c] + sg * t[c] * dist;
+                        xix[c] = xi[c] - xs[c];
+                    }
+                    double g[3];
+                    K.gradW(xix, g);
+                    const double s = d * mub * vol / (sqn3(xix) + 0.01 * h2);
+                    for (int r0 = 0; r0 < 3; r0++)
+                        for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xix[c]);
+                }
+            }
+        }
+    }
+    for (int r0 = 0; r0 < 3; r0++)
+        for (int c = 0; c < 3; c++) m[3 * r0 + c] = (r0 == c ? 1.0 : 0.0) - (dtl / density_i) * m[3 * r0 + c];
+}
+
+// Viscosity_Weiler2018::step/computeRHS/applyForces
+// (Viscosity_Weiler2018.cpp:891-1254), static boundaries.
+void Sim::viscosity_weiler() {
+    if (P.viscosity_method != SPH_VISCOSITY_WEILER2018 || n == 0) return;
+    const double dtl = dt;
+    std::vector<double> inv(9 * n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        double m[9];
+        weiler_diag(i, m);
+        inverse3(m, &inv[9 * i]);
+    }
+    std::vector<double> b(3 * n), g(3 * n), sol;
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) {
+            b[3 * i + c] = v[3 * i + c];  // boundary velocities are zero
+            g[3 * i + c] = v[3 * i + c] + vdiff[3 * i + c];
+        }
+    sol = g;
+    auto A = [&](const std::vector<double> &in, std::vector<double> &out) { weiler_matvec(in, out); };
+    auto M = [&](const std::vector<double> &in, std::vector<double> &out) {
+        for (size_t i = 0; i < n; i++) {
+            const double *m = &inv[9 * i];
+            for (int r0 = 0; r0 < 3; r0++)
+                out[3 * i + r0] = (m[3 * r0] * in[3 * i] + m[3 * r0 + 1] * in[3 * i + 1]) + m[3 * r0 + 2] * in[3 * i + 2];
+        }
+    };
+    CGResult cr = conjugate_gradient(3 * n, A, M, b, sol, P.visco_max_iter, P.visco_max_error);
+    viscoIterations = cr.iterations;
+    totVisco += cr.iterations;
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) {
+            const double nv = sol[3 * i + c];
+            a[3 * i + c] += (1.0 / dtl) * (nv - v[3 * i + c]);
+            vdiff[3 * i + c] = nv - v[3 * i + c];
+        }
+}
+
+// XSPH::step, scalar branch (XSPH.cpp:138-216), static boundaries.
+void Sim::xsph() {
+    const double cf = P.xsph, cb = P.xsph_boundary;
+    if (cf == 0.0 && cb == 0.0) return;
+    const double h = dt, invH = 1.0 / h, density0 = P.density0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i], *vi = &v[3 * i];
+        double *ai = &a[3 * i];
+        const double density_i = rho[i];
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double *xj = &x[3 * j], *vj = &v[3 * j];
+            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            const double w = K.W(d);
+            const double s = invH * cf * (mass / rho[j]);
+            for (int c = 0; c < 3; c++) ai[c] -= s * (vi[c] - vj[c]) * w;
+        }
+        if (cb != 0.0) {
+            if (has_boundary_particles()) {
+                for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                    const uint32_t j = nbd.idx[k];
+                    const double *xj = &bx[3 * (size_t)j];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    const double w = K.W(d);
+                    const double s = -invH * cb * (density0 * bV[j] / density_i);
+                    for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
+                }
+            } else if (has_volume_map() && vmV[i] > 0.0) {
+                const double Vj = vmV[i];
+                const double *xj = &vmX[3 * i];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                const double w = K.W(d);
+                const double s = -invH * cb * (density0 * Vj / density_i);
+                for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
+            }
+        }
+    }
+}
+
+// Simulation::updateTimeStepSize / updateTimeStepSizeCFL (Simulation.cpp:395-493).
+void Sim::cfl() {
+    if (P.cfl_method != 1 && P.cfl_method != 2) return;
+    const double hPrev = dt;
+    double h = dt;
+    double maxVel = 0.0;
+    const double diameter = 2.0 * r;
+    for (size_t i = 0; i < n; i++) {
+        const double w[3] = {v[3 * i] + a[3 * i] * h, v[3 * i + 1] + a[3 * i + 1] * h, v[3 * i + 2] + a[3 * i + 2] * h};
+        const double velMag = sqn3(w);
+        if (velMag > maxVel) maxVel = velMag;
+    }
+    if (maxVel < 1.0e-9) maxVel = 1.0e-9;
+    h = P.cfl_factor * 0.4 * (diameter / std::sqrt(maxVel));
+    h = std::min(h, P.cfl_max_time_step_size);
+    h = std::max(h, P.cfl_min_time_step_size);
+    dt = h;
+    if (P.cfl_method == 2) {
+        double hh = hPrev;
+        if (iterations == 0) return;
+        if (iterations > 10)
+            hh *= 0.9;
+        else if (iterations < 5)
+            hh *= 1.1;
+        hh = std::min(hh, dt);
+        dt = hh;
+    }
+}
+
+// TimeStepDFSPH.cpp:192-208 (uses h cached at the start of the step).
+void Sim::update_velocity() {
+    for (size_t i = 0; i < 3 * n; i++) v[i] += h_step * a[i];
+}
+
+// pressureSolve set-up loop (TimeStepDFSPH.cpp:252-309).
+void Sim::pressure_setup() {
+    const double h = dt, h2 = h * h, invH2 = 1.0 / h2, density0 = P.density0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double delta = div_sum(*this, i);
+        rho_adv[i] = rho[i] / density0 + h * delta;
+        factor[i] *= invH2;
+        if (rho_adv[i] > 1.0)
+            p[i] = 0.5 * std::min(p[i], 0.00025) * invH2;
+        else
+            p[i] = 0.0;
+    }
+}
+
+// pressureSolveIteration (TimeStepDFSPH.cpp:544-619).
+double Sim::pressure_iteration() {
+    const double h = dt, density0 = P.density0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) pressure_accel(p, i, &ap[3 * i]);
+    std::vector<double> err(n);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        double aij = aij_pj(i);
+        aij *= h * h;
+        const double s_i = 1.0 - rho_adv[i];
+        const double residuum = std::min(s_i - aij, 0.0);
+        p[i] = std::max(p[i] - 0.5 * (s_i - aij) * factor[i], 0.0);
+        err[i] = -density0 * residuum;
+    }
+    double e = 0.0;
+    for (size_t i = 0; i < n; i++) e += err[i];
+    return n ? e / (double)n : 0.0;
+}
+
+// pressureSolve finalisation (TimeStepDFSPH.cpp:345-383).
+void Sim::pressure_finish() {
+    const double h = dt, h2 = h * h;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        pressure_accel(p, i, &ap[3 * i]);
+        for (int c = 0; c < 3; c++) v[3 * i + c] += h * ap[3 * i + c];
+    }
+    for (size_t i = 0; i < n; i++) p[i] *= h2;
+}
+
+// TimeStepDFSPH::pressureSolve (TimeStepDFSPH.cpp:252-384).
+void Sim::pressure_solve() {
+    pressure_setup();
+    iterations = 0;
+    bool chk = false;
+    double avg = 0.0;
+    while ((!chk || (iterations < P.min_iterations)) && (iterations < P.max_iterations)) {
+        avg = pressure_iteration();
+        const double eta = P.max_error * 0.01 * P.density0;
+        chk = (avg <= eta);
+        iterations++;
+    }
+    lastErr = avg;
+    totIt += iterations;
+    pressure_finish();
+}
+
+// TimeStepDFSPH.cpp:220-248.
+void Sim::update_position() {
+    for (size_t i = 0; i < 3 * n; i++) x[i] += h_step * v[i];
+    time += h_step;
+}
+
+// TimeStepDFSPH::step (TimeStepDFSPH.cpp:117-249).
+void Sim::step() {
+    neighborhood_search();
+    volume_maps();
+    densities();
+    dfsph_factor();
+    if (P.enable_divergence_solver)
+        divergence_solve();
+    else
+        iterationsV = 0;
+    clear_accelerations();
+    viscosity_weiler();
+    xsph();
+    cfl();
+    update_velocity();
+    pressure_solve();
+    update_position();
+    steps++;
+}
+
+int Sim::stage(int s) {
+    switch (s) {
+        case SPH_STAGE_NEIGHBORHOOD_SEARCH: neighborhood_search(); break;
+        case SPH_STAGE_VOLUME_MAPS: volume_maps(); break;
+        case SPH_STAGE_DENSITIES: densities(); break;
+        case SPH_STAGE_DFSPH_FACTOR: dfsph_factor(); break;
+        case SPH_STAGE_DIVERGENCE_SETUP: divergence_setup(); break;
+        case SPH_STAGE_DIVERGENCE_ITER: lastErrV = divergence_iteration(); break;
+        case SPH_STAGE_DIVERGENCE_FINISH: divergence_finish(); break;
+        case SPH_STAGE_DIVERGENCE_SOLVE: divergence_solve(); break;
+        case SPH_STAGE_CLEAR_ACCEL: clear_accelerations(); break;
+        case SPH_STAGE_VISCOSITY: viscosity_weiler(); break;
+        case SPH_STAGE_XSPH: xsph(); break;
+        case SPH_STAGE_CFL: cfl(); break;
+        case SPH_STAGE_UPDATE_VELOCITY: update_velocity(); break;
+        case SPH_STAGE_PRESSURE_SETUP: pressure_setup(); break;
+        case SPH_STAGE_PRESSURE_ITER: lastErr = pressure_iteration(); break;
+        case SPH_STAGE_PRESSURE_FINISH: pressure_finish(); break;
+        case SPH_STAGE_PRESSURE_SOLVE: pressure_solve(); break;
+        case SPH_STAGE_UPDATE_POSITION: update_position(); break;
+        default: return SPH_ERR_INVALID;
+    }
+    return SPH_OK;
+}
+
+}  // namespace orc
+
+// ===================================================================== C API
+// Flat entry points for the ctypes binding in oracle/oracle.py.
+using orc::CSR;
+using orc::Sim;
+
+namespace {
+std::string g_err;
+int fail(const std::string &m) {
+    g_err = m;
+    return SPH_ERR_INVALID;
+}
+template <class R>
+int export_csr(const CSR &c, size_t nq, unsigned *off, unsigned *idx, size_t cap, size_t *nnz) {
+    (void)sizeof(R);
+    if (nnz) *nnz = c.idx.size();
+    if (off)
+        for (size_t i = 0; i <= nq; i++) off[i] = c.off[i];
+    if (idx) {
+        if (cap < c.idx.size()) return fail("neighbour capacity too small");
+        std::copy(c.idx.begin(), c.idx.end(), idx);
+    }
+    return SPH_OK;
+}
+struct FieldRef {
+    std::vector<double> *vec;
+    int comps;
+};
+bool field_of(Sim *s, const char *name, FieldRef &f) {
+    const std::string n(name);
+    if (n == "position") f = {&s->x, 3};
+    else if (n == "velocity") f = {&s->v, 3};
+    else if (n == "acceleration") f = {&s->a, 3};
+    else if (n == "density") f = {&s->rho, 1};
+    else if (n == "factor") f = {&s->factor, 1};
+    else if (n == "advected density") f = {&s->rho_adv, 1};
+    else if (n == "p / rho^2") f = {&s->p, 1};
+    else if (n == "p_v / rho^2") f = {&s->pv, 1};
+    else if (n == "pressure acceleration") f = {&s->ap, 3};
+    else if (n == "velocity difference") f = {&s->vdiff, 3};
+    else if (n == "boundary volume") f = {&s->vmV, 1};
+    else if (n == "boundary x") f = {&s->vmX, 3};
+    else if (n == "boundary particle volume") f = {&s->bV, 1};
+    else return false;
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+int orc_create(const sph_dfsph_params_t *p, void **out) {
+    if (!p || !out) return fail("null argument");
+    *out = new Sim(*p);
+    return SPH_OK;
+}
+int orc_destroy(void *h) {
+    delete (Sim *)h;
+    return SPH_OK;
+}
+int orc_set_fluid(void *h, const double *x, const double *v, size_t n) {
+    ((Sim *)h)->set_fluid(x, v, n);
+    return SPH_OK;
+}
+int orc_add_boundary_particles(void *h, const double *x, size_t n) {
+    ((Sim *)h)->add_boundary(x, n);
+    return SPH_OK;
+}
+int orc_set_volume_map_box(void *h, const double *lo, const double *hi, int invert, double thickness,
+                           const unsigned *res) {
+    orc::BoxSDF s;
+    for (int c = 0; c < 3; c++) {
+        s.lo[c] = lo[c];
+        s.hi[c] = hi[c];
+    }
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    ((Sim *)h)->set_volume_map(s, res);
+    return SPH_OK;
+}
+int orc_init(void *h) {
+    ((Sim *)h)->init();
+    return SPH_OK;
+}
+int orc_step(void *h, int n) {
+    for (int i = 0; i < n; i++) ((Sim *)h)->step();
+    return SPH_OK;
+}
+int orc_stage(void *h, int s) { return ((Sim *)h)->stage(s); }
+int orc_num_particles(void *h, size_t *out) {
+    *out = ((Sim *)h)->n;
+    return SPH_OK;
+}
+int orc_get_field(void *h, const char *name, double *out, size_t count) {
+    Sim *s = (Sim *)h;
+    const std::string n(name);
+    if (n == "fluid neighbor count" || n == "boundary neighbor count") {
+        const CSR &c = (n[0] == 'f') ? s->nf : s->nbd;
+        if (count != s->n) return fail("size mismatch");
+        for (size_t i = 0; i < s->n; i++) out[i] = c.off.empty() ? 0.0 : (double)c.count(i);
+        return SPH_OK;
+    }
+    FieldRef f;
+    if (!field_of(s, name, f)) return fail("unknown field");
+    if (count != f.vec->size()) return fail("size mismatch");
+    std::copy(f.vec->begin(), f.vec->end(), out);
+    return SPH_OK;
+}
+int orc_set_field(void *h, const char *name, const double *in, size_t count) {
+    Sim *s = (Sim *)h;
+    FieldRef f;
+    if (!field_of(s, name, f)) return fail("unknown field");
+    if (count != f.vec->size()) return fail("size mismatch");
+    std::copy(in, in + count, f.vec->begin());
+    return SPH_OK;
+}
+int orc_get_counters(void *h, sph_dfsph_counters_t *c) {
+    Sim *s = (Sim *)h;
+    std::memset(c, 0, sizeof(*c));
+    c->time = s->time;
+    c->time_step_size = s->dt;
+    c->avg_density_err = s->lastErr;
+    c->avg_divergence_err = s->lastErrV;
+    c->iterations = s->iterations;
+    c->iterations_v = s->iterationsV;
+    c->visco_iterations = s->viscoIterations;
+    c->steps = s->steps;
+    c->total_iterations = s->totIt;
+    c->total_iterations_v = s->totItV;
+    c->total_visco_iterations = s->totVisco;
+    unsigned mf = 0, mb = 0;
+    for (size_t i = 0; i + 1 < s->nf.off.size(); i++) mf = std::max(mf, s->nf.count(i));
+    for (size_t i = 0; i + 1 < s->nbd.off.size(); i++) mb = std::max(mb, s->nbd.count(i));
+    c->max_fluid_neighbors = (int)mf;
+    c->max_boundary_neighbors = (int)mb;
+    return SPH_OK;
+}
+int orc_set_time(void *h, double t, double dt) {
+    Sim *s = (Sim *)h;
+    s->time = t;
+    s->dt = dt;
+    s->h_step = dt;
+    return SPH_OK;
+}
+int orc_get_neighbors(void *h, int boundary, unsigned *off, unsigned *idx, size_t cap, size_t *nnz) {
+    Sim *s = (Sim *)h;
+    return export_csr<double>(boundary ? s->nbd : s->nf, s->n, off, idx, cap, nnz);
+}
+
+// Neighbour search on raw arrays: brute != 0 selects the O(n^2) search.
+int orc_ns_f32(const float *xq, size_t nq, const float *xs, size_t ns, float radius, int same_set, int brute,
+               unsigned *off, unsigned *idx, size_t cap, size_t *nnz) {
+    CSR c;
+    if (brute)
+        orc::ns_brute<float>(xq, nq, xs, ns, radius, same_set != 0, c);
+    else
+        orc::ns_grid<float>(xq, nq, xs, ns, radius, same_set != 0, c);
+    return export_csr<float>(c, nq, off, idx, cap, nnz);
+}
+int orc_ns_f64(const double *xq, size_t nq, const double *xs, size_t ns, double radius, int same_set, int brute,
+               unsigned *off, unsigned *idx, size_t cap, size_t *nnz) {
+    CSR c;
+    if (brute)
+        orc::ns_brute<double>(xq, nq, xs, ns, radius, same_set != 0, c);
+    else
+        orc::ns_grid<double>(xq, nq, xs, ns, radius, same_set != 0, c);
+    return export_csr<double>(c, nq, off, idx, cap, nnz);
+}
+
+// Table of PrecomputedKernel<CubicKernel,10000> for support radius h:
+// T[10000], G[10001]; scalars[4] = {invStep, W_zero, k, l}.
+int orc_kernel_table(double h, double *T, double *G, double *scalars) {
+    orc::PreCubic K;
+    K.set(h);
+    std::copy(K.T.begin(), K.T.end(), T);
+    std::copy(K.G.begin(), K.G.end(), G);
+    scalars[0] = K.invStep;
+    scalars[1] = K.W0;
+    scalars[2] = K.cubic.k;
+    scalars[3] = K.cubic.l;
+    return SPH_OK;
+}
+// precomputed != 0: table kernel, else analytic CubicKernel.
+int orc_kernel_eval(double h, int precomputed, const double *r, size_t n, double *w, double *g) {
+    orc::PreCubic K;
+    K.set(h);
+    for (size_t i = 0; i < n; i++) {
+        if (precomputed) {
+            w[i] = K.W(r + 3 * i);
+            K.gradW(r + 3 * i, g + 3 * i);
+        } else {
+            w[i] = K.cubic.W(std::sqrt(orc::sqn3(r + 3 * i)));
+            K.cubic.gradW(r + 3 * i, g + 3 * i);
+        }
+    }
+    return SPH_OK;
+}
+int orc_gauss_legendre(int n, double *x, double *w) {
+    if (n < 1 || n > 64) return fail("n out of range");
+    orc::gauss_legendre(n, x, w);
+    return SPH_OK;
+}
+int orc_serendipity(const double *xi, double *N, double *dN) {
+    orc::serendipity(xi, N, dN);
+    return SPH_OK;
+}
+int orc_serendipity_node(int k, double *xi) {
+    if (k < 0 || k > 31) return fail("node out of range");
+    orc::serendipity_node(k, xi);
+    return SPH_OK;
+}
+int orc_box_sdf(const double *lo, const double *hi, int invert, double thickness, const double *x, size_t n,
+                double *out) {
+    orc::BoxSDF s;
+    for (int c = 0; c < 3; c++) {
+        s.lo[c] = lo[c];
+        s.hi[c] = hi[c];
+    }
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    for (size_t i = 0; i < n; i++) out[i] = s.eval(x + 3 * i);
+    return SPH_OK;
+}
+// Node values of the volume map of a box (field 0 sdf, field 1 volume) at n
+// points, and interpolated values + gradients at m points.
+int orc_volume_map_eval(const double *lo, const double *hi, int invert, double thickness, const unsigned *res,
+                        double support_radius, int field, int interpolate, const double *x, size_t n,
+                        double *val, double *grad, int *ok) {
+    orc::BoxSDF s;
+    for (int c = 0; c < 3; c++) {
+        s.lo[c] = lo[c];
+        s.hi[c] = hi[c];
+    }
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    orc::VolumeMap vm;
+    vm.init(s, support_radius, res);
+    if (interpolate) {
+        std::vector<uint64_t> cells;
+        for (size_t i = 0; i < n; i++) {
+            uint64_t c;
+            double xi[3];
+            if (vm.locate(x + 3 * i, c, xi)) cells.push_back(c);
+        }
+        vm.prefetch(cells);
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (interpolate) {
+            ok[i] = vm.interpolate(field, x + 3 * i, val + i, grad ? grad + 3 * i : nullptr) ? 1 : 0;
+        } else {
+            val[i] = field == 0 ? vm.node_sdf(x + 3 * i) : vm.node_volume(x + 3 * i);
+            ok[i] = 1;
+        }
+    }
+    return SPH_OK;
+}
+// Volume map domain: lo[3], hi[3], cell size[3].
+int orc_volume_map_domain(const double *blo, const double *bhi, double thickness, const unsigned *res,
+                          double support_radius, double *lo, double *hi, double *cs) {
+    orc::BoxSDF s;
+    for (int c = 0; c < 3; c++) {
+        s.lo[c] = blo[c];
+        s.hi[c] = bhi[c];
+    }
+    s.thickness = thickness;
+    orc::VolumeMap vm;
+    vm.init(s, support_radius, res);
+    for (int c = 0; c < 3; c++) {
+        lo[c] = vm.lo[c];
+        hi[c] = vm.hi[c];
+        cs[c] = vm.cs[c];
+    }
+    return SPH_OK;
+}
+// Block-Jacobi preconditioned CG on a dense SPD matrix A (n x n, n = 3m,
+// row-major), Eigen's algorithm. x holds the initial guess on entry.
+int orc_cg_dense(const double *A, size_t n, const double *b, double *x, int maxIter, double tol, int *iters,
+                 double *err) {
+    if (n % 3) return fail("n must be a multiple of 3");
+    const size_t m = n / 3;
+    std::vector<double> inv(9 * m);
+    for (size_t i = 0; i < m; i++) {
+        double blk[9];
+        for (int r0 = 0; r0 < 3; r0++)
+            for (int c = 0; c < 3; c++) blk[3 * r0 + c] = A[(3 * i + r0) * n + 3 * i + c];
+        orc::inverse3(blk, &inv[9 * i]);
+    }
+    auto mv = [&](const std::vector<double> &in, std::vector<double> &out) {
+        for (size_t r0 = 0; r0 < n; r0++) {
+            double s = 0.0;
+            for (size_t c = 0; c < n; c++) s += A[r0 * n + c] * in[c];
+            out[r0] = s;
+        }
+    };
+    auto M = [&](const std::vector<double> &in, std::vector<double> &out) {
+        for (size_t i = 0; i < m; i++) {
+            const double *q = &inv[9 * i];
+            for (int r0 = 0; r0 < 3; r0++)
+                out[3 * i + r0] = (q[3 * r0] * in[3 * i] + q[3 * r0 + 1] * in[3 * i + 1]) + q[3 * r0 + 2] * in[3 * i + 2];
+        }
+    };
+    std::vector<double> bb(b, b + n), xx(x, x + n);
+    orc::CGResult r = orc::conjugate_gradient(n, mv, M, bb, xx, maxIter, tol);
+    std::copy(xx.begin(), xx.end(), x);
+    *iters = r.iterations;
+    *err = r.error;
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""ctypes mirrors of the plain structs declared in include/sph_capi.h.
+
+The layouts must match the C declarations field by field; both the HIP
+library binding (splishsplash_amd.capi) and the oracle binding use them.
+"""
+import ctypes as C
+
+BOUNDARY_NONE = -1
+BOUNDARY_AKINCI2012 = 0
+BOUNDARY_BENDER2019 = 2
+VISCOSITY_NONE = 0
+VISCOSITY_WEILER2018 = 6
+
+STAGES = {
+    "neighborhood_search": 0, "volume_maps": 1, "densities": 2,
+    "dfsph_factor": 3, "divergence_setup": 4, "divergence_iter": 5,
+    "divergence_finish": 6, "divergence_solve": 7, "clear_accel": 8,
+    "viscosity": 9, "xsph": 10, "cfl": 11, "update_velocity": 12,
+    "pressure_setup": 13, "pressure_iter": 14, "pressure_finish": 15,
+    "pressure_solve": 16, "update_position": 17,
+}
+
+
+class DFSPHParams(C.Structure):
+    """sph_dfsph_params_t."""
+    _fields_ = [
+        ("particle_radius", C.c_double),
+        ("density0", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("time_step_size", C.c_double),
+        ("cfl_method", C.c_int),
+        ("cfl_factor", C.c_double),
+        ("cfl_min_time_step_size", C.c_double),
+        ("cfl_max_time_step_size", C.c_double),
+        ("min_iterations", C.c_int),
+        ("max_iterations", C.c_int),
+        ("max_error", C.c_double),
+        ("max_iterations_v", C.c_int),
+        ("max_error_v", C.c_double),
+        ("enable_divergence_solver", C.c_int),
+        ("xsph", C.c_double),
+        ("xsph_boundary", C.c_double),
+        ("viscosity_method", C.c_int),
+        ("viscosity", C.c_double),
+        ("viscosity_boundary", C.c_double),
+        ("visco_max_iter", C.c_int),
+        ("visco_max_error", C.c_double),
+        ("boundary_handling_method", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+class DFSPHCounters(C.Structure):
+    """sph_dfsph_counters_t."""
+    _fields_ = [
+        ("time", C.c_double),
+        ("time_step_size", C.c_double),
+        ("avg_density_err", C.c_double),
+        ("avg_divergence_err", C.c_double),
+        ("iterations", C.c_int),
+        ("iterations_v", C.c_int),
+        ("visco_iterations", C.c_int),
+        ("steps", C.c_int),
+        ("total_iterations", C.c_longlong),
+        ("total_iterations_v", C.c_longlong),
+        ("total_visco_iterations", C.c_longlong),
+        ("max_fluid_neighbors", C.c_int),
+        ("max_boundary_neighbors", C.c_int),
+        ("flags", C.c_int),
+        ("reserved", C.c_int * 7),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def default_params(**kw):
+    """Reference constructor defaults (Simulation.cpp:66-89,
+    TimeStepDFSPH.cpp:29-41, TimeManager.cpp:12, Viscosity_Weiler2018.cpp:21-29),
+    overridden by keyword arguments."""
+    p = DFSPHParams()
+    p.particle_radius = 0.025
+    p.density0 = 1000.0
+    p.gravity[:] = (0.0, -9.81, 0.0)
+    p.time_step_size = 0.001
+    p.cfl_method = 1
+    p.cfl_factor = 0.5
+    p.cfl_min_time_step_size = 0.0001
+    p.cfl_max_time_step_size = 0.005
+    p.min_iterations = 2
+    p.max_iterations = 100
+    p.max_error = 0.01
+    p.max_iterations_v = 100
+    p.max_error_v = 0.1
+    p.enable_divergence_solver = 1
+    p.xsph = 0.0
+    p.xsph_boundary = 0.0
+    p.viscosity_method = VISCOSITY_NONE
+    p.viscosity = 0.01
+    p.viscosity_boundary = 0.0
+    p.visco_max_iter = 100
+    p.visco_max_error = 0.01
+    p.boundary_handling_method = BOUNDARY_AKINCI2012
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity[:] = tuple(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+FIELD_COMPONENTS = {
+    "position": 3, "velocity": 3, "acceleration": 3, "density": 1,
+    "factor": 1, "advected density": 1, "p / rho^2": 1, "p_v / rho^2": 1,
+    "pressure acceleration": 3, "velocity difference": 3,
+    "boundary volume": 1, "boundary x": 3,
+    "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1,
+}
