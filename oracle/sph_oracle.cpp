@@ -1417,8 +1417,7 @@ int orc_get_counters(void *h, sph_dfsph_counters_t *c) {
 int orc_set_time(void *h, double t, double dt) {
     Sim *s = (Sim *)h;
     s->time = t;
-    s->dt = dt;
-    s->h_step = dt;
+    s->dt = dt;  // h_step is taken from dt by the next neighbourhood search
     return SPH_OK;
 }
 int orc_get_neighbors(void *h, int boundary, unsigned *off, unsigned *idx, size_t cap, size_t *nnz) {

@@ -1,0 +1,356 @@
+"""ctypes binding of libsph_amd.so (the C-ABI of include/sph_capi.h).
+
+`NeighborhoodSearch` mirrors CompactNSearch::NeighborhoodSearch as the
+reference uses it (SPlisHSPlasH/NeighborhoodSearch.h, Simulation.cpp), and
+`TimeStepDFSPH` mirrors SPH::TimeStepDFSPH plus the per-particle state it
+steps. There is no fallback: if the library is missing, loading raises, and
+if no gfx950 device is visible every compute call raises SPHError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import DFSPHCounters, DFSPHParams, FIELD_COMPONENTS, STAGES, default_params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsph_amd.so")
+
+SPH_F32, SPH_F64 = 0, 1
+_ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "invalid state",
+           -5: "no gfx950 device", -6: "unknown field"}
+
+
+class SPHError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_up = C.POINTER(C.c_uint)
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_szp = C.POINTER(C.c_size_t)
+_vp = C.c_void_p
+
+_SIGS = {
+    "sph_last_error": (C.c_char_p, []),
+    "sph_version": (C.c_int, []),
+    "sph_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "sph_device_synchronize": (C.c_int, []),
+    "sph_ns_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(_vp)]),
+    "sph_ns_destroy": (C.c_int, [_vp]),
+    "sph_ns_set_radius": (C.c_int, [_vp, C.c_double]),
+    "sph_ns_get_radius": (C.c_int, [_vp, _dp]),
+    "sph_ns_reset": (C.c_int, [_vp]),
+    "sph_ns_add_point_set": (C.c_int, [_vp, _vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _up]),
+    "sph_ns_resize_point_set": (C.c_int, [_vp, C.c_uint, _vp, C.c_size_t]),
+    "sph_ns_update_point_sets": (C.c_int, [_vp]),
+    "sph_ns_find_neighbors": (C.c_int, [_vp, C.c_int]),
+    "sph_ns_query_point": (C.c_int, [_vp, _dp, C.c_uint, _up, C.c_uint, _up]),
+    "sph_ns_z_sort": (C.c_int, [_vp]),
+    "sph_ns_sort_field": (C.c_int, [_vp, C.c_uint, _vp, C.c_size_t]),
+    "sph_ns_set_active_all": (C.c_int, [_vp, C.c_int]),
+    "sph_ns_set_active_set": (C.c_int, [_vp, C.c_uint, C.c_int, C.c_int]),
+    "sph_ns_set_active_pair": (C.c_int, [_vp, C.c_uint, C.c_uint, C.c_int]),
+    "sph_ns_is_active_pair": (C.c_int, [_vp, C.c_uint, C.c_uint, C.POINTER(C.c_int)]),
+    "sph_ns_n_point_sets": (C.c_int, [_vp, _up]),
+    "sph_ns_n_points": (C.c_int, [_vp, C.c_uint, _szp]),
+    "sph_ns_user_data": (C.c_int, [_vp, C.c_uint, C.POINTER(_vp)]),
+    "sph_ns_n_neighbors": (C.c_int, [_vp, C.c_uint, C.c_uint, C.c_uint, _up]),
+    "sph_ns_neighbor": (C.c_int, [_vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, _up]),
+    "sph_ns_neighbor_list": (C.c_int, [_vp, C.c_uint, C.c_uint, C.c_uint, C.POINTER(_up), _up]),
+    "sph_ns_neighbor_csr": (C.c_int, [_vp, C.c_uint, C.c_uint, C.POINTER(_up), C.POINTER(_up)]),
+    "sph_dfsph_default_params": (None, [C.POINTER(DFSPHParams)]),
+    "sph_dfsph_create": (C.c_int, [C.POINTER(DFSPHParams), C.POINTER(_vp)]),
+    "sph_dfsph_destroy": (C.c_int, [_vp]),
+    "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
+    "sph_dfsph_add_boundary_particles": (C.c_int, [_vp, _fp, C.c_size_t]),
+    "sph_dfsph_set_volume_map_box": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _up]),
+    "sph_dfsph_init": (C.c_int, [_vp]),
+    "sph_dfsph_step": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_stage": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_num_particles": (C.c_int, [_vp, _szp]),
+    "sph_dfsph_get_field": (C.c_int, [_vp, C.c_char_p, _vp, C.c_size_t]),
+    "sph_dfsph_set_field": (C.c_int, [_vp, C.c_char_p, _vp, C.c_size_t]),
+    "sph_dfsph_get_neighbors": (C.c_int, [_vp, C.c_int, _up, _up, C.c_size_t, _szp]),
+    "sph_dfsph_get_counters": (C.c_int, [_vp, C.POINTER(DFSPHCounters)]),
+    "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_kernel_stats": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_longlong),
+                                         _dp, C.POINTER(C.c_int)]),
+    "sph_dfsph_reset_kernel_stats": (C.c_int, [_vp]),
+    "sph_dfsph_synchronize": (C.c_int, [_vp]),
+    "sph_kernel_eval": (C.c_int, [C.c_double, _fp, C.c_size_t, _fp, _fp]),
+}
+
+
+def lib():
+    """Loads libsph_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                              f"g.build()'` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise SPHError(rc, lib().sph_last_error().decode())
+
+
+def device_count():
+    n = C.c_int(0)
+    _chk(lib().sph_device_count(C.byref(n)))
+    return n.value
+
+
+def synchronize():
+    _chk(lib().sph_device_synchronize())
+
+
+def kernel_eval(support_radius, r):
+    """Device evaluation of the precomputed cubic kernel at FP32 offsets r."""
+    r = np.ascontiguousarray(r, dtype=np.float32).reshape(-1, 3)
+    n = len(r)
+    w = np.zeros(n, np.float32)
+    g = np.zeros((n, 3), np.float32)
+    _chk(lib().sph_kernel_eval(support_radius, r.ctypes.data_as(_fp), n, w.ctypes.data_as(_fp),
+                               g.ctypes.data_as(_fp)))
+    return w, g
+
+
+class PointSet:
+    """View of one point set (CompactNSearch::PointSet)."""
+
+    def __init__(self, ns, index):
+        self._ns = ns
+        self.index = index
+
+    def n_points(self):
+        n = C.c_size_t(0)
+        _chk(lib().sph_ns_n_points(self._ns.h, self.index, C.byref(n)))
+        return n.value
+
+    def n_neighbors(self, set_j, i):
+        out = C.c_uint(0)
+        _chk(lib().sph_ns_n_neighbors(self._ns.h, self.index, set_j, i, C.byref(out)))
+        return out.value
+
+    def neighbor(self, set_j, i, k):
+        out = C.c_uint(0)
+        _chk(lib().sph_ns_neighbor(self._ns.h, self.index, set_j, i, k, C.byref(out)))
+        return out.value
+
+    def neighbor_list(self, set_j, i):
+        p = _up()
+        cnt = C.c_uint(0)
+        _chk(lib().sph_ns_neighbor_list(self._ns.h, self.index, set_j, i, C.byref(p), C.byref(cnt)))
+        return np.ctypeslib.as_array(p, shape=(cnt.value,)).copy() if cnt.value else np.zeros(0, np.uint32)
+
+    def sort_field(self, arr):
+        """Permutes arr (first axis = points) in place with the last z_sort."""
+        assert arr.flags.c_contiguous
+        _chk(lib().sph_ns_sort_field(self._ns.h, self.index, arr.ctypes.data, arr.nbytes // max(len(arr), 1)))
+
+    def get_user_data(self):
+        out = _vp()
+        _chk(lib().sph_ns_user_data(self._ns.h, self.index, C.byref(out)))
+        return out.value
+
+
+class NeighborhoodSearch:
+    """Drop-in for CompactNSearch::NeighborhoodSearch on the device."""
+
+    def __init__(self, radius, erase_empty_cells=False):
+        h = _vp()
+        _chk(lib().sph_ns_create(radius, int(erase_empty_cells), C.byref(h)))
+        self.h = h
+        self._arrays = []
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.sph_ns_destroy(self.h)
+            self.h = None
+
+    def set_radius(self, r):
+        _chk(lib().sph_ns_set_radius(self.h, r))
+
+    def radius(self):
+        r = C.c_double(0)
+        _chk(lib().sph_ns_get_radius(self.h, C.byref(r)))
+        return r.value
+
+    def reset(self):
+        _chk(lib().sph_ns_reset(self.h))
+        self._arrays = []
+
+    def add_point_set(self, x, is_dynamic=True, search_neighbors=True, find_neighbors=True):
+        """x: C-contiguous (n, 3) float32 or float64 array, kept by reference
+        (the library reads it on update_point_sets/find_neighbors)."""
+        assert x.flags.c_contiguous and x.ndim == 2 and x.shape[1] == 3
+        dtype = SPH_F64 if x.dtype == np.float64 else SPH_F32
+        assert x.dtype in (np.float32, np.float64)
+        idx = C.c_uint(0)
+        _chk(lib().sph_ns_add_point_set(self.h, x.ctypes.data, len(x), dtype, int(is_dynamic),
+                                        int(search_neighbors), int(find_neighbors), None, C.byref(idx)))
+        self._arrays.append(x)
+        return idx.value
+
+    def resize_point_set(self, i, x):
+        assert x.flags.c_contiguous
+        _chk(lib().sph_ns_resize_point_set(self.h, i, x.ctypes.data, len(x)))
+        self._arrays[i] = x
+
+    def update_point_sets(self):
+        _chk(lib().sph_ns_update_point_sets(self.h))
+
+    def find_neighbors(self, points_changed=True):
+        _chk(lib().sph_ns_find_neighbors(self.h, int(points_changed)))
+
+    def find_neighbors_point(self, x, set_j, cap=4096):
+        out = np.zeros(cap, np.uint32)
+        cnt = C.c_uint(0)
+        xx = np.ascontiguousarray(x, np.float64)
+        _chk(lib().sph_ns_query_point(self.h, xx.ctypes.data_as(_dp), set_j, out.ctypes.data_as(_up), cap,
+                                      C.byref(cnt)))
+        return out[: min(cnt.value, cap)]
+
+    def z_sort(self):
+        _chk(lib().sph_ns_z_sort(self.h))
+
+    def set_active(self, *args):
+        if len(args) == 1:
+            _chk(lib().sph_ns_set_active_all(self.h, int(args[0])))
+        elif len(args) == 3 and isinstance(args[2], bool) and isinstance(args[1], int) and not isinstance(args[1], bool):
+            _chk(lib().sph_ns_set_active_pair(self.h, args[0], args[1], int(args[2])))
+        else:
+            i, s, f = args[0], args[1] if len(args) > 1 else True, args[2] if len(args) > 2 else True
+            _chk(lib().sph_ns_set_active_set(self.h, i, int(s), int(f)))
+
+    def is_active(self, i, j):
+        out = C.c_int(0)
+        _chk(lib().sph_ns_is_active_pair(self.h, i, j, C.byref(out)))
+        return bool(out.value)
+
+    def n_point_sets(self):
+        out = C.c_uint(0)
+        _chk(lib().sph_ns_n_point_sets(self.h, C.byref(out)))
+        return out.value
+
+    def point_set(self, i):
+        return PointSet(self, i)
+
+    def neighbors_csr(self, i, j):
+        """(offsets, indices) copies of all lists of set i in set j."""
+        po, pi = _up(), _up()
+        _chk(lib().sph_ns_neighbor_csr(self.h, i, j, C.byref(po), C.byref(pi)))
+        n = self.point_set(i).n_points()
+        off = np.ctypeslib.as_array(po, shape=(n + 1,)).copy()
+        idx = np.ctypeslib.as_array(pi, shape=(int(off[-1]),)).copy() if off[-1] else np.zeros(0, np.uint32)
+        return off, idx
+
+
+class TimeStepDFSPH:
+    """Device-resident DFSPH time step (SPH::TimeStepDFSPH) with the fluid,
+    Akinci2012 boundary particles or a Bender2019 volume-map box."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else default_params(**kw)
+        h = _vp()
+        _chk(lib().sph_dfsph_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        self.n = 0
+        self.nb = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.sph_dfsph_destroy(self.h)
+            self.h = None
+
+    def set_fluid(self, x, v=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+        self.n = len(x)
+        _chk(lib().sph_dfsph_set_fluid(self.h, x.ctypes.data_as(_fp), v.ctypes.data_as(_fp), self.n))
+
+    def add_boundary_particles(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        self.nb += len(x)
+        _chk(lib().sph_dfsph_add_boundary_particles(self.h, x.ctypes.data_as(_fp), len(x)))
+
+    def set_volume_map_box(self, lo, hi, res, invert=True, thickness=0.0):
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        r = np.ascontiguousarray(res, np.uint32)
+        _chk(lib().sph_dfsph_set_volume_map_box(self.h, lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp),
+                                                int(invert), thickness, r.ctypes.data_as(_up)))
+
+    def init(self):
+        _chk(lib().sph_dfsph_init(self.h))
+
+    def step(self, n=1):
+        _chk(lib().sph_dfsph_step(self.h, n))
+
+    def stage(self, name):
+        _chk(lib().sph_dfsph_stage(self.h, STAGES[name] if isinstance(name, str) else int(name)))
+
+    def get(self, name):
+        if name == "boundary particle volume":
+            out = np.zeros(self.nb, np.float32)
+        else:
+            comps = FIELD_COMPONENTS[name]
+            dt = np.uint32 if name in ("id", "fluid neighbor count", "boundary neighbor count") else np.float32
+            out = np.zeros((self.n, comps) if comps > 1 else self.n, dt)
+        _chk(lib().sph_dfsph_get_field(self.h, name.encode(), out.ctypes.data, out.nbytes))
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        _chk(lib().sph_dfsph_set_field(self.h, name.encode(), a.ctypes.data, a.nbytes))
+
+    def neighbors(self, boundary=False):
+        off = np.zeros(self.n + 1, np.uint32)
+        nnz = C.c_size_t(0)
+        _chk(lib().sph_dfsph_get_neighbors(self.h, int(boundary), off.ctypes.data_as(_up), None, 0, C.byref(nnz)))
+        idx = np.zeros(max(nnz.value, 1), np.uint32)
+        _chk(lib().sph_dfsph_get_neighbors(self.h, int(boundary), off.ctypes.data_as(_up),
+                                           idx.ctypes.data_as(_up), idx.size, C.byref(nnz)))
+        return off, idx[: nnz.value]
+
+    def counters(self):
+        c = DFSPHCounters()
+        _chk(lib().sph_dfsph_get_counters(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def set_time(self, t, dt):
+        _chk(lib().sph_dfsph_set_time(self.h, t, dt))
+
+    def enable_timing(self, on=True):
+        _chk(lib().sph_dfsph_enable_timing(self.h, int(on)))
+
+    def reset_kernel_stats(self):
+        _chk(lib().sph_dfsph_reset_kernel_stats(self.h))
+
+    def kernel_stats(self):
+        cnt = C.c_int(0)
+        _chk(lib().sph_dfsph_kernel_stats(self.h, -1, None, None, None, C.byref(cnt)))
+        out = {}
+        for k in range(cnt.value):
+            nm = C.c_char_p()
+            la = C.c_longlong(0)
+            ms = C.c_double(0)
+            _chk(lib().sph_dfsph_kernel_stats(self.h, k, C.byref(nm), C.byref(la), C.byref(ms), C.byref(cnt)))
+            out[nm.value.decode()] = (la.value, ms.value)
+        return out
+
+    def synchronize(self):
+        _chk(lib().sph_dfsph_synchronize(self.h))

@@ -1,0 +1,200 @@
+// common.hpp -- host/device utilities shared by the HIP sources of
+// libsph_amd.so: status handling, device buffers, launch shapes, timing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_capi.h"
+
+namespace sph {
+
+// ----------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+void set_last_error(const std::string &m);
+
+#define SPH_HIP(call)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw ::sph::Error(e_ == hipErrorOutOfMemory ? SPH_ERR_OOM : SPH_ERR_HIP,            \
+                               std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+#define SPH_REQUIRE(cond, code, msg)                       \
+    do {                                                   \
+        if (!(cond)) throw ::sph::Error((code), (msg));    \
+    } while (0)
+
+// Runs f() and converts exceptions into a status code + last error.
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return SPH_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return SPH_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return SPH_ERR_INVALID;
+    }
+}
+
+// Throws SPH_ERR_DEVICE unless a gfx950 device is current.
+void require_device();
+
+// ---------------------------------------------------------- device buffer
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grows to at least m elements (contents not preserved).
+    void reserve(size_t m) {
+        if (m <= n) return;
+        release();
+        if (m == 0) return;
+        SPH_HIP(hipMalloc(&p, m * sizeof(T)));
+        n = m;
+    }
+    void swap(DBuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+    T *get() const { return p; }
+};
+
+// ------------------------------------------------------------- launching
+constexpr int BLOCK = 256;
+inline unsigned nblocks(size_t n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
+
+// Workgroups of one dispatch are spread round-robin over the 8 XCDs of the
+// MI355X, each with its own L2. Remapping the hardware block id so that each
+// XCD receives one contiguous range of blocks keeps spatially adjacent
+// particles (and the neighbour data they share) in the same L2.
+__device__ inline unsigned xcd_block() {
+    const unsigned b = blockIdx.x, B = gridDim.x;
+    const unsigned per = B / 8u, rem = B % 8u;
+    const unsigned xcd = b % 8u, local = b / 8u;
+    return xcd < rem ? xcd * (per + 1u) + local : rem * (per + 1u) + (xcd - rem) * per + local;
+}
+
+// ------------------------------------------------------------- wave math
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline unsigned wave_max_u(unsigned v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned w = __shfl_xor(v, o, 64);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+__device__ inline double wave_max_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide sum of a double (BLOCK threads); result valid in thread 0.
+__device__ inline double block_sum(double v, double *lds /* BLOCK/64 */) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += lds[w];
+    __syncthreads();
+    return r;
+}
+__device__ inline double block_max(double v, double *lds) {
+    v = wave_max_d(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = lds[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmax(r, lds[w]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Last-block detection for grid reductions: every block publishes its
+// partial, fences, and takes a ticket; the block drawing the final ticket
+// sees all partials and finishes the reduction. Returns true in that block
+// (all threads).
+__device__ inline bool last_block(unsigned *counter) {
+    __shared__ bool amLast;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicAdd(counter, 1u);
+        amLast = (t == gridDim.x - 1u);
+    }
+    __syncthreads();
+    if (amLast) __threadfence();
+    return amLast;
+}
+
+// ------------------------------------------------------------------ timing
+// Per-kernel HIP-event timing on one stream (enabled on demand).
+class KernelTimer {
+public:
+    struct Stat {
+        std::string name;
+        long long launches = 0;
+        double ms = 0.0;
+    };
+    bool enabled = false;
+    int begin(hipStream_t s, const char *name);
+    void end(hipStream_t s, int slot);
+    void collect();  // synchronises pending events into the stats
+    void reset();
+    std::vector<Stat> stats;
+    ~KernelTimer();
+
+private:
+    struct Pending {
+        hipEvent_t a, b;
+        int stat;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    int stat_index(const char *name);
+    hipEvent_t get_event();
+};
+
+struct TimedScope {
+    KernelTimer &t;
+    hipStream_t s;
+    int slot;
+    TimedScope(KernelTimer &tt, hipStream_t ss, const char *name) : t(tt), s(ss), slot(tt.enabled ? tt.begin(ss, name) : -1) {}
+    ~TimedScope() {
+        if (slot >= 0) t.end(s, slot);
+    }
+};
+
+}  // namespace sph

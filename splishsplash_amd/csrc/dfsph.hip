@@ -1,0 +1,1507 @@
+// dfsph.hip -- device-resident DFSPH time step.
+//
+// One thread per particle, particles kept in cell order (re-sorted every
+// step by a counting sort; the caller-visible order is the particle id).
+// The reference's sweeps (TimeStepDFSPH.cpp, scalar branch) are fused where
+// no global dependency separates them:
+//   sweep_init   densities + DFSPH factor + divergence set-up (1 sweep)
+//   accel/jacobi the two sweeps of every Jacobi iteration; the FP64 error
+//                sum is finished by the last block, which also decides
+//                convergence on the device
+//   post         clear accelerations + Weiler2018 apply + XSPH + CFL
+//                + v += h a + p_v scaling (1 sweep)
+//   finish/final pressure accelerations + v update, then p scaling and the
+//                compensated position update
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "dfsph.hpp"
+
+namespace sph {
+
+__device__ __forceinline__ unsigned gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+__device__ __forceinline__ float3 f3(const float4 &a) { return make_float3(a.x, a.y, a.z); }
+
+// ------------------------------------------------------------------ small
+__global__ void k_set_hstep(Ctl *C) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) C->h_step = C->h;
+}
+__global__ void k_reset(Ctl *C, int which) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (which == 0) {
+        C->it = 0;
+        C->done = 0;
+    } else if (which == 1) {
+        C->it_v = 0;
+        C->done_v = 0;
+    } else if (which == 2) {
+        C->maxcnt_f = 0;
+        C->maxcnt_b = 0;
+    } else if (which == 3) {
+        C->cg_it = 0;
+        C->cg_done = 0;
+    }
+}
+
+// --------------------------------------------------- densities/factor/div
+// TimeStep::computeDensities (TimeStep.cpp:116-169), computeDFSPHFactor
+// (TimeStepDFSPH.cpp:1106-1186), computeDensityChange + divergence warm start
+// (TimeStepDFSPH.cpp:398-463, 1247-1295).
+template <bool DENS, bool FACT, bool DIV>
+__global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float4 xi = D.x[i];
+    const float4 vi = DIV ? D.v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float dens = D.V * D.K.W0;
+    float sgk = 0.f, gpx = 0.f, gpy = 0.f, gpz = 0.f, div = 0.f, divb = 0.f;
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float4 xj = D.x[j];
+        float3 r;
+        float w, g;
+        kernel_WG(xi, xj, D.K, r, w, g);
+        if (DENS) dens += D.V * w;
+        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+        if (FACT) {
+            const float qx = -D.V * gx, qy = -D.V * gy, qz = -D.V * gz;
+            sgk += sqn(qx, qy, qz);
+            gpx -= qx;
+            gpy -= qy;
+            gpz -= qz;
+        }
+        if (DIV) {
+            const float4 vj = D.v[j];
+            div += ((vi.x - vj.x) * gx + (vi.y - vj.y) * gy) + (vi.z - vj.z) * gz;
+        }
+    }
+    unsigned nnb = cnt;
+    if (D.bmode == 1) {
+        const unsigned cb = D.cb[i];
+        nnb += cb;
+        for (unsigned k = 0; k < cb; k++) {
+            const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+            float3 r;
+            float w, g;
+            kernel_WG(xi, xb, D.K, r, w, g);
+            if (DENS) dens += xb.w * w;
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            if (FACT) {
+                gpx -= -xb.w * gx;
+                gpy -= -xb.w * gy;
+                gpz -= -xb.w * gz;
+            }
+            if (DIV) divb += xb.w * (((vi.x * gx + vi.y * gy) + vi.z * gz));
+        }
+    } else if (D.bmode == 2) {
+        const float4 xb = D.vmb[i];
+        if (xb.w > 0.f) {
+            float3 r;
+            float w, g;
+            kernel_WG(xi, xb, D.K, r, w, g);
+            if (DENS) dens += xb.w * w;
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            if (FACT) {
+                gpx -= -xb.w * gx;
+                gpy -= -xb.w * gy;
+                gpz -= -xb.w * gz;
+            }
+            if (DIV) divb += xb.w * (((vi.x * gx + vi.y * gy) + vi.z * gz));
+        }
+    }
+    if (DENS) D.rho[i] = dens * D.rho0;
+    if (FACT || DIV) {
+        const float invH = (float)(1.0 / D.ctl->h);
+        float a;
+        if (FACT) {
+            sgk += sqn(gpx, gpy, gpz);
+            a = (sgk > D.eps) ? 1.0f / sgk : 0.0f;
+        } else {
+            a = D.alpha[i];
+        }
+        if (DIV) a *= invH;
+        D.alpha[i] = a;
+        if (DIV) {
+            const float rd = div * D.V + divb;
+            D.radv[i] = rd;
+            float dAdv = fmaxf(rd, 0.0f);
+            if (nnb < 20u) dAdv = 0.0f;
+            D.pv[i] = (dAdv > 0.0f) ? 0.5f * fminf(D.pv[i], 0.5f) * invH : 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------- pressure accelerations
+// computePressureAccel (TimeStepDFSPH.cpp:1299-1367) for particle i with
+// pressure field P.
+__device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float *__restrict__ P) {
+    const float4 xi = D.x[i];
+    const float pi = P[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float ps = pi + P[j];
+        if (fabsf(ps) > D.eps) {
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            ax += ps * (-D.V * (g * r.x));
+            ay += ps * (-D.V * (g * r.y));
+            az += ps * (-D.V * (g * r.z));
+        }
+    }
+    if (fabsf(pi) > D.eps) {
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned k = 0; k < cb; k++) {
+                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                ax += pi * (-xb.w * (g * r.x));
+                ay += pi * (-xb.w * (g * r.y));
+                az += pi * (-xb.w * (g * r.z));
+            }
+        } else if (D.bmode == 2) {
+            const float4 xb = D.vmb[i];
+            if (xb.w > 0.f) {
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                ax += pi * (-xb.w * (g * r.x));
+                ay += pi * (-xb.w * (g * r.y));
+                az += pi * (-xb.w * (g * r.z));
+            }
+        }
+    }
+    return make_float3(ax, ay, az);
+}
+
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
+    if (check_done && (PRESSURE ? D.ctl->done : D.ctl->done_v)) return;
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float3 a = accel_i(D, i, PRESSURE ? D.p : D.pv);
+    D.ap[i] = make_float4(a.x, a.y, a.z, 0.f);
+}
+
+// compute_aij_pj (TimeStepDFSPH.cpp:1370-1420) and the Jacobi update of
+// pressureSolveIteration / divergenceSolveIteration (:544-706), FP64 error
+// sum finished in the last block.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    if (check_done && (PRESSURE ? C->done : C->done_v)) return;
+    const unsigned i = gtid();
+    const double h = C->h;
+    double err = 0.0;
+    if (i < D.n) {
+        const float4 xi = D.x[i];
+        const float4 ai = D.ap[i];
+        float s = 0.f;
+        const unsigned cnt = D.cf[i];
+        for (unsigned k = 0; k < cnt; k++) {
+            const unsigned j = D.nf[(size_t)k * D.stride + i];
+            const float4 aj = D.ap[j];
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            s += ((ai.x - aj.x) * (g * r.x) + (ai.y - aj.y) * (g * r.y)) + (ai.z - aj.z) * (g * r.z);
+        }
+        s *= D.V;
+        unsigned nnb = cnt;
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            nnb += cb;
+            for (unsigned k = 0; k < cb; k++) {
+                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                s += xb.w * (((ai.x * (g * r.x) + ai.y * (g * r.y)) + ai.z * (g * r.z)));
+            }
+        } else if (D.bmode == 2) {
+            const float4 xb = D.vmb[i];
+            if (xb.w > 0.f) {
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                s += xb.w * (((ai.x * (g * r.x) + ai.y * (g * r.y)) + ai.z * (g * r.z)));
+            }
+        }
+        const float alpha = D.alpha[i];
+        if (PRESSURE) {
+            const float aij = s * (float)(h * h);
+            const float si = 1.0f - D.radv[i];
+            const float res = fminf(si - aij, 0.0f);
+            D.p[i] = fmaxf(D.p[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            err = -(double)D.rho0 * (double)res;
+        } else {
+            const float aij = s * (float)h;
+            const float si = -D.radv[i];
+            float res = fminf(si - aij, 0.0f);
+            if (nnb < 20u) res = 0.0f;
+            D.pv[i] = fmaxf(D.pv[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            err = -(double)D.rho0 * (double)res;
+        }
+    }
+    const double bs = block_sum(err, lds);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+    if (last_block(&C->ticket)) {
+        double t = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += D.partial[b];
+        t = block_sum(t, lds);
+        if (threadIdx.x == 0) {
+            const double avg = D.n ? t / (double)D.n : 0.0;
+            const bool finite = isfinite(avg);
+            if (!finite) C->flags |= 1;
+            if (PRESSURE) {
+                C->avg_err = avg;
+                C->it += 1;
+                const double eta = D.maxError * 0.01 * (double)D.rho0;
+                const bool chk = avg <= eta;
+                C->done = ((chk && C->it >= D.minIt) || C->it >= D.maxIt || !finite) ? 1 : 0;
+            } else {
+                C->avg_err_v = avg;
+                C->it_v += 1;
+                const double eta = (1.0 / h) * D.maxErrorV * 0.01 * (double)D.rho0;
+                const bool chk = avg <= eta;
+                C->done_v = ((chk && C->it_v >= 1) || C->it_v >= D.maxItV || !finite) ? 1 : 0;
+            }
+            C->ticket = 0;
+        }
+    }
+}
+
+// divergenceSolve / pressureSolve finalisation (TimeStepDFSPH.cpp:345-383,
+// 500-540): v += h a_p; the divergence solve also restores alpha *= h.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_finish(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float h = (float)D.ctl->h;
+    const float3 a = accel_i(D, i, PRESSURE ? D.p : D.pv);
+    D.ap[i] = make_float4(a.x, a.y, a.z, 0.f);
+    float4 v = D.v[i];
+    v.x += h * a.x;
+    v.y += h * a.y;
+    v.z += h * a.z;
+    D.v[i] = v;
+    if (!PRESSURE) D.alpha[i] *= h;
+}
+
+// p *= h^2 (which 0) or p_v *= h (which 1).
+__global__ __launch_bounds__(BLOCK) void k_scale(Dev D, float *q, int which) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const double h = D.ctl->h;
+    q[i] *= (which == 0) ? (float)(h * h) : (float)h;
+}
+
+// ----------------------------------------------------- non-pressure forces
+// clearAccelerations (TimeStep.cpp:35-50), Viscosity_Weiler2018::applyForces
+// (:928-940), XSPH::step (XSPH.cpp:138-216), updateTimeStepSizeCFL
+// (Simulation.cpp:415-493) and v += h a (TimeStepDFSPH.cpp:192-208).
+template <bool CLEAR, bool APPLY, bool XSPH, bool CFL, bool VUPD, bool SCALEPV>
+__global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    const unsigned i = gtid();
+    const double h = C->h;
+    const float hf = (float)h;
+    double vm = 0.0;
+    if (i < D.n) {
+        const float4 vi = D.v[i];
+        float4 ai = CLEAR ? make_float4(D.grav.x, D.grav.y, D.grav.z, 0.f) : D.a[i];
+        if (APPLY) {
+            float4 xs = D.cgx[i];
+            if (C->cg_done == 2) xs = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float dx = xs.x - vi.x, dy = xs.y - vi.y, dz = xs.z - vi.z;
+            const float ih = (float)(1.0 / h);
+            ai.x += ih * dx;
+            ai.y += ih * dy;
+            ai.z += ih * dz;
+            D.vdiff[i] = make_float4(dx, dy, dz, 0.f);
+        }
+        if (XSPH) {
+            const float4 xi = D.x[i];
+            const float invH = (float)(1.0 / h);
+            if (D.xsph != 0.f) {
+                const unsigned cnt = D.cf[i];
+                for (unsigned k = 0; k < cnt; k++) {
+                    const unsigned j = D.nf[(size_t)k * D.stride + i];
+                    const float4 vj = D.v[j];
+                    const float w = kernel_W(xi, D.x[j], D.K);
+                    const float s = invH * D.xsph * (D.mass / D.rho[j]);
+                    ai.x -= s * (vi.x - vj.x) * w;
+                    ai.y -= s * (vi.y - vj.y) * w;
+                    ai.z -= s * (vi.z - vj.z) * w;
+                }
+            }
+            if (D.xsphB != 0.f) {
+                const float rhoi = D.rho[i];
+                if (D.bmode == 1) {
+                    const unsigned cb = D.cb[i];
+                    for (unsigned k = 0; k < cb; k++) {
+                        const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                        const float w = kernel_W(xi, xb, D.K);
+                        const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
+                        ai.x += s * vi.x * w;
+                        ai.y += s * vi.y * w;
+                        ai.z += s * vi.z * w;
+                    }
+                } else if (D.bmode == 2) {
+                    const float4 xb = D.vmb[i];
+                    if (xb.w > 0.f) {
+                        const float w = kernel_W(xi, xb, D.K);
+                        const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
+                        ai.x += s * vi.x * w;
+                        ai.y += s * vi.y * w;
+                        ai.z += s * vi.z * w;
+                    }
+                }
+            }
+        }
+        if (CLEAR || APPLY || XSPH) D.a[i] = ai;
+        if (CFL) {
+            const double wx = (double)vi.x + (double)ai.x * h, wy = (double)vi.y + (double)ai.y * h,
+                         wz = (double)vi.z + (double)ai.z * h;
+            vm = (wx * wx + wy * wy) + wz * wz;
+        }
+        if (VUPD) {
+            const float hs = (float)C->h_step;
+            D.v2[i] = make_float4(vi.x + hs * ai.x, vi.y + hs * ai.y, vi.z + hs * ai.z, 0.f);
+        }
+        if (SCALEPV) D.pv[i] *= hf;
+    }
+    if (CFL) {
+        const double bm = block_max(vm, lds);
+        if (threadIdx.x == 0) D.partial[blockIdx.x] = bm;
+        if (last_block(&C->ticket)) {
+            double t = 0.0;
+            for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t = fmax(t, D.partial[b]);
+            t = block_max(t, lds);
+            if (threadIdx.x == 0) {
+                double maxVel = t;
+                if (!isfinite(maxVel)) C->flags |= 1;
+                if (maxVel < 1.0e-9) maxVel = 1.0e-9;
+                const double diameter = 2.0 * D.radius;
+                double hn = D.cflFactor * 0.4 * (diameter / sqrt(maxVel));
+                hn = fmin(hn, D.cflMax);
+                hn = fmax(hn, D.cflMin);
+                if (D.cflMethod == 2 && C->it != 0) {
+                    double hh = h;
+                    if (C->it > 10)
+                        hh *= 0.9;
+                    else if (C->it < 5)
+                        hh *= 1.1;
+                    hn = fmin(hh, hn);
+                }
+                C->h = hn;
+                C->ticket = 0;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------- pressure set-up
+// computeDensityAdv + warm start (TimeStepDFSPH.cpp:264-309, 1191-1242).
+__global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const double h = D.ctl->h;
+    const float4 xi = D.x[i], vi = D.v[i];
+    float delta = 0.f;
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float4 vj = D.v[j];
+        float3 r;
+        const float g = kernel_G(xi, D.x[j], D.K, r);
+        delta += ((vi.x - vj.x) * (g * r.x) + (vi.y - vj.y) * (g * r.y)) + (vi.z - vj.z) * (g * r.z);
+    }
+    delta *= D.V;
+    if (D.bmode == 1) {
+        const unsigned cb = D.cb[i];
+        for (unsigned k = 0; k < cb; k++) {
+            const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+            float3 r;
+            const float g = kernel_G(xi, xb, D.K, r);
+            delta += xb.w * ((vi.x * (g * r.x) + vi.y * (g * r.y)) + vi.z * (g * r.z));
+        }
+    } else if (D.bmode == 2) {
+        const float4 xb = D.vmb[i];
+        if (xb.w > 0.f) {
+            float3 r;
+            const float g = kernel_G(xi, xb, D.K, r);
+            delta += xb.w * ((vi.x * (g * r.x) + vi.y * (g * r.y)) + vi.z * (g * r.z));
+        }
+    }
+    const float radv = D.rho[i] / D.rho0 + (float)h * delta;
+    D.radv[i] = radv;
+    const float invH2 = (float)(1.0 / (h * h));
+    D.alpha[i] *= invH2;
+    D.p[i] = (radv > 1.0f) ? 0.5f * fminf(D.p[i], 0.00025f) * invH2 : 0.0f;
+}
+
+// p *= h^2 and the compensated position update x += h_step v (Kahan), then
+// t += h_step (TimeStepDFSPH.cpp:220-248, 369-383).
+template <bool SCALEP, bool UPDX>
+__global__ __launch_bounds__(BLOCK) void k_final(Dev D) {
+    const unsigned i = gtid();
+    Ctl *C = D.ctl;
+    if (UPDX && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the time advances once per launch; every block reads h_step only
+        C->time += C->h_step;
+    }
+    if (i >= D.n) return;
+    if (SCALEP) {
+        const double h = C->h;
+        D.p[i] *= (float)(h * h);
+    }
+    if (UPDX) {
+        const float hs = (float)C->h_step;
+        const float4 v = D.v[i];
+        float4 x = D.x[i], c = D.xc[i];
+        const float yx = hs * v.x - c.x, yy = hs * v.y - c.y, yz = hs * v.z - c.z;
+        const float tx = x.x + yx, ty = x.y + yy, tz = x.z + yz;
+        c.x = (tx - x.x) - yx;
+        c.y = (ty - x.y) - yy;
+        c.z = (tz - x.z) - yz;
+        x.x = tx;
+        x.y = ty;
+        x.z = tz;
+        D.x[i] = x;
+        D.xc[i] = c;
+    }
+}
+
+// ------------------------------------------------------------ Weiler2018
+// diagonalMatrixElement, block form (Viscosity_Weiler2018.cpp:605-742),
+// inverted for the block-Jacobi preconditioner (MatrixFreeSolver.h:231-245),
+// and the warm start g = v + vDiff (:1244-1251).
+__device__ __forceinline__ void sym_inverse(const double m[6], double inv[6]) {
+    // m = (xx, xy, xz, yy, yz, zz)
+    const double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5];
+    const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+    const double det = a * c00 + b * c01 + c * c02;
+    const double id = 1.0 / det;
+    inv[0] = c00 * id;
+    inv[1] = c01 * id;
+    inv[2] = c02 * id;
+    inv[3] = (a * f - c * c) * id;
+    inv[4] = (b * c - a * e) * id;
+    inv[5] = (a * d - b * b) * id;
+}
+
+// Tangent frame of MathFunctions::getOrthogonalVectors (MathFunctions.cpp:234-247).
+__device__ __forceinline__ void ortho(const float3 &n, float3 &t1, float3 &t2) {
+    float3 v = make_float3(1.f, 0.f, 0.f);
+    if (fabsf(n.x) > 0.999f) v = make_float3(0.f, 1.f, 0.f);
+    t1 = make_float3(n.y * v.z - n.z * v.y, n.z * v.x - n.x * v.z, n.x * v.y - n.y * v.x);
+    t2 = make_float3(n.y * t1.z - n.z * t1.y, n.z * t1.x - n.x * t1.z, n.x * t1.y - n.y * t1.x);
+    const float l1 = sqrtf(sqn(t1.x, t1.y, t1.z)), l2 = sqrtf(sqn(t2.x, t2.y, t2.z));
+    t1 = make_float3(t1.x / l1, t1.y / l1, t1.z / l1);
+    t2 = make_float3(t2.x / l2, t2.y / l2, t2.z / l2);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_weiler_diag(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float4 xi = D.x[i];
+    const float rhoi = D.rho[i];
+    const float dd = 10.0f;
+    float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        float3 r;
+        const float g = kernel_G(xi, D.x[j], D.K, r);
+        const float s = dd * D.mu * (D.mass / D.rho[j]) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+        m[0] += s * (gx * r.x);
+        m[1] += s * (gx * r.y);
+        m[2] += s * (gx * r.z);
+        m[3] += s * (gy * r.y);
+        m[4] += s * (gy * r.z);
+        m[5] += s * (gz * r.z);
+    }
+    if (D.mub != 0.f) {
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned k = 0; k < cb; k++) {
+                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                const float s = dd * D.mub * (D.rho0 * xb.w / rhoi) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+                const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+                m[0] += s * (gx * r.x);
+                m[1] += s * (gx * r.y);
+                m[2] += s * (gx * r.z);
+                m[3] += s * (gy * r.y);
+                m[4] += s * (gy * r.z);
+                m[5] += s * (gz * r.z);
+            }
+        } else if (D.bmode == 2) {
+            const float4 xb = D.vmb[i];
+            if (xb.w > 0.f) {
+                float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
+                const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
+                if (nl > 0.0001f) {
+                    nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
+                    float3 t1, t2;
+                    ortho(nrm, t1, t2);
+                    const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
+                    for (int q = 0; q < 4; q++) {
+                        const float3 t = q < 2 ? t1 : t2;
+                        const float sg = (q % 2 == 0) ? -1.f : 1.f;
+                        const float4 xs = make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist,
+                                                      xb.z + sg * t.z * dist, 0.f);
+                        float3 r;
+                        const float g = kernel_G(xi, xs, D.K, r);
+                        const float s = dd * D.mub * vol / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+                        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+                        m[0] += s * (gx * r.x);
+                        m[1] += s * (gx * r.y);
+                        m[2] += s * (gx * r.z);
+                        m[3] += s * (gy * r.y);
+                        m[4] += s * (gy * r.z);
+                        m[5] += s * (gz * r.z);
+                    }
+                }
+            }
+        }
+    }
+    const double f = (double)D.ctl->h / (double)rhoi;
+    double md[6] = {1.0 - f * m[0], -f * m[1], -f * m[2], 1.0 - f * m[3], -f * m[4], 1.0 - f * m[5]};
+    double inv[6];
+    sym_inverse(md, inv);
+    D.minv[2 * i] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
+    D.minv[2 * i + 1] = make_float4((float)inv[4], (float)inv[5], 0.f, 0.f);
+    const float4 v = D.v[i], vd = D.vdiff[i];
+    D.cgx[i] = make_float4(v.x + vd.x, v.y + vd.y, v.z + vd.z, 0.f);
+}
+
+__device__ __forceinline__ float3 minv_apply(const Dev &D, unsigned i, const float3 &r) {
+    const float4 a = D.minv[2 * i], b = D.minv[2 * i + 1];
+    return make_float3((a.x * r.x + a.y * r.y) + a.z * r.z, (a.y * r.x + a.w * r.y) + b.x * r.z,
+                       (a.z * r.x + b.x * r.y) + b.y * r.z);
+}
+
+// matrixVecProd (Viscosity_Weiler2018.cpp:271-441): (A vec)_i.
+__device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, const float4 *__restrict__ vec) {
+    const float4 xi = D.x[i];
+    const float4 vi = vec[i];
+    const float rhoi = D.rho[i];
+    const float dd = 10.0f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float4 vj = vec[j];
+        float3 r;
+        const float g = kernel_G(xi, D.x[j], D.K, r);
+        const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
+        const float s = dd * D.mu * (D.mass / D.rho[j]) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+        ax += s * (g * r.x);
+        ay += s * (g * r.y);
+        az += s * (g * r.z);
+    }
+    if (D.mub != 0.f) {
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned k = 0; k < cb; k++) {
+                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                const float dot = (vi.x * r.x + vi.y * r.y) + vi.z * r.z;
+                const float s = dd * D.mub * (D.rho0 * xb.w / rhoi) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+                ax += s * (g * r.x);
+                ay += s * (g * r.y);
+                az += s * (g * r.z);
+            }
+        } else if (D.bmode == 2) {
+            const float4 xb = D.vmb[i];
+            if (xb.w > 0.f) {
+                float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
+                const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
+                if (nl > 0.0001f) {
+                    nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
+                    float3 t1, t2;
+                    ortho(nrm, t1, t2);
+                    const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
+                    for (int q = 0; q < 4; q++) {
+                        const float3 t = q < 2 ? t1 : t2;
+                        const float sg = (q % 2 == 0) ? -1.f : 1.f;
+                        const float4 xs = make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist,
+                                                      xb.z + sg * t.z * dist, 0.f);
+                        float3 r;
+                        const float g = kernel_G(xi, xs, D.K, r);
+                        const float dot = (vi.x * r.x + vi.y * r.y) + vi.z * r.z;
+                        const float s = dd * D.mub * vol * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+                        ax += s * (g * r.x);
+                        ay += s * (g * r.y);
+                        az += s * (g * r.z);
+                    }
+                }
+            }
+        }
+    }
+    const float f = (float)D.ctl->h / rhoi;
+    return make_float3(vi.x - f * ax, vi.y - f * ay, vi.z - f * az);
+}
+
+// Initial residual r = b - A x (b = v for static boundaries), z = M^-1 r,
+// p = z; last block sets up Eigen's CG state (ConjugateGradient.h:43-70).
+__global__ __launch_bounds__(BLOCK) void k_cg_init(Dev D) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    const unsigned i = gtid();
+    double b2 = 0.0, r2 = 0.0, rz = 0.0;
+    if (i < D.n) {
+        const float3 Ax = weiler_Av(D, i, D.cgx);
+        const float4 b = D.v[i];
+        const float3 r = make_float3(b.x - Ax.x, b.y - Ax.y, b.z - Ax.z);
+        const float3 z = minv_apply(D, i, r);
+        D.cgr[i] = make_float4(r.x, r.y, r.z, 0.f);
+        D.cgz[i] = make_float4(z.x, z.y, z.z, 0.f);
+        D.cgp[i] = make_float4(z.x, z.y, z.z, 0.f);
+        b2 = ((double)b.x * b.x + (double)b.y * b.y) + (double)b.z * b.z;
+        r2 = ((double)r.x * r.x + (double)r.y * r.y) + (double)r.z * r.z;
+        rz = ((double)r.x * z.x + (double)r.y * z.y) + (double)r.z * z.z;
+    }
+    const double s0 = block_sum(b2, lds), s1 = block_sum(r2, lds), s2 = block_sum(rz, lds);
+    if (threadIdx.x == 0) {
+        D.partial[3 * blockIdx.x] = s0;
+        D.partial[3 * blockIdx.x + 1] = s1;
+        D.partial[3 * blockIdx.x + 2] = s2;
+    }
+    if (last_block(&C->ticket)) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            t0 += D.partial[3 * b];
+            t1 += D.partial[3 * b + 1];
+            t2 += D.partial[3 * b + 2];
+        }
+        t0 = block_sum(t0, lds);
+        t1 = block_sum(t1, lds);
+        t2 = block_sum(t2, lds);
+        if (threadIdx.x == 0) {
+            C->cg_rhs2 = t0;
+            C->cg_r2 = t1;
+            C->cg_absNew = t2;
+            C->cg_it = 0;
+            if (t0 == 0.0) {
+                C->cg_done = 2;  // x = 0, no iteration
+            } else {
+                const double thr = fmax(D.viscoTol * D.viscoTol * t0, DBL_MIN);
+                C->cg_thresh = thr;
+                C->cg_done = (t1 < thr || D.viscoMaxIt <= 0) ? 1 : 0;
+            }
+            C->ticket = 0;
+        }
+    }
+}
+
+// tmp = A p and alpha = absNew / (p . tmp).
+__global__ __launch_bounds__(BLOCK) void k_cg_matvec(Dev D) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->cg_done) return;
+    const unsigned i = gtid();
+    double pt = 0.0;
+    if (i < D.n) {
+        const float3 t = weiler_Av(D, i, D.cgp);
+        D.cgt[i] = make_float4(t.x, t.y, t.z, 0.f);
+        const float4 p = D.cgp[i];
+        pt = ((double)p.x * t.x + (double)p.y * t.y) + (double)p.z * t.z;
+    }
+    const double bs = block_sum(pt, lds);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+    if (last_block(&C->ticket)) {
+        double t = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += D.partial[b];
+        t = block_sum(t, lds);
+        if (threadIdx.x == 0) {
+            C->cg_alpha = C->cg_absNew / t;
+            C->ticket = 0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha tmp, z = M^-1 r; convergence test on |r|^2 and
+// beta = (r.z)_new / (r.z)_old (ConjugateGradient.h:71-88).
+__global__ __launch_bounds__(BLOCK) void k_cg_update(Dev D) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->cg_done) return;
+    const unsigned i = gtid();
+    const float alpha = (float)C->cg_alpha;
+    double r2 = 0.0, rz = 0.0;
+    if (i < D.n) {
+        const float4 p = D.cgp[i], t = D.cgt[i];
+        float4 x = D.cgx[i], r = D.cgr[i];
+        x.x += alpha * p.x;
+        x.y += alpha * p.y;
+        x.z += alpha * p.z;
+        r.x -= alpha * t.x;
+        r.y -= alpha * t.y;
+        r.z -= alpha * t.z;
+        D.cgx[i] = x;
+        D.cgr[i] = r;
+        const float3 z = minv_apply(D, i, make_float3(r.x, r.y, r.z));
+        D.cgz[i] = make_float4(z.x, z.y, z.z, 0.f);
+        r2 = ((double)r.x * r.x + (double)r.y * r.y) + (double)r.z * r.z;
+        rz = ((double)r.x * z.x + (double)r.y * z.y) + (double)r.z * z.z;
+    }
+    const double s1 = block_sum(r2, lds), s2 = block_sum(rz, lds);
+    if (threadIdx.x == 0) {
+        D.partial[2 * blockIdx.x] = s1;
+        D.partial[2 * blockIdx.x + 1] = s2;
+    }
+    if (last_block(&C->ticket)) {
+        double t1 = 0.0, t2 = 0.0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            t1 += D.partial[2 * b];
+            t2 += D.partial[2 * b + 1];
+        }
+        t1 = block_sum(t1, lds);
+        t2 = block_sum(t2, lds);
+        if (threadIdx.x == 0) {
+            C->cg_r2 = t1;
+            if (!isfinite(t1)) C->flags |= 1;
+            if (t1 < C->cg_thresh || !isfinite(t1)) {
+                C->cg_done = 1;
+            } else {
+                const double absOld = C->cg_absNew;
+                C->cg_absNew = t2;
+                C->cg_beta = t2 / absOld;
+                C->cg_it += 1;
+                if (C->cg_it >= D.viscoMaxIt) C->cg_done = 3;  // p no longer needed
+            }
+            C->ticket = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cg_pupdate(Dev D) {
+    Ctl *C = D.ctl;
+    if (C->cg_done) return;
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float beta = (float)C->cg_beta;
+    const float4 z = D.cgz[i];
+    float4 p = D.cgp[i];
+    p.x = z.x + beta * p.x;
+    p.y = z.y + beta * p.y;
+    p.z = z.z + beta * p.z;
+    D.cgp[i] = p;
+}
+
+// Akinci2012 boundary volume V_b = 1 / (W0 + sum_b' W) (BoundaryModel_Akinci2012.cpp:48-75).
+__global__ __launch_bounds__(BLOCK) void k_boundary_volume(float4 *bx, unsigned nb, const unsigned *nbr,
+                                                           const unsigned *cnt, unsigned stride, KernelConst K) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const float4 xi = bx[i];
+    float delta = K.W0;
+    const unsigned c = cnt[i];
+    for (unsigned k = 0; k < c; k++) delta += kernel_W(xi, bx[nbr[(size_t)k * stride + i]], K);
+    bx[i].w = 1.0f / delta;
+}
+
+// ==================================================================== host
+DFSPH::DFSPH(const sph_dfsph_params_t &p) : P(p) {
+    require_device();
+    SPH_REQUIRE(P.particle_radius > 0 && P.density0 > 0, SPH_ERR_INVALID, "particle_radius and density0 must be > 0");
+    r = P.particle_radius;
+    hs = 4.0 * r;
+    const double diam = 2.0 * r;
+    V = 0.8 * diam * diam * diam;
+    mass = V * P.density0;
+    K = make_kernel_const(hs);
+    SPH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    SPH_HIP(hipHostMalloc((void **)&hctl_, sizeof(Ctl), hipHostMallocDefault));
+    std::memset(hctl_, 0, sizeof(Ctl));
+    hctl_->h = P.time_step_size;
+    hctl_->h_step = P.time_step_size;
+    ctl_.reserve(1);
+    SPH_HIP(hipMemcpyAsync(ctl_.get(), hctl_, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+    SPH_HIP(hipStreamSynchronize(stream));
+}
+
+DFSPH::~DFSPH() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (hctl_) (void)hipHostFree(hctl_);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void DFSPH::set_fluid(const float *x, const float *v, size_t n) {
+    SPH_REQUIRE(n < (size_t)0x7fffffff, SPH_ERR_INVALID, "too many particles");
+    hx_.assign(x, x + 3 * n);
+    if (v)
+        hv_.assign(v, v + 3 * n);
+    else
+        hv_.assign(3 * n, 0.f);
+    n_ = n;
+    initialized_ = false;
+}
+
+void DFSPH::add_boundary(const float *x, size_t n) {
+    hbx_.insert(hbx_.end(), x, x + 3 * n);
+    nb_ += n;
+    initialized_ = false;
+}
+
+void DFSPH::set_volume_map_box(const double lo[3], const double hi[3], int invert, double thickness,
+                               const unsigned res[3]) {
+    for (int c = 0; c < 3; c++) {
+        vm_lo_[c] = lo[c];
+        vm_hi_[c] = hi[c];
+        vm_res_[c] = res[c];
+    }
+    vm_invert_ = invert;
+    vm_thick_ = thickness;
+    vm_requested_ = true;
+    initialized_ = false;
+}
+
+Dev DFSPH::dev() const {
+    Dev D{};
+    D.n = (unsigned)n_;
+    D.stride = stride_;
+    D.x = x_.get();
+    D.v = v_.get();
+    D.v2 = v2_.get();
+    D.a = a_.get();
+    D.ap = ap_.get();
+    D.vdiff = vdiff_.get();
+    D.xc = xc_.get();
+    D.vmb = vmb_.get();
+    D.rho = rho_.get();
+    D.alpha = alpha_.get();
+    D.radv = radv_.get();
+    D.p = p_.get();
+    D.pv = pv_.get();
+    D.nf = nf_.get();
+    D.cf = cf_.get();
+    D.nb = nb_l_.get();
+    D.cb = cb_.get();
+    D.bx = bx_.get();
+    const int bm = P.boundary_handling_method;
+    D.bmode = (bm == SPH_BOUNDARY_AKINCI2012 && nb_ > 0) ? 1 : ((bm == SPH_BOUNDARY_BENDER2019 && vmap.built) ? 2 : 0);
+    D.V = (float)V;
+    D.rho0 = (float)P.density0;
+    D.eps = 1.0e-5f;
+    D.mass = (float)mass;
+    D.grav = make_float3((float)P.gravity[0], (float)P.gravity[1], (float)P.gravity[2]);
+    D.K = K;
+    D.ctl = ctl_.get();
+    D.partial = partial_.get();
+    D.minIt = P.min_iterations;
+    D.maxIt = P.max_iterations;
+    D.maxItV = P.max_iterations_v;
+    D.maxError = P.max_error;
+    D.maxErrorV = P.max_error_v;
+    D.xsph = (float)P.xsph;
+    D.xsphB = (float)P.xsph_boundary;
+    D.cflMethod = P.cfl_method;
+    D.cflFactor = P.cfl_factor;
+    D.cflMin = P.cfl_min_time_step_size;
+    D.cflMax = P.cfl_max_time_step_size;
+    D.radius = r;
+    D.minv = minv_.get();
+    D.cgx = cgx_.get();
+    D.cgr = cgr_.get();
+    D.cgp = cgp_.get();
+    D.cgz = cgz_.get();
+    D.cgt = cgt_.get();
+    D.mu = (float)(P.viscosity * P.density0);
+    D.mub = (float)(P.viscosity_boundary * P.density0);
+    D.hs2 = (float)(hs * hs);
+    D.viscoMaxIt = P.visco_max_iter;
+    D.viscoTol = P.visco_max_error;
+    return D;
+}
+
+void DFSPH::read_ctl() {
+    SPH_HIP(hipMemcpyAsync(hctl_, ctl_.get(), sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    SPH_HIP(hipStreamSynchronize(stream));
+}
+
+void DFSPH::upload_ctl_field(size_t off, const void *src, size_t bytes) {
+    SPH_HIP(hipStreamSynchronize(stream));
+    std::memcpy(reinterpret_cast<char *>(hctl_) + off, src, bytes);
+    SPH_HIP(hipMemcpyAsync(reinterpret_cast<char *>(ctl_.get()) + off, reinterpret_cast<char *>(hctl_) + off, bytes,
+                           hipMemcpyHostToDevice, stream));
+    SPH_HIP(hipStreamSynchronize(stream));
+}
+
+void DFSPH::init() {
+    SPH_REQUIRE(n_ > 0, SPH_ERR_STATE, "set_fluid must be called first");
+    const size_t n = n_;
+    stride_ = (unsigned)((n + 255) / 256 * 256);
+    const size_t S = stride_;
+    for (DBuf<float4> *b : {&x_, &v_, &v2_, &a_, &ap_, &vdiff_, &xc_, &vmb_, &x2_, &vdiff2_, &xc2_}) b->reserve(S);
+    for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_, &p2_, &pv2_}) b->reserve(S);
+    id_.reserve(S);
+    id2_.reserve(S);
+    cf_.reserve(S);
+    cb_.reserve(S);
+    partial_.reserve(3 * (size_t)nblocks(S) + 64);
+    {
+        std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        std::vector<unsigned> ids(n);
+        for (size_t i = 0; i < n; i++) {
+            hx[i] = make_float4(hx_[3 * i], hx_[3 * i + 1], hx_[3 * i + 2], 0.f);
+            hv[i] = make_float4(hv_[3 * i], hv_[3 * i + 1], hv_[3 * i + 2], 0.f);
+            ids[i] = (unsigned)i;
+        }
+        std::vector<float> zf(n, 0.f);
+        SPH_HIP(hipMemcpy(x_.get(), hx.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(v_.get(), hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        for (DBuf<float4> *b : {&a_, &ap_, &vdiff_, &xc_, &vmb_})
+            SPH_HIP(hipMemcpy(b->get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_})
+            SPH_HIP(hipMemcpy(b->get(), zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(id_.get(), ids.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    // grid over fluid, boundary particles and the volume-map box
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    auto ext = [&](const std::vector<float> &a) {
+        for (size_t i = 0; i + 2 < a.size(); i += 3)
+            for (int c = 0; c < 3; c++) {
+                lo[c] = std::min(lo[c], (double)a[i + c]);
+                hi[c] = std::max(hi[c], (double)a[i + c]);
+            }
+    };
+    ext(hx_);
+    ext(hbx_);
+    if (vm_requested_)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], vm_lo_[c]);
+            hi[c] = std::max(hi[c], vm_hi_[c]);
+        }
+    for (int c = 0; c < 3; c++) {
+        lo[c] -= 2.0 * hs;
+        hi[c] += 2.0 * hs;
+    }
+    grid_ = make_grid(lo, hi, hs);
+    // Akinci boundary: sort once, volumes from boundary-boundary neighbours
+    if (nb_ > 0) {
+        const size_t nbp = nb_;
+        const unsigned bstride = (unsigned)((nbp + 255) / 256 * 256);
+        DBuf<float4> tmp;
+        tmp.reserve(nbp);
+        std::vector<float4> hb(nbp);
+        for (size_t i = 0; i < nbp; i++) hb[i] = make_float4(hbx_[3 * i], hbx_[3 * i + 1], hbx_[3 * i + 2], 0.f);
+        SPH_HIP(hipMemcpy(tmp.get(), hb.data(), nbp * sizeof(float4), hipMemcpyHostToDevice));
+        bsort_.sort(stream, tmp.get(), (unsigned)nbp, grid_, timer);
+        bx_.reserve(nbp);
+        permute_f4(stream, bsort_.perm.get(), (unsigned)nbp, tmp.get(), bx_.get());
+        bperm_.reserve(nbp);
+        SPH_HIP(hipMemcpyAsync(bperm_.get(), bsort_.perm.get(), nbp * sizeof(unsigned), hipMemcpyDeviceToDevice,
+                               stream));
+        bstart_.reserve((size_t)grid_.ncells + 1);
+        SPH_HIP(hipMemcpyAsync(bstart_.get(), bsort_.start.get(), ((size_t)grid_.ncells + 1) * sizeof(unsigned),
+                               hipMemcpyDeviceToDevice, stream));
+        unsigned cap = 64;
+        DBuf<unsigned> bl, bc;
+        bc.reserve(bstride);
+        for (;;) {
+            bl.reserve((size_t)cap * bstride);
+            k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 2);
+            build_lists_colmajor(stream, bx_.get(), (unsigned)nbp, bx_.get(), bstart_.get(), grid_, K.h2, true,
+                                 bl.get(), bstride, cap, bc.get(), &ctl_.get()->maxcnt_f, timer, "boundary_lists");
+            read_ctl();
+            if (hctl_->maxcnt_f <= cap) break;
+            cap = (hctl_->maxcnt_f + 15) / 16 * 16;
+        }
+        k_boundary_volume<<<nblocks(nbp), BLOCK, 0, stream>>>(bx_.get(), (unsigned)nbp, bl.get(), bc.get(), bstride,
+                                                              K);
+        SPH_HIP(hipGetLastError());
+        SPH_HIP(hipStreamSynchronize(stream));
+    }
+    if (vm_requested_ && P.boundary_handling_method == SPH_BOUNDARY_BENDER2019)
+        vmap.build(stream, vm_lo_, vm_hi_, vm_invert_, vm_thick_, vm_res_, hs);
+    if (P.viscosity_method == SPH_VISCOSITY_WEILER2018) {
+        minv_.reserve(2 * S);
+        for (DBuf<float4> *b : {&cgx_, &cgr_, &cgp_, &cgz_, &cgt_}) b->reserve(S);
+    }
+    initialized_ = true;
+    have_lists_ = false;
+}
+
+void DFSPH::build_lists() {
+    const Dev D = dev();
+    for (;;) {
+        nf_.reserve((size_t)capF_ * stride_);
+        if (D.bmode == 1) nb_l_.reserve((size_t)capB_ * stride_);
+        k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 2);
+        build_lists_colmajor(stream, x_.get(), (unsigned)n_, x_.get(), fsort_.start.get(), grid_, K.h2, true,
+                             nf_.get(), stride_, capF_, cf_.get(), &ctl_.get()->maxcnt_f, timer, "fluid_lists");
+        if (P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0)
+            build_lists_colmajor(stream, x_.get(), (unsigned)n_, bx_.get(), bstart_.get(), grid_, K.h2, false,
+                                 nb_l_.get(), stride_, capB_, cb_.get(), &ctl_.get()->maxcnt_b, timer,
+                                 "boundary_lists");
+        read_ctl();
+        bool again = false;
+        if (hctl_->maxcnt_f > capF_) {
+            capF_ = (hctl_->maxcnt_f + 16) / 16 * 16;
+            again = true;
+        }
+        if (hctl_->maxcnt_b > capB_) {
+            capB_ = (hctl_->maxcnt_b + 16) / 16 * 16;
+            again = true;
+        }
+        if (!again) break;
+    }
+    have_lists_ = true;
+}
+
+void DFSPH::neighborhood_search() {
+    if (!initialized_) init();
+    k_set_hstep<<<1, 1, 0, stream>>>(ctl_.get());
+    const unsigned n = (unsigned)n_;
+    fsort_.sort(stream, x_.get(), n, grid_, timer);
+    {
+        TimedScope ts(timer, stream, "permute");
+        const unsigned *perm = fsort_.perm.get();
+        permute_f4(stream, perm, n, x_.get(), x2_.get());
+        x_.swap(x2_);
+        permute_f4(stream, perm, n, v_.get(), v2_.get());
+        v_.swap(v2_);
+        permute_f4(stream, perm, n, xc_.get(), xc2_.get());
+        xc_.swap(xc2_);
+        permute_f4(stream, perm, n, vdiff_.get(), vdiff2_.get());
+        vdiff_.swap(vdiff2_);
+        permute_f1(stream, perm, n, p_.get(), p2_.get());
+        p_.swap(p2_);
+        permute_f1(stream, perm, n, pv_.get(), pv2_.get());
+        pv_.swap(pv2_);
+        permute_u1(stream, perm, n, id_.get(), id2_.get());
+        id_.swap(id2_);
+    }
+    build_lists();
+}
+
+void DFSPH::volume_maps() {
+    if (P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 || !vmap.built) return;
+    sample_volume_map(stream, vmap.d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(), hs, r, timer);
+}
+
+#define LAUNCH(name, kern, ...)                                   \
+    do {                                                          \
+        TimedScope ts_(timer, stream, name);                      \
+        kern<<<nblocks(n_), BLOCK, 0, stream>>>(__VA_ARGS__);     \
+        SPH_HIP(hipGetLastError());                               \
+    } while (0)
+
+void DFSPH::sweep_init(bool dens, bool fact, bool div) {
+    const Dev D = dev();
+    if (dens && fact && div)
+        LAUNCH("density_factor_divsetup", (k_sweep_init<true, true, true>), D);
+    else if (dens && fact)
+        LAUNCH("density_factor", (k_sweep_init<true, true, false>), D);
+    else if (dens)
+        LAUNCH("density", (k_sweep_init<true, false, false>), D);
+    else if (fact)
+        LAUNCH("factor", (k_sweep_init<false, true, false>), D);
+    else if (div)
+        LAUNCH("divergence_setup", (k_sweep_init<false, false, true>), D);
+}
+
+void DFSPH::div_iteration(bool force) {
+    const Dev D = dev();
+    LAUNCH("pressure_accel_v", k_accel<false>, D, force ? 0 : 1);
+    LAUNCH("jacobi_v", k_jacobi<false>, D, force ? 0 : 1);
+}
+
+void DFSPH::pressure_iteration(bool force) {
+    const Dev D = dev();
+    LAUNCH("pressure_accel", k_accel<true>, D, force ? 0 : 1);
+    LAUNCH("jacobi", k_jacobi<true>, D, force ? 0 : 1);
+}
+
+void DFSPH::div_finish(bool scale_pv) {
+    const Dev D = dev();
+    LAUNCH("divergence_finish", k_finish<false>, D);
+    if (scale_pv) LAUNCH("scale", k_scale, D, pv_.get(), 1);
+}
+
+void DFSPH::pressure_setup() {
+    const Dev D = dev();
+    LAUNCH("pressure_setup", k_psetup, D);
+}
+
+void DFSPH::pressure_finish(bool finalize_too) {
+    const Dev D = dev();
+    LAUNCH("pressure_finish", k_finish<true>, D);
+    if (finalize_too) finalize(true, true);
+}
+
+void DFSPH::finalize(bool scale_p, bool update_x) {
+    const Dev D = dev();
+    if (scale_p && update_x)
+        LAUNCH("finalize", (k_final<true, true>), D);
+    else if (scale_p)
+        LAUNCH("finalize", (k_final<true, false>), D);
+    else if (update_x)
+        LAUNCH("finalize", (k_final<false, true>), D);
+}
+
+void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, bool scale_pv) {
+    const Dev D = dev();
+    // the step uses the full fusion; single stages use the one-flag forms
+    if (clear && !apply && !xsph && !cfl && !vupd && !scale_pv)
+        LAUNCH("clear_accel", (k_post<true, false, false, false, false, false>), D);
+    else if (!clear && apply && !xsph && !cfl && !vupd)
+        LAUNCH("viscosity_apply", (k_post<false, true, false, false, false, false>), D);
+    else if (!clear && !apply && xsph && !cfl && !vupd)
+        LAUNCH("xsph", (k_post<false, false, true, false, false, false>), D);
+    else if (!clear && !apply && !xsph && cfl && !vupd)
+        LAUNCH("cfl", (k_post<false, false, false, true, false, false>), D);
+    else if (!clear && !apply && !xsph && !cfl && vupd)
+        LAUNCH("update_velocity", (k_post<false, false, false, false, true, false>), D);
+    else {
+        // step: clear, [apply], [xsph], [cfl], v update, [pv scale]
+#define POSTV(A, X, C, S) LAUNCH("post_forces", (k_post<true, A, X, C, true, S>), D)
+        const int key = (apply ? 8 : 0) | (xsph ? 4 : 0) | (cfl ? 2 : 0) | (scale_pv ? 1 : 0);
+        switch (key) {
+            case 0: POSTV(false, false, false, false); break;
+            case 1: POSTV(false, false, false, true); break;
+            case 2: POSTV(false, false, true, false); break;
+            case 3: POSTV(false, false, true, true); break;
+            case 4: POSTV(false, true, false, false); break;
+            case 5: POSTV(false, true, false, true); break;
+            case 6: POSTV(false, true, true, false); break;
+            case 7: POSTV(false, true, true, true); break;
+            case 8: POSTV(true, false, false, false); break;
+            case 9: POSTV(true, false, false, true); break;
+            case 10: POSTV(true, false, true, false); break;
+            case 11: POSTV(true, false, true, true); break;
+            case 12: POSTV(true, true, false, false); break;
+            case 13: POSTV(true, true, false, true); break;
+            case 14: POSTV(true, true, true, false); break;
+            default: POSTV(true, true, true, true); break;
+        }
+#undef POSTV
+    }
+    if (vupd) v_.swap(v2_);
+}
+
+int DFSPH::run_solver(bool pressure) {
+    const int maxIt = pressure ? P.max_iterations : P.max_iterations_v;
+    int launched = 0;
+    int chunk = std::max(1, pressure ? std::max(lastIt_, P.min_iterations) : lastItV_);
+    for (;;) {
+        for (int c = 0; c < chunk && launched < maxIt; c++, launched++) {
+            if (pressure)
+                pressure_iteration(false);
+            else
+                div_iteration(false);
+        }
+        read_ctl();
+        const int done = pressure ? hctl_->done : hctl_->done_v;
+        if (done || launched >= maxIt) break;
+        chunk = 1;
+    }
+    return pressure ? hctl_->it : hctl_->it_v;
+}
+
+int DFSPH::run_cg() {
+    const Dev D = dev();
+    LAUNCH("weiler_diag", k_weiler_diag, D);
+    k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 3);
+    LAUNCH("cg_init", k_cg_init, D);
+    int launched = 0, chunk = std::max(1, lastCg_ + 1);
+    for (;;) {
+        for (int c = 0; c < chunk && launched < P.visco_max_iter; c++, launched++) {
+            LAUNCH("cg_matvec", k_cg_matvec, D);
+            LAUNCH("cg_update", k_cg_update, D);
+            LAUNCH("cg_pupdate", k_cg_pupdate, D);
+        }
+        read_ctl();
+        if (hctl_->cg_done || launched >= P.visco_max_iter) break;
+        chunk = 2;
+    }
+    return hctl_->cg_it;
+}
+
+void DFSPH::viscosity(bool apply_separately) {
+    if (P.viscosity_method != SPH_VISCOSITY_WEILER2018 || n_ == 0) return;
+    cgLast_ = run_cg();
+    lastCg_ = cgLast_;
+    totVisco_ += cgLast_;
+    if (apply_separately) post_forces(false, true, false, false, false, false);
+}
+
+void DFSPH::step() {
+    if (!initialized_) init();
+    neighborhood_search();
+    volume_maps();
+    const bool div = P.enable_divergence_solver != 0;
+    if (div) k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
+    sweep_init(true, true, div);
+    if (div) {
+        itVLast_ = run_solver(false);
+        lastItV_ = std::max(1, itVLast_);
+        totItV_ += itVLast_;
+        div_finish(false);
+    } else {
+        itVLast_ = 0;
+    }
+    const bool weiler = P.viscosity_method == SPH_VISCOSITY_WEILER2018;
+    if (weiler) viscosity(false);
+    post_forces(true, weiler, P.xsph != 0.0 || P.xsph_boundary != 0.0, P.cfl_method == 1 || P.cfl_method == 2, true,
+                div);
+    k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
+    pressure_setup();
+    itLast_ = run_solver(true);
+    lastIt_ = std::max(P.min_iterations, itLast_);
+    totIt_ += itLast_;
+    pressure_finish(true);
+    steps_++;
+}
+
+void DFSPH::stage(int s) {
+    if (!initialized_) init();
+    if (s != SPH_STAGE_NEIGHBORHOOD_SEARCH) SPH_REQUIRE(have_lists_, SPH_ERR_STATE, "run the neighbourhood search first");
+    switch (s) {
+        case SPH_STAGE_NEIGHBORHOOD_SEARCH: neighborhood_search(); break;
+        case SPH_STAGE_VOLUME_MAPS: volume_maps(); break;
+        case SPH_STAGE_DENSITIES: sweep_init(true, false, false); break;
+        case SPH_STAGE_DFSPH_FACTOR: sweep_init(false, true, false); break;
+        case SPH_STAGE_DIVERGENCE_SETUP:
+            k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
+            sweep_init(false, false, true);
+            break;
+        case SPH_STAGE_DIVERGENCE_ITER: div_iteration(true); break;
+        case SPH_STAGE_DIVERGENCE_FINISH: div_finish(true); break;
+        case SPH_STAGE_DIVERGENCE_SOLVE:
+            k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
+            sweep_init(false, false, true);
+            itVLast_ = run_solver(false);
+            totItV_ += itVLast_;
+            div_finish(true);
+            break;
+        case SPH_STAGE_CLEAR_ACCEL: post_forces(true, false, false, false, false, false); break;
+        case SPH_STAGE_VISCOSITY: viscosity(true); break;
+        case SPH_STAGE_XSPH:
+            if (P.xsph != 0.0 || P.xsph_boundary != 0.0) post_forces(false, false, true, false, false, false);
+            break;
+        case SPH_STAGE_CFL:
+            if (P.cfl_method == 1 || P.cfl_method == 2) post_forces(false, false, false, true, false, false);
+            break;
+        case SPH_STAGE_UPDATE_VELOCITY: post_forces(false, false, false, false, true, false); break;
+        case SPH_STAGE_PRESSURE_SETUP:
+            k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
+            pressure_setup();
+            break;
+        case SPH_STAGE_PRESSURE_ITER: pressure_iteration(true); break;
+        case SPH_STAGE_PRESSURE_FINISH:
+            pressure_finish(false);
+            finalize(true, false);
+            break;
+        case SPH_STAGE_PRESSURE_SOLVE:
+            k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
+            pressure_setup();
+            itLast_ = run_solver(true);
+            totIt_ += itLast_;
+            pressure_finish(false);
+            finalize(true, false);
+            break;
+        case SPH_STAGE_UPDATE_POSITION: finalize(false, true); break;
+        default: throw Error(SPH_ERR_INVALID, "unknown stage");
+    }
+    SPH_HIP(hipStreamSynchronize(stream));
+}
+
+// --------------------------------------------------------------- fields
+std::vector<unsigned> DFSPH::ids_host() {
+    std::vector<unsigned> id(n_);
+    SPH_HIP(hipStreamSynchronize(stream));
+    if (n_) SPH_HIP(hipMemcpy(id.data(), id_.get(), n_ * sizeof(unsigned), hipMemcpyDeviceToHost));
+    return id;
+}
+
+namespace {
+struct FieldInfo {
+    int comps;      // 1 or 3
+    int kind;       // 0 float4 array, 1 float array, 2 unsigned array, 3 vmb.w, 4 vmb.xyz
+    const char *name;
+};
+}  // namespace
+
+void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
+    SPH_REQUIRE(initialized_, SPH_ERR_STATE, "not initialised");
+    const size_t n = n_;
+    if (name == "boundary particle volume") {
+        SPH_REQUIRE(bytes == nb_ * sizeof(float), SPH_ERR_INVALID, "size mismatch");
+        if (!nb_) return;
+        std::vector<float4> b(nb_);
+        std::vector<unsigned> perm(nb_);
+        SPH_HIP(hipStreamSynchronize(stream));
+        SPH_HIP(hipMemcpy(b.data(), bx_.get(), nb_ * sizeof(float4), hipMemcpyDeviceToHost));
+        SPH_HIP(hipMemcpy(perm.data(), bperm_.get(), nb_ * sizeof(unsigned), hipMemcpyDeviceToHost));
+        float *out = static_cast<float *>(host);
+        for (size_t s = 0; s < nb_; s++) out[perm[s]] = b[s].w;
+        return;
+    }
+    const float4 *f4 = nullptr;
+    const float *f1 = nullptr;
+    const unsigned *u1 = nullptr;
+    int comps = 1, vmpart = 0;
+    if (name == "position") f4 = x_.get(), comps = 3;
+    else if (name == "velocity") f4 = v_.get(), comps = 3;
+    else if (name == "acceleration") f4 = a_.get(), comps = 3;
+    else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
+    else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
+    else if (name == "boundary x") f4 = vmb_.get(), comps = 3, vmpart = 1;
+    else if (name == "boundary volume") f4 = vmb_.get(), comps = 1, vmpart = 2;
+    else if (name == "density") f1 = rho_.get();
+    else if (name == "factor") f1 = alpha_.get();
+    else if (name == "advected density") f1 = radv_.get();
+    else if (name == "p / rho^2") f1 = p_.get();
+    else if (name == "p_v / rho^2") f1 = pv_.get();
+    else if (name == "id") u1 = id_.get();
+    else if (name == "fluid neighbor count") u1 = cf_.get();
+    else if (name == "boundary neighbor count") u1 = cb_.get();
+    else throw Error(SPH_ERR_NOTFOUND, "unknown field '" + name + "'");
+    SPH_REQUIRE(bytes == n * comps * 4, SPH_ERR_INVALID, "size mismatch for field '" + name + "'");
+    const std::vector<unsigned> id = ids_host();
+    if (f4) {
+        std::vector<float4> t(n);
+        SPH_HIP(hipMemcpy(t.data(), f4, n * sizeof(float4), hipMemcpyDeviceToHost));
+        float *out = static_cast<float *>(host);
+        for (size_t s = 0; s < n; s++) {
+            const size_t k = id[s];
+            if (vmpart == 2) {
+                out[k] = t[s].w;
+            } else {
+                out[3 * k] = t[s].x;
+                out[3 * k + 1] = t[s].y;
+                out[3 * k + 2] = t[s].z;
+            }
+        }
+    } else if (f1) {
+        std::vector<float> t(n);
+        SPH_HIP(hipMemcpy(t.data(), f1, n * sizeof(float), hipMemcpyDeviceToHost));
+        float *out = static_cast<float *>(host);
+        for (size_t s = 0; s < n; s++) out[id[s]] = t[s];
+    } else {
+        std::vector<unsigned> t(n);
+        if (name == "boundary neighbor count" && !(P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0))
+            std::fill(t.begin(), t.end(), 0u);
+        else
+            SPH_HIP(hipMemcpy(t.data(), u1, n * sizeof(unsigned), hipMemcpyDeviceToHost));
+        unsigned *out = static_cast<unsigned *>(host);
+        for (size_t s = 0; s < n; s++) out[id[s]] = t[s];
+    }
+}
+
+void DFSPH::set_field(const std::string &name, const void *host, size_t bytes) {
+    SPH_REQUIRE(initialized_, SPH_ERR_STATE, "not initialised");
+    const size_t n = n_;
+    float4 *f4 = nullptr;
+    float *f1 = nullptr;
+    int comps = 1;
+    if (name == "position") f4 = x_.get(), comps = 3;
+    else if (name == "velocity") f4 = v_.get(), comps = 3;
+    else if (name == "acceleration") f4 = a_.get(), comps = 3;
+    else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
+    else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
+    else if (name == "density") f1 = rho_.get();
+    else if (name == "factor") f1 = alpha_.get();
+    else if (name == "advected density") f1 = radv_.get();
+    else if (name == "p / rho^2") f1 = p_.get();
+    else if (name == "p_v / rho^2") f1 = pv_.get();
+    else throw Error(SPH_ERR_NOTFOUND, "unknown or read-only field '" + name + "'");
+    SPH_REQUIRE(bytes == n * comps * 4, SPH_ERR_INVALID, "size mismatch for field '" + name + "'");
+    const std::vector<unsigned> id = ids_host();
+    const float *in = static_cast<const float *>(host);
+    if (f4) {
+        std::vector<float4> t(n);
+        for (size_t s = 0; s < n; s++) {
+            const size_t k = id[s];
+            t[s] = make_float4(in[3 * k], in[3 * k + 1], in[3 * k + 2], 0.f);
+        }
+        SPH_HIP(hipMemcpy(f4, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        if (name == "position") {
+            std::vector<float4> z(n, make_float4(0.f, 0.f, 0.f, 0.f));
+            SPH_HIP(hipMemcpy(xc_.get(), z.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        }
+    } else {
+        std::vector<float> t(n);
+        for (size_t s = 0; s < n; s++) t[s] = in[id[s]];
+        SPH_HIP(hipMemcpy(f1, t.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+}
+
+void DFSPH::get_neighbors(int boundary, unsigned *offsets, unsigned *indices, size_t capacity, size_t *nnz) {
+    SPH_REQUIRE(have_lists_, SPH_ERR_STATE, "no neighbour lists yet");
+    const size_t n = n_;
+    const bool bset = boundary != 0;
+    const bool akinci = P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0;
+    const unsigned cap = bset ? capB_ : capF_;
+    std::vector<unsigned> cnt(n, 0u), lists, id = ids_host(), bperm;
+    if (!bset || akinci) {
+        SPH_HIP(hipMemcpy(cnt.data(), bset ? cb_.get() : cf_.get(), n * sizeof(unsigned), hipMemcpyDeviceToHost));
+        lists.resize((size_t)cap * stride_);
+        SPH_HIP(hipMemcpy(lists.data(), bset ? nb_l_.get() : nf_.get(), lists.size() * sizeof(unsigned),
+                          hipMemcpyDeviceToHost));
+    }
+    if (bset && akinci) {
+        bperm.resize(nb_);
+        SPH_HIP(hipMemcpy(bperm.data(), bperm_.get(), nb_ * sizeof(unsigned), hipMemcpyDeviceToHost));
+    }
+    std::vector<unsigned> rowcnt(n);
+    for (size_t s = 0; s < n; s++) rowcnt[id[s]] = cnt[s];
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t i = 0; i < n; i++) off[i + 1] = off[i] + rowcnt[i];
+    if (nnz) *nnz = off[n];
+    if (offsets)
+        for (size_t i = 0; i <= n; i++) offsets[i] = (unsigned)off[i];
+    if (!indices) return;
+    SPH_REQUIRE(capacity >= off[n], SPH_ERR_INVALID, "neighbour capacity too small");
+    for (size_t s = 0; s < n; s++) {
+        const size_t row = id[s];
+        unsigned *o = indices + off[row];
+        for (unsigned k = 0; k < cnt[s]; k++) {
+            const unsigned j = lists[(size_t)k * stride_ + s];
+            o[k] = bset ? bperm[j] : id[j];
+        }
+        std::sort(o, o + cnt[s]);
+    }
+}
+
+void DFSPH::counters(sph_dfsph_counters_t *c) {
+    read_ctl();
+    std::memset(c, 0, sizeof(*c));
+    c->time = hctl_->time;
+    c->time_step_size = hctl_->h;
+    c->avg_density_err = hctl_->avg_err;
+    c->avg_divergence_err = hctl_->avg_err_v;
+    c->iterations = hctl_->it;
+    c->iterations_v = P.enable_divergence_solver ? hctl_->it_v : 0;
+    c->visco_iterations = hctl_->cg_it;
+    c->steps = steps_;
+    c->total_iterations = totIt_;
+    c->total_iterations_v = totItV_;
+    c->total_visco_iterations = totVisco_;
+    c->max_fluid_neighbors = (int)hctl_->maxcnt_f;
+    c->max_boundary_neighbors = (int)hctl_->maxcnt_b;
+    c->flags = hctl_->flags;
+}
+
+void DFSPH::set_time(double t, double h) {
+    SPH_REQUIRE(h > 0, SPH_ERR_INVALID, "time step size must be > 0");
+    // h_step is taken from h by the next neighbourhood search
+    upload_ctl_field(offsetof(Ctl, h), &h, sizeof(double));
+    upload_ctl_field(offsetof(Ctl, time), &t, sizeof(double));
+}
+
+void DFSPH::sync() { SPH_HIP(hipStreamSynchronize(stream)); }
+
+}  // namespace sph

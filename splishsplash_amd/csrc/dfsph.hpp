@@ -1,0 +1,141 @@
+// dfsph.hpp -- device-resident DFSPH time step (TimeStepDFSPH,
+// SPlisHSPlasH/DFSPH/TimeStepDFSPH.cpp) with its neighbour search, Akinci2012
+// and Bender2019 boundaries, XSPH and the Weiler2018 implicit viscosity.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "grid.hpp"
+#include "kernel_math.hpp"
+#include "volume_map.hpp"
+
+namespace sph {
+
+// Scalars shared by host and kernels, resident in device memory so that
+// time-step control and solver convergence never need a host round trip
+// inside a step.
+struct Ctl {
+    double h;          // TimeManager time step size
+    double h_step;     // h cached at the start of the step (TimeStepDFSPH.cpp:121)
+    double time;
+    double avg_err;    // last pressure iteration, average density error
+    double avg_err_v;  // last divergence iteration
+    double cg_rhs2, cg_thresh, cg_absNew, cg_alpha, cg_beta, cg_r2;
+    int it, done;      // pressure solver
+    int it_v, done_v;  // divergence solver
+    int cg_it, cg_done;
+    int it_prev;       // pressure iterations of the previous solve (CFL method 2)
+    int flags;         // bit0: non-finite value in a reduction
+    unsigned ticket;   // last-block counter (returns to 0 after each use)
+    unsigned maxcnt_f, maxcnt_b;
+    unsigned pad;
+};
+
+// Everything a sweep kernel needs, passed by value.
+struct Dev {
+    unsigned n, stride;
+    float4 *x, *v, *v2, *a, *ap, *vdiff, *xc, *vmb;
+    float *rho, *alpha, *radv, *p, *pv;
+    const unsigned *nf, *cf, *nb, *cb;
+    const float4 *bx;  // Akinci particles (x, y, z, volume), cell-sorted
+    int bmode;         // 0 none, 1 Akinci2012, 2 Bender2019
+    float V, rho0, eps, mass;
+    float3 grav;
+    KernelConst K;
+    Ctl *ctl;
+    double *partial;
+    // solver parameters
+    int minIt, maxIt, maxItV;
+    double maxError, maxErrorV;
+    float xsph, xsphB;
+    int cflMethod;
+    double cflFactor, cflMin, cflMax, radius;
+    // Weiler2018
+    float4 *minv;  // 2 float4 per particle: symmetric 3x3 inverse (xx xy xz yy | yz zz - -)
+    float4 *cgx, *cgr, *cgp, *cgz, *cgt;
+    float mu, mub, hs2;
+    int viscoMaxIt;
+    double viscoTol;
+};
+
+class DFSPH {
+public:
+    explicit DFSPH(const sph_dfsph_params_t &p);
+    ~DFSPH();
+
+    sph_dfsph_params_t P;
+    double r, hs, V, mass;
+    KernelConst K;
+    hipStream_t stream = nullptr;
+    KernelTimer timer;
+
+    void set_fluid(const float *x, const float *v, size_t n);
+    void add_boundary(const float *x, size_t n);
+    void set_volume_map_box(const double lo[3], const double hi[3], int invert, double thickness,
+                            const unsigned res[3]);
+    void init();
+    void step();
+    void stage(int s);
+
+    void get_field(const std::string &name, void *host, size_t bytes);
+    void set_field(const std::string &name, const void *host, size_t bytes);
+    void get_neighbors(int boundary, unsigned *offsets, unsigned *indices, size_t capacity, size_t *nnz);
+    void counters(sph_dfsph_counters_t *c);
+    void set_time(double t, double h);
+    void sync();
+    size_t n() const { return n_; }
+
+    VolumeMap vmap;
+
+private:
+    size_t n_ = 0;
+    unsigned stride_ = 0;
+    bool initialized_ = false, have_lists_ = false;
+    unsigned capF_ = 64, capB_ = 64;
+    GridDesc grid_{};
+    std::vector<float> hx_, hv_, hbx_;  // staged host inputs
+    size_t nb_ = 0;
+    bool vm_requested_ = false;
+    double vm_lo_[3], vm_hi_[3], vm_thick_ = 0;
+    unsigned vm_res_[3];
+    int vm_invert_ = 1;
+
+    // device state
+    DBuf<float4> x_, v_, v2_, a_, ap_, vdiff_, xc_, vmb_, bx_;
+    DBuf<float4> x2_, vdiff2_, xc2_;  // permutation targets
+    DBuf<float> rho_, alpha_, radv_, p_, pv_, p2_, pv2_;
+    DBuf<unsigned> id_, id2_, nf_, cf_, nb_l_, cb_, bperm_, bstart_;
+    DBuf<float4> minv_, cgx_, cgr_, cgp_, cgz_, cgt_;
+    DBuf<double> partial_;
+    DBuf<Ctl> ctl_;
+    Ctl *hctl_ = nullptr;  // pinned mirror
+    CellSorter fsort_, bsort_;
+    long long totIt_ = 0, totItV_ = 0, totVisco_ = 0;
+    int steps_ = 0, lastIt_ = 2, lastItV_ = 1, lastCg_ = 4;
+    int itLast_ = 0, itVLast_ = 0, cgLast_ = 0;
+
+    Dev dev() const;
+    void upload_ctl_field(size_t off, const void *src, size_t bytes);
+    void read_ctl();
+    void neighborhood_search();
+    void build_lists();
+    void volume_maps();
+    void sweep_init(bool density, bool factor, bool div_setup);
+    void div_iteration(bool force);
+    void div_finish(bool scale_pv);
+    void scale(float *p, int which);
+    void clear_accel();
+    void viscosity(bool apply_separately);
+    void post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupdate, bool scale_pv);
+    void pressure_setup();
+    void pressure_iteration(bool force);
+    void pressure_finish(bool finalize);
+    void finalize(bool scale_p, bool update_x);
+    int run_solver(bool pressure);
+    int run_cg();
+    std::vector<unsigned> ids_host();
+};
+
+}  // namespace sph

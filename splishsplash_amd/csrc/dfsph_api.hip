@@ -1,0 +1,192 @@
+// dfsph_api.hip -- sph_dfsph_* : C-ABI over the device-resident DFSPH step
+// (TimeStepDFSPH and the state it steps; see include/sph_capi.h).
+#include <cstring>
+
+#include "dfsph.hpp"
+
+using namespace sph;
+
+struct sph_dfsph {
+    std::unique_ptr<DFSPH> s;
+};
+
+#define DCHK(h) SPH_REQUIRE((h) != nullptr && (h)->s, SPH_ERR_INVALID, "null handle")
+
+extern "C" {
+
+void sph_dfsph_default_params(sph_dfsph_params_t *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->particle_radius = 0.025;
+    p->density0 = 1000.0;
+    p->gravity[1] = -9.81;
+    p->time_step_size = 0.001;
+    p->cfl_method = 1;
+    p->cfl_factor = 0.5;
+    p->cfl_min_time_step_size = 0.0001;
+    p->cfl_max_time_step_size = 0.005;
+    p->min_iterations = 2;
+    p->max_iterations = 100;
+    p->max_error = 0.01;
+    p->max_iterations_v = 100;
+    p->max_error_v = 0.1;
+    p->enable_divergence_solver = 1;
+    p->viscosity_method = SPH_VISCOSITY_NONE;
+    p->viscosity = 0.01;
+    p->visco_max_iter = 100;
+    p->visco_max_error = 0.01;
+    p->boundary_handling_method = SPH_BOUNDARY_BENDER2019;
+}
+
+int sph_dfsph_create(const sph_dfsph_params_t *params, sph_dfsph_t *out) {
+    return guarded([&] {
+        SPH_REQUIRE(params && out, SPH_ERR_INVALID, "null argument");
+        auto *h = new sph_dfsph();
+        try {
+            h->s.reset(new DFSPH(*params));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int sph_dfsph_destroy(sph_dfsph_t s) {
+    return guarded([&] { delete s; });
+}
+
+int sph_dfsph_set_fluid(sph_dfsph_t s, const float *x, const float *v, size_t n) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(x && n > 0, SPH_ERR_INVALID, "need at least one particle");
+        s->s->set_fluid(x, v, n);
+    });
+}
+
+int sph_dfsph_add_boundary_particles(sph_dfsph_t s, const float *x, size_t n) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(x || n == 0, SPH_ERR_INVALID, "null positions");
+        s->s->add_boundary(x, n);
+    });
+}
+
+int sph_dfsph_set_volume_map_box(sph_dfsph_t s, const double box_min[3], const double box_max[3], int invert,
+                                 double thickness, const unsigned resolution[3]) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(box_min && box_max && resolution, SPH_ERR_INVALID, "null argument");
+        s->s->set_volume_map_box(box_min, box_max, invert, thickness, resolution);
+    });
+}
+
+int sph_dfsph_init(sph_dfsph_t s) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->init();
+    });
+}
+
+int sph_dfsph_step(sph_dfsph_t s, int n_steps) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(n_steps >= 0, SPH_ERR_INVALID, "n_steps must be >= 0");
+        for (int i = 0; i < n_steps; i++) s->s->step();
+        s->s->sync();
+    });
+}
+
+int sph_dfsph_stage(sph_dfsph_t s, int stage) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(stage >= 0 && stage < SPH_STAGE_COUNT, SPH_ERR_INVALID, "unknown stage");
+        s->s->stage(stage);
+    });
+}
+
+int sph_dfsph_num_particles(sph_dfsph_t s, size_t *out) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(out, SPH_ERR_INVALID, "null argument");
+        *out = s->s->n();
+    });
+}
+
+int sph_dfsph_get_field(sph_dfsph_t s, const char *name, void *host, size_t bytes) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(name && host, SPH_ERR_INVALID, "null argument");
+        s->s->get_field(name, host, bytes);
+    });
+}
+
+int sph_dfsph_set_field(sph_dfsph_t s, const char *name, const void *host, size_t bytes) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(name && host, SPH_ERR_INVALID, "null argument");
+        s->s->set_field(name, host, bytes);
+    });
+}
+
+int sph_dfsph_get_neighbors(sph_dfsph_t s, int boundary, unsigned *offsets, unsigned *indices, size_t capacity,
+                            size_t *nnz) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->get_neighbors(boundary, offsets, indices, capacity, nnz);
+    });
+}
+
+int sph_dfsph_get_counters(sph_dfsph_t s, sph_dfsph_counters_t *out) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(out, SPH_ERR_INVALID, "null argument");
+        s->s->counters(out);
+    });
+}
+
+int sph_dfsph_set_time(sph_dfsph_t s, double time, double time_step_size) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_time(time, time_step_size);
+    });
+}
+
+int sph_dfsph_enable_timing(sph_dfsph_t s, int enable) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->timer.enabled = enable != 0;
+    });
+}
+
+int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name, long long *launches, double *total_ms,
+                           int *count) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->sync();
+        s->s->timer.collect();
+        auto &st = s->s->timer.stats;
+        if (count) *count = (int)st.size();
+        if (k < 0 || k >= (int)st.size()) return;
+        if (name) *name = st[k].name.c_str();
+        if (launches) *launches = st[k].launches;
+        if (total_ms) *total_ms = st[k].ms;
+    });
+}
+
+int sph_dfsph_reset_kernel_stats(sph_dfsph_t s) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->sync();
+        s->s->timer.reset();
+    });
+}
+
+int sph_dfsph_synchronize(sph_dfsph_t s) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->sync();
+    });
+}
+
+}  // extern "C"

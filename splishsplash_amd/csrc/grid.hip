@@ -1,0 +1,339 @@
+// grid.hip -- counting sort into cells, scans, permutations and the
+// 27-cell neighbour sweep (see grid.hpp).
+#include <algorithm>
+#include <cmath>
+
+#include "grid.hpp"
+
+namespace sph {
+
+GridDesc make_grid(const double lo[3], const double hi[3], double radius, size_t max_cells) {
+    double cs = radius * (1.0 + 1e-4);
+    GridDesc g{};
+    for (int it = 0; it < 64; it++) {
+        size_t total = 1;
+        int n[3];
+        for (int c = 0; c < 3; c++) {
+            const double ext = (hi[c] - lo[c]) + 2.0 * cs;
+            n[c] = std::max(1, (int)std::ceil(ext / cs));
+            total *= (size_t)n[c];
+        }
+        if (total <= max_cells) {
+            g.ox = (float)(lo[0] - cs);
+            g.oy = (float)(lo[1] - cs);
+            g.oz = (float)(lo[2] - cs);
+            g.inv = (float)(1.0 / cs);
+            g.nx = n[0];
+            g.ny = n[1];
+            g.nz = n[2];
+            g.ncells = (unsigned)total;
+            return g;
+        }
+        cs *= 1.26;
+    }
+    throw Error(SPH_ERR_INVALID, "grid: domain too large");
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cell_hist(const float4 *__restrict__ x, unsigned n, GridDesc g,
+                                                     unsigned *__restrict__ pcell, unsigned *__restrict__ prank,
+                                                     unsigned *__restrict__ count) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned c = cell_id(cell_of(x[i], g), g);
+    pcell[i] = c;
+    prank[i] = atomicAdd(&count[c], 1u);
+}
+
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of a tile of SCAN_TILE values held SCAN_ITEMS per thread;
+// returns the thread's exclusive offset within the tile, and the tile total.
+__device__ __forceinline__ unsigned block_excl_scan(unsigned tsum, unsigned &total) {
+    __shared__ unsigned wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned incl = wave_incl_scan(tsum);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+    for (int w = 0; w < BLOCK / 64; w++) {
+        if (w < wid) before += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    total = tot;
+    return before + incl - tsum;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_local(const unsigned *__restrict__ in, unsigned *__restrict__ out,
+                                                      unsigned n, unsigned *__restrict__ bsum) {
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    unsigned v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    unsigned total;
+    unsigned off = block_excl_scan(s, total);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = off;
+        off += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// One block scans the nb block totals in place (exclusive) and writes the
+// grand total to *total_out.
+__global__ __launch_bounds__(BLOCK) void k_scan_bsum(unsigned *__restrict__ bsum, unsigned nb,
+                                                     unsigned *__restrict__ total_out) {
+    unsigned carry = 0;
+    for (unsigned t0 = 0; t0 < nb; t0 += SCAN_TILE) {
+        const unsigned base = t0 + threadIdx.x * SCAN_ITEMS;
+        unsigned v[SCAN_ITEMS], s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            v[k] = (base + k < nb) ? bsum[base + k] : 0u;
+            s += v[k];
+        }
+        unsigned total;
+        unsigned off = block_excl_scan(s, total) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            if (base + k < nb) bsum[base + k] = off;
+            off += v[k];
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_add(unsigned *__restrict__ out, unsigned n,
+                                                    const unsigned *__restrict__ bsum) {
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    const unsigned add = bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) out[base + k] += add;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(const unsigned *__restrict__ pcell,
+                                                   const unsigned *__restrict__ prank,
+                                                   const unsigned *__restrict__ start, unsigned n,
+                                                   unsigned *__restrict__ perm) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    perm[start[pcell[i]] + prank[i]] = i;
+}
+
+// Orders the slots of each cell by point index (insertion sort; cells hold a
+// handful of points).
+__global__ __launch_bounds__(BLOCK) void k_sort_cells(const unsigned *__restrict__ start, unsigned ncells,
+                                                      unsigned *__restrict__ perm) {
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const unsigned s = start[c], e = start[c + 1];
+    for (unsigned a = s + 1; a < e; a++) {
+        const unsigned v = perm[a];
+        unsigned b = a;
+        while (b > s && perm[b - 1] > v) {
+            perm[b] = perm[b - 1];
+            b--;
+        }
+        perm[b] = v;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_permute(const unsigned *__restrict__ perm, unsigned n,
+                                                   const T *__restrict__ src, T *__restrict__ dst) {
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[perm[k]];
+}
+
+// 27-cell sweep, one thread per query, column-major output.
+template <bool SAME>
+__global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q, unsigned nq,
+                                                    const float4 *__restrict__ src,
+                                                    const unsigned *__restrict__ start, GridDesc g, float r2,
+                                                    unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
+                                                    unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned k = 0;
+    if (i < nq) {
+        const float4 xi = q[i];
+        const int3 c = cell_of(xi, g);
+        const int x0 = c.x > 0 ? c.x - 1 : 0, x1 = c.x < g.nx - 1 ? c.x + 1 : g.nx - 1;
+        for (int dz = -1; dz <= 1; dz++) {
+            const int z = c.z + dz;
+            if (z < 0 || z >= g.nz) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                const int y = c.y + dy;
+                if (y < 0 || y >= g.ny) continue;
+                const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
+                const unsigned s = start[row + x0], e = start[row + x1 + 1];
+                for (unsigned j = s; j < e; j++) {
+                    if (SAME && j == i) continue;
+                    const float4 xj = src[j];
+                    const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
+                    const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
+                    if (d2 < r2) {
+                        if (k < cap) nbr[(size_t)k * stride + i] = j;
+                        k++;
+                    }
+                }
+            }
+        }
+        cnt[i] = k;
+    }
+    const unsigned m = wave_max_u(k);
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(maxcnt, m);
+}
+
+template <bool SAME, bool FILL>
+__global__ __launch_bounds__(BLOCK) void k_lists_csr(const float4 *__restrict__ q, unsigned nq,
+                                                     const float4 *__restrict__ src,
+                                                     const unsigned *__restrict__ srcmap,
+                                                     const unsigned *__restrict__ start, GridDesc g, float r2,
+                                                     const unsigned *__restrict__ rowoff, unsigned *__restrict__ out,
+                                                     unsigned *__restrict__ cnt) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float4 xi = q[i];
+    const int3 c = cell_of(xi, g);
+    const int x0 = c.x > 0 ? c.x - 1 : 0, x1 = c.x < g.nx - 1 ? c.x + 1 : g.nx - 1;
+    unsigned k = 0;
+    const unsigned o = FILL ? rowoff[i] : 0u;
+    for (int dz = -1; dz <= 1; dz++) {
+        const int z = c.z + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int y = c.y + dy;
+            if (y < 0 || y >= g.ny) continue;
+            const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
+            const unsigned s = start[row + x0], e = start[row + x1 + 1];
+            for (unsigned j = s; j < e; j++) {
+                const unsigned jj = srcmap[j];
+                if (SAME && jj == i) continue;
+                const float4 xj = src[j];
+                const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
+                const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
+                if (d2 < r2) {
+                    if (FILL) out[o + k] = jj;
+                    k++;
+                }
+            }
+        }
+    }
+    if (!FILL) cnt[i] = k;
+}
+
+// -------------------------------------------------------------------- host
+void fill_u32(hipStream_t s, unsigned *p, size_t n, unsigned v) {
+    if (n == 0) return;
+    k_fill_u32<<<(unsigned)((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(p, n, v);
+    SPH_HIP(hipGetLastError());
+}
+
+void exclusive_scan(hipStream_t s, const unsigned *in, unsigned *out, unsigned n, DBuf<unsigned> &bsum,
+                    KernelTimer &T) {
+    TimedScope ts(T, s, "scan");
+    const unsigned nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    bsum.reserve(std::max(nb, 1u));
+    if (n == 0) {
+        fill_u32(s, out, 1, 0u);
+        return;
+    }
+    k_scan_local<<<nb, BLOCK, 0, s>>>(in, out, n, bsum.get());
+    k_scan_bsum<<<1, BLOCK, 0, s>>>(bsum.get(), nb, out + n);
+    k_scan_add<<<nb, BLOCK, 0, s>>>(out, n, bsum.get());
+    SPH_HIP(hipGetLastError());
+}
+
+void CellSorter::sort(hipStream_t s, const float4 *x, unsigned nn, const GridDesc &g, KernelTimer &T) {
+    n = nn;
+    grid = g;
+    count.reserve(g.ncells);
+    start.reserve((size_t)g.ncells + 1);
+    pcell.reserve(std::max(n, 1u));
+    prank.reserve(std::max(n, 1u));
+    perm.reserve(std::max(n, 1u));
+    fill_u32(s, count.get(), g.ncells, 0u);
+    if (n) {
+        TimedScope ts(T, s, "cell_hist");
+        k_cell_hist<<<nblocks(n), BLOCK, 0, s>>>(x, n, g, pcell.get(), prank.get(), count.get());
+        SPH_HIP(hipGetLastError());
+    }
+    exclusive_scan(s, count.get(), start.get(), g.ncells, bsum, T);
+    if (n) {
+        TimedScope ts(T, s, "cell_scatter");
+        k_scatter<<<nblocks(n), BLOCK, 0, s>>>(pcell.get(), prank.get(), start.get(), n, perm.get());
+        k_sort_cells<<<nblocks(g.ncells), BLOCK, 0, s>>>(start.get(), g.ncells, perm.get());
+        SPH_HIP(hipGetLastError());
+    }
+}
+
+void permute_f4(hipStream_t s, const unsigned *perm, unsigned n, const float4 *src, float4 *dst) {
+    if (!n) return;
+    k_permute<float4><<<nblocks(n), BLOCK, 0, s>>>(perm, n, src, dst);
+    SPH_HIP(hipGetLastError());
+}
+void permute_f1(hipStream_t s, const unsigned *perm, unsigned n, const float *src, float *dst) {
+    if (!n) return;
+    k_permute<float><<<nblocks(n), BLOCK, 0, s>>>(perm, n, src, dst);
+    SPH_HIP(hipGetLastError());
+}
+void permute_u1(hipStream_t s, const unsigned *perm, unsigned n, const unsigned *src, unsigned *dst) {
+    if (!n) return;
+    k_permute<unsigned><<<nblocks(n), BLOCK, 0, s>>>(perm, n, src, dst);
+    SPH_HIP(hipGetLastError());
+}
+
+void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *start,
+                          const GridDesc &g, float r2, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
+                          unsigned *cnt, unsigned *maxcnt, KernelTimer &T, const char *name) {
+    if (!nq) return;
+    TimedScope ts(T, s, name);
+    if (same_set)
+        k_lists_cm<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+    else
+        k_lists_cm<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+    SPH_HIP(hipGetLastError());
+}
+
+void build_lists_csr(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *srcmap,
+                     const unsigned *start, const GridDesc &g, float r2, bool same_set, const unsigned *rowoff,
+                     unsigned *out, unsigned *cnt) {
+    if (!nq) return;
+    if (rowoff) {
+        if (same_set)
+            k_lists_csr<true, true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, srcmap, start, g, r2, rowoff, out, cnt);
+        else
+            k_lists_csr<false, true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, srcmap, start, g, r2, rowoff, out, cnt);
+    } else {
+        if (same_set)
+            k_lists_csr<true, false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, srcmap, start, g, r2, rowoff, out, cnt);
+        else
+            k_lists_csr<false, false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, srcmap, start, g, r2, rowoff, out, cnt);
+    }
+    SPH_HIP(hipGetLastError());
+}
+
+}  // namespace sph

@@ -1,0 +1,79 @@
+// grid.hpp -- uniform cell grid, counting sort of points into cells and the
+// neighbour-list sweep over the 27 surrounding cells.
+#pragma once
+#include "common.hpp"
+
+namespace sph {
+
+// Dense grid of cubic cells of edge >= search radius. Cells are numbered
+// x-fastest, so the three cells (cx-1..cx+1, cy, cz) of one row are
+// consecutive and a cell-sorted point array holds each row of the 27-cell
+// neighbourhood as one contiguous slot range: 9 ranges per query.
+struct GridDesc {
+    float ox, oy, oz;  // origin
+    float inv;         // 1 / cell edge
+    int nx, ny, nz;
+    unsigned ncells;
+};
+
+// Grid covering [lo, hi] (plus one cell of margin) with cell edge
+// radius * (1 + 1e-4); points outside are clamped to the border cells, which
+// keeps the search exact (clamping never separates two points by more than
+// one cell) at some cost in efficiency. The edge grows when the box would
+// need more than max_cells cells.
+GridDesc make_grid(const double lo[3], const double hi[3], double radius, size_t max_cells = (size_t)1 << 27);
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ int3 cell_of(const float4 &p, const GridDesc &g) {
+    return make_int3(clampi((int)floorf((p.x - g.ox) * g.inv), 0, g.nx - 1),
+                     clampi((int)floorf((p.y - g.oy) * g.inv), 0, g.ny - 1),
+                     clampi((int)floorf((p.z - g.oz) * g.inv), 0, g.nz - 1));
+}
+__device__ __forceinline__ unsigned cell_id(const int3 &c, const GridDesc &g) {
+    return ((unsigned)c.z * (unsigned)g.ny + (unsigned)c.y) * (unsigned)g.nx + (unsigned)c.x;
+}
+
+// Counting sort of n points into the grid:
+//   perm[slot]  = index of the point placed at `slot`
+//   start[c]    = first slot of cell c, start[ncells] = n
+// Points of one cell are ordered by ascending index, so the result depends
+// only on the input order (no atomics order leaks into it).
+class CellSorter {
+public:
+    DBuf<unsigned> count, start, pcell, prank, perm, bsum;
+    unsigned n = 0;
+    GridDesc grid{};
+    void sort(hipStream_t s, const float4 *x, unsigned n, const GridDesc &g, KernelTimer &T);
+};
+
+// Exclusive scan of n unsigned values into out (out[n] = total); uses bsum
+// as scratch.
+void exclusive_scan(hipStream_t s, const unsigned *in, unsigned *out, unsigned n, DBuf<unsigned> &bsum, KernelTimer &T);
+
+// Gathers dst[k] = src[perm[k]] for k < n.
+void permute_f4(hipStream_t s, const unsigned *perm, unsigned n, const float4 *src, float4 *dst);
+void permute_f1(hipStream_t s, const unsigned *perm, unsigned n, const float *src, float *dst);
+void permute_u1(hipStream_t s, const unsigned *perm, unsigned n, const unsigned *src, unsigned *dst);
+void fill_u32(hipStream_t s, unsigned *p, size_t n, unsigned v);
+
+// Neighbour lists of every query point among the cell-sorted source points.
+// Column-major output: neighbour k of query i at nbr[k * stride + i] (so a
+// wavefront of consecutive queries reads 64 consecutive words per k), the
+// source slot as value; cnt[i] = number found (may exceed cap: then the
+// entries beyond cap are dropped and *maxcnt reports the true maximum).
+// same_set excludes slot == i.
+void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *start,
+                          const GridDesc &g, float r2, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
+                          unsigned *cnt, unsigned *maxcnt, KernelTimer &T, const char *name);
+
+// CSR variant for the drop-in search: queries in caller order, sources
+// cell-sorted with srcmap[slot] = caller index of the source point. With
+// rowoff == nullptr only counts are written (pass 1); with rowoff the lists
+// are written at out[rowoff[i] ...] (pass 2). same_set excludes
+// srcmap[slot] == i.
+void build_lists_csr(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *srcmap,
+                     const unsigned *start, const GridDesc &g, float r2, bool same_set, const unsigned *rowoff,
+                     unsigned *out, unsigned *cnt);
+
+}  // namespace sph

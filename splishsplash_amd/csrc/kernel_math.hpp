@@ -1,0 +1,123 @@
+// kernel_math.hpp -- device evaluation of the cubic-spline SPH kernel with
+// the semantics of PrecomputedKernel<CubicKernel, 10000>
+// (SPlisHSPlasH/SPHKernels.h:614-691):
+//
+//   W(r)     = 0.5 (T[p] + T[p+1]),            T[i] = W_cubic(i s)
+//   gradW(r) = 0.5 (G[p] + G[p+1]) r,          G[i] = dW_cubic/dx(i s) / (i s)
+//   p        = min(floor(|r| / s), 9998),      s = h / 9999
+//
+// The table entries are evaluated in closed form from q = i / 9999 instead of
+// being read from memory, so the gather-bound sweeps spend no bandwidth on
+// the table. The index p is taken from an FP32 distance; when |r|/s lies
+// within GUARD of an integer the FP32 value could round to the other side of
+// the step, and p is recomputed from the FP64 distance of the original FP32
+// coordinates -- the value an FP64 build computes from the same positions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sph {
+
+struct KernelConst {
+    float h, h2;        // support radius, h*h (float)
+    float invStep_f;    // 9999 / h
+    float k, l;         // 8/(pi h^3), 48/(pi h^3)
+    float W0;           // W(0) = k
+    float invH2;        // 1 / h^2
+    float stepq;        // 1 / 9999
+    double invStep_d;   // 1 / (h / 9999) in double, as the reference computes it
+    double h_d;
+};
+
+KernelConst make_kernel_const(double h);
+
+constexpr unsigned KRES = 10000u;
+constexpr float GUARD = 2.5e-3f;
+
+// T[i] = W_cubic(q_i), SPHKernels.h:36-54
+__device__ __forceinline__ float table_T(unsigned i, const KernelConst &K) {
+    const float q = (float)i * K.stepq;
+    if (q <= 0.5f) {
+        const float q2 = q * q;
+        return K.k * (6.0f * q2 * q - 6.0f * q2 + 1.0f);
+    }
+    const float f = 1.0f - q;
+    return K.k * (2.0f * f * f * f);
+}
+
+// G[i] = gradW_cubic((i s, 0, 0))_x / (i s), SPHKernels.h:61-85 and :637-641
+__device__ __forceinline__ float table_G(unsigned i, const KernelConst &K) {
+    if (i == 0u) return 0.0f;
+    const float q = (float)i * K.stepq;
+    if (q <= 0.5f) return K.l * (3.0f * q - 2.0f) * K.invH2;
+    const float f = 1.0f - q;
+    return -K.l * f * f * K.invH2 / q;
+}
+
+__device__ __forceinline__ float sqn(float x, float y, float z) { return (x * x + y * y) + z * z; }
+
+// Table position for the pair (xi, xj) with FP32 squared distance d2.
+__device__ __forceinline__ unsigned table_pos(const float4 &xi, const float4 &xj, float d2, const KernelConst &K) {
+    const float t = sqrtf(d2) * K.invStep_f;
+    unsigned pos = (unsigned)t;
+    const float fr = t - (float)pos;
+    if (fr < GUARD || fr > 1.0f - GUARD) {
+        const double dx = (double)xi.x - (double)xj.x;
+        const double dy = (double)xi.y - (double)xj.y;
+        const double dz = (double)xi.z - (double)xj.z;
+        const double d2d = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+        pos = (unsigned)__dmul_rn(__dsqrt_rn(d2d), K.invStep_d);
+    }
+    return pos < KRES - 2u ? pos : KRES - 2u;
+}
+
+// W(xi - xj); zero outside the support (r^2 > h^2).
+__device__ __forceinline__ float kernel_W(const float4 &xi, const float4 &xj, const KernelConst &K) {
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float d2 = sqn(dx, dy, dz);
+    if (!(d2 <= K.h2)) return 0.0f;
+    const unsigned p = table_pos(xi, xj, d2, K);
+    return 0.5f * (table_T(p, K) + table_T(p + 1u, K));
+}
+
+// Scalar g with gradW(xi - xj) = g (xi - xj); zero outside the support.
+// Also returns r = xi - xj.
+__device__ __forceinline__ float kernel_G(const float4 &xi, const float4 &xj, const KernelConst &K, float3 &r) {
+    r = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+    const float d2 = sqn(r.x, r.y, r.z);
+    if (!(d2 <= K.h2)) return 0.0f;
+    const unsigned p = table_pos(xi, xj, d2, K);
+    return 0.5f * (table_G(p, K) + table_G(p + 1u, K));
+}
+
+// Both W and the gradient scalar for one pair.
+__device__ __forceinline__ void kernel_WG(const float4 &xi, const float4 &xj, const KernelConst &K, float3 &r, float &w,
+                                          float &g) {
+    r = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+    const float d2 = sqn(r.x, r.y, r.z);
+    if (!(d2 <= K.h2)) {
+        w = 0.0f;
+        g = 0.0f;
+        return;
+    }
+    const unsigned p = table_pos(xi, xj, d2, K);
+    w = 0.5f * (table_T(p, K) + table_T(p + 1u, K));
+    g = 0.5f * (table_G(p, K) + table_G(p + 1u, K));
+}
+
+__device__ __forceinline__ float dot3(const float3 &a, const float3 &b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// Cubic spline W(r) for a scalar distance, double (SPHKernels.h:36-54).
+__device__ __forceinline__ double cubic_W_d(double r, double h, double k) {
+    const double q = r / h;
+    if (q <= 1.0) {
+        if (q <= 0.5) {
+            const double q2 = q * q;
+            return k * (6.0 * (q2 * q) - 6.0 * q2 + 1.0);
+        }
+        const double f = 1.0 - q;
+        return k * (2.0 * (f * f * f));
+    }
+    return 0.0;
+}
+
+}  // namespace sph

@@ -1,0 +1,379 @@
+// volume_map.hip -- construction and per-particle sampling of the Bender2019
+// volume map of a box wall (see volume_map.hpp).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "kernel_math.hpp"
+#include "volume_map.hpp"
+
+namespace sph {
+
+struct Gauss16 {
+    double x[16], w[16];
+};
+
+// Gauss-Legendre rule on [-1, 1] (Newton on the Legendre recurrence, in
+// extended precision), abscissae ascending.
+static Gauss16 gauss16() {
+    Gauss16 g{};
+    typedef long double LD;
+    const int n = 16, m = 8;
+    const LD pi = 3.141592653589793238462643383279502884L;
+    for (int i = 1; i <= m; i++) {
+        LD z = std::cos(pi * (i - 0.25L) / (n + 0.5L)), pp = 0;
+        for (int it = 0; it < 100; it++) {
+            LD p1 = 1, p2 = 0;
+            for (int j = 1; j <= n; j++) {
+                const LD p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0L);
+            const LD z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= 1e-19L) break;
+        }
+        LD p1 = 1, p2 = 0;
+        for (int j = 1; j <= n; j++) {
+            const LD p3 = p2;
+            p2 = p1;
+            p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0L);
+        g.x[i - 1] = (double)-z;
+        g.x[n - i] = (double)z;
+        g.w[i - 1] = g.w[n - i] = (double)(2.0L / ((1.0L - z) * (1.0L + z) * pp * pp));
+    }
+    return g;
+}
+
+// Signed distance of the box surface: sign * (d_mesh - thickness) with
+// d_mesh < 0 inside (SimulatorBase.cpp:2609-2611).
+__device__ __forceinline__ double box_sdf(const double *p, const VMapDesc &d) {
+    bool in = true;
+    double dmin = DBL_MAX, q2 = 0.0;
+    for (int c = 0; c < 3; c++) {
+        if (p[c] < d.box_lo[c] || p[c] > d.box_hi[c]) in = false;
+        dmin = fmin(dmin, fmin(p[c] - d.box_lo[c], d.box_hi[c] - p[c]));
+        const double q = fmax(fmax(d.box_lo[c] - p[c], 0.0), p[c] - d.box_hi[c]);
+        q2 += q * q;
+    }
+    const double dm = in ? -dmin : sqrt(q2);
+    return (d.invert ? -1.0 : 1.0) * (dm - d.thickness);
+}
+
+struct NodeCounts {
+    unsigned long long nc, nx, ny, nz;  // corners, x-, y-, z-edge nodes
+};
+static NodeCounts node_counts(const unsigned *r) {
+    const unsigned long long a = r[0], b = r[1], c = r[2];
+    return {(a + 1) * (b + 1) * (c + 1), 2 * a * (b + 1) * (c + 1), 2 * (a + 1) * b * (c + 1), 2 * (a + 1) * (b + 1) * c};
+}
+
+// Position of global node g.
+__device__ __forceinline__ void node_pos(unsigned long long g, const VMapDesc &d, NodeCounts nc, double *p) {
+    const unsigned long long a = d.res[0], b = d.res[1];
+    double f[3];
+    if (g < nc.nc) {
+        const unsigned long long i = g % (a + 1), j = (g / (a + 1)) % (b + 1), k = g / ((a + 1) * (b + 1));
+        f[0] = (double)i;
+        f[1] = (double)j;
+        f[2] = (double)k;
+    } else {
+        g -= nc.nc;
+        int e;
+        unsigned long long dims[3];
+        if (g < nc.nx) {
+            e = 0;
+        } else if (g < nc.nx + nc.ny) {
+            g -= nc.nx;
+            e = 1;
+        } else {
+            g -= nc.nx + nc.ny;
+            e = 2;
+        }
+        const unsigned t = (unsigned)(g & 1ull);
+        g >>= 1;
+        for (int c = 0; c < 3; c++) dims[c] = (c == e) ? d.res[c] : d.res[c] + 1;
+        const unsigned long long i = g % dims[0], j = (g / dims[0]) % dims[1], k = g / (dims[0] * dims[1]);
+        f[0] = (double)i;
+        f[1] = (double)j;
+        f[2] = (double)k;
+        f[e] += t ? 2.0 / 3.0 : 1.0 / 3.0;
+    }
+    for (int c = 0; c < 3; c++) p[c] = d.lo[c] + d.cs[c] * f[c];
+}
+
+struct QuadArgs {
+    Gauss16 g;
+    double h, k, W0;
+};
+
+// Node values of both fields; volume by the p = 30 product rule
+// (GaussQuadrature.cpp:5930-5962) over [-h, h]^3 with the integrand of
+// SimulatorBase.cpp:2637-2650 (the analytic box distance replaces the
+// interpolated field 0 inside the integrand).
+__global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, unsigned long long total, QuadArgs q,
+                                                    float *sdf_node, float *vol_node) {
+    const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    double x[3];
+    node_pos(g, d, nc, x);
+    const double dist_x = box_sdf(x, d);
+    sdf_node[g] = (float)dist_x;
+    const double h = q.h;
+    double res = 0.0;
+    if (!(dist_x > 2.0 * h)) {
+        double dom_hi[3];
+        for (int c = 0; c < 3; c++) dom_hi[c] = d.lo[c] + d.cs[c] * d.res[c];
+        for (int i = 0; i < 16; i++) {
+            const double wi = q.g.w[i];
+            for (int j = 0; j < 16; j++) {
+                const double wij = wi * q.g.w[j];
+                for (int k = 0; k < 16; k++) {
+                    const double wijk = wij * q.g.w[k];
+                    const double e[3] = {h * q.g.x[i], h * q.g.x[j], h * q.g.x[k]};
+                    double f = 0.0;
+                    if (!(((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]) > h * h)) {
+                        const double y[3] = {x[0] + e[0], x[1] + e[1], x[2] + e[2]};
+                        bool indom = true;
+                        for (int c = 0; c < 3; c++)
+                            if (!(y[c] >= d.lo[c] && y[c] <= dom_hi[c])) indom = false;
+                        if (indom) {
+                            const double dist = box_sdf(y, d);
+                            if (dist <= 0.0)
+                                f = 1.0;
+                            else if (dist < h)
+                                f = cubic_W_d(dist, h, q.k) / q.W0;
+                        }
+                    }
+                    res += wijk * f;
+                }
+            }
+        }
+        res *= h * h * h;
+        res *= 0.8;
+    }
+    vol_node[g] = (float)res;
+}
+
+// Global node of local serendipity node n of cell (ci, cj, ck). Local order:
+// 8 corners (bit0 x, bit1 y, bit2 z), then x-, y-, z-edge pairs; in each
+// pair t = 0 sits at 1/3 and t = 1 at 2/3 of the edge.
+__device__ __forceinline__ unsigned long long cell_node(int n, unsigned ci, unsigned cj, unsigned ck, const VMapDesc &d,
+                                                       NodeCounts nc) {
+    const unsigned long long a = d.res[0], b = d.res[1], c = d.res[2];
+    if (n < 8) {
+        const unsigned long long i = ci + (n & 1), j = cj + ((n >> 1) & 1), k = ck + ((n >> 2) & 1);
+        return (k * (b + 1) + j) * (a + 1) + i;
+    }
+    const int e = (n - 8) / 8, r = (n - 8) % 8, t = r & 1, bb = r >> 1;
+    const unsigned u0 = bb & 1, u1 = (bb >> 1) & 1;
+    if (e == 0) {  // x-edge at (ci, cj+u0, ck+u1)
+        const unsigned long long i = ci, j = cj + u0, k = ck + u1;
+        return nc.nc + 2 * ((k * (b + 1) + j) * a + i) + t;
+    }
+    if (e == 1) {  // y-edge at (ci+u0, cj, ck+u1)
+        const unsigned long long i = ci + u0, j = cj, k = ck + u1;
+        return nc.nc + nc.nx + 2 * ((k * b + j) * (a + 1) + i) + t;
+    }
+    // z-edge at (ci+u0, cj+u1, ck)
+    const unsigned long long i = ci + u0, j = cj + u1, k = ck;
+    (void)c;
+    return nc.nc + nc.nx + nc.ny + 2 * ((k * (b + 1) + j) * (a + 1) + i) + t;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_vm_cells(VMapDesc d, NodeCounts nc, unsigned long long ncells,
+                                                    const float *sdf_node, const float *vol_node, float *sdf_cell,
+                                                    float *vol_cell) {
+    const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells * 32ull) return;
+    const unsigned long long cell = t >> 5;
+    const int n = (int)(t & 31ull);
+    const unsigned ci = (unsigned)(cell % d.res[0]), cj = (unsigned)((cell / d.res[0]) % d.res[1]),
+                   ck = (unsigned)(cell / ((unsigned long long)d.res[0] * d.res[1]));
+    const unsigned long long g = cell_node(n, ci, cj, ck, d, nc);
+    sdf_cell[t] = sdf_node[g];
+    vol_cell[t] = vol_node[g];
+}
+
+void VolumeMap::build(hipStream_t s, const double box_lo[3], const double box_hi[3], int invert, double thickness,
+                      const unsigned res[3], double h) {
+    SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
+    const double pad = 8.0 * h + thickness;
+    for (int c = 0; c < 3; c++) {
+        SPH_REQUIRE(box_hi[c] > box_lo[c], SPH_ERR_INVALID, "volume map: empty box");
+        d.box_lo[c] = box_lo[c];
+        d.box_hi[c] = box_hi[c];
+        d.lo[c] = box_lo[c] - pad;
+        const double hi = box_hi[c] + pad;
+        d.res[c] = res[c];
+        d.cs[c] = (hi - d.lo[c]) / (double)res[c];
+        d.inv[c] = 1.0 / d.cs[c];
+    }
+    d.invert = invert;
+    d.thickness = thickness;
+    ncells = (size_t)res[0] * res[1] * res[2];
+    const NodeCounts nc = node_counts(res);
+    const unsigned long long total = nc.nc + nc.nx + nc.ny + nc.nz;
+    DBuf<float> sn, vn;
+    sn.reserve(total);
+    vn.reserve(total);
+    sdf.reserve(ncells * 32);
+    vol.reserve(ncells * 32);
+    QuadArgs q;
+    q.g = gauss16();
+    q.h = h;
+    const double pi = 3.14159265358979323846;
+    q.k = 8.0 / (pi * h * h * h);
+    q.W0 = q.k;
+    k_vm_nodes<<<(unsigned)((total + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+    SPH_HIP(hipGetLastError());
+    const unsigned long long nt = (unsigned long long)ncells * 32ull;
+    k_vm_cells<<<(unsigned)((nt + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, ncells, sn.get(), vn.get(), sdf.get(),
+                                                                       vol.get());
+    SPH_HIP(hipGetLastError());
+    SPH_HIP(hipStreamSynchronize(s));
+    d.sdf = sdf.get();
+    d.vol = vol.get();
+    built = true;
+}
+
+void VolumeMap::cell_values(const uint64_t *cells, size_t n, float *out) const {
+    SPH_REQUIRE(built, SPH_ERR_STATE, "volume map not built");
+    for (size_t i = 0; i < n; i++) {
+        SPH_REQUIRE(cells[i] < ncells, SPH_ERR_INVALID, "volume map: cell out of range");
+        SPH_HIP(hipMemcpy(out + 64 * i, sdf.get() + 32 * cells[i], 32 * sizeof(float), hipMemcpyDeviceToHost));
+        SPH_HIP(hipMemcpy(out + 64 * i + 32, vol.get() + 32 * cells[i], 32 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+}
+
+// ---------------------------------------------------------------- sampling
+// Serendipity shape functions and gradients at local coordinates xi.
+__device__ __forceinline__ void shape32(const double *xi, double *N, double (*dN)[3]) {
+    const double x = xi[0], y = xi[1], z = xi[2];
+    const double r2 = x * x + y * y + z * z;
+    const double g = 9.0 * r2 - 19.0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const double px = (n & 1) ? 1.0 : -1.0, py = (n & 2) ? 1.0 : -1.0, pz = (n & 4) ? 1.0 : -1.0;
+        const double fx = 1.0 + x * px, fy = 1.0 + y * py, fz = 1.0 + z * pz;
+        N[n] = (1.0 / 64.0) * fx * fy * fz * g;
+        dN[n][0] = (1.0 / 64.0) * (px * fy * fz * g + fx * fy * fz * 18.0 * x);
+        dN[n][1] = (1.0 / 64.0) * (fx * py * fz * g + fx * fy * fz * 18.0 * y);
+        dN[n][2] = (1.0 / 64.0) * (fx * fy * pz * g + fx * fy * fz * 18.0 * z);
+    }
+#pragma unroll
+    for (int n = 8; n < 32; n++) {
+        const int e = (n - 8) / 8, r = (n - 8) % 8, t = r & 1, bb = r >> 1;
+        const int o1 = (e + 1) % 3, o2 = (e + 2) % 3;
+        const int a0 = o1 < o2 ? o1 : o2, a1 = o1 < o2 ? o2 : o1;
+        double p[3];
+        p[e] = t ? 1.0 / 3.0 : -1.0 / 3.0;
+        p[a0] = (bb & 1) ? 1.0 : -1.0;
+        p[a1] = (bb & 2) ? 1.0 : -1.0;
+        const double u = xi[e], fa = 1.0 - u * u, fb = 1.0 + 9.0 * u * p[e];
+        const double f1 = 1.0 + xi[o1] * p[o1], f2 = 1.0 + xi[o2] * p[o2];
+        N[n] = (9.0 / 64.0) * fa * fb * f1 * f2;
+        dN[n][e] = (9.0 / 64.0) * (-2.0 * u * fb + fa * 9.0 * p[e]) * f1 * f2;
+        dN[n][o1] = (9.0 / 64.0) * fa * fb * p[o1] * f2;
+        dN[n][o2] = (9.0 / 64.0) * fa * fb * f1 * p[o2];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,
+                                                     float4 *__restrict__ v, float4 *__restrict__ xc,
+                                                     float4 *__restrict__ vmb, double supportRadius,
+                                                     double particleRadius) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xf = x[i];
+    const double xi[3] = {(double)xf.x, (double)xf.y, (double)xf.z};
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    double dist = DBL_MAX, normal[3] = {0.0, 0.0, 0.0};
+    bool chk = true;
+    double hi[3];
+    for (int c = 0; c < 3; c++) {
+        hi[c] = d.lo[c] + d.cs[c] * d.res[c];
+        if (!(xi[c] >= d.lo[c] && xi[c] <= hi[c])) chk = false;
+    }
+    double N[32], dN[32][3], c0[3];
+    unsigned long long cell = 0;
+    if (chk) {
+        unsigned mi[3];
+        double loc[3];
+        for (int c = 0; c < 3; c++) {
+            mi[c] = (unsigned)((xi[c] - d.lo[c]) * d.inv[c]);
+            if (mi[c] >= d.res[c]) mi[c] = d.res[c] - 1;
+            const double smin = d.lo[c] + d.cs[c] * mi[c], smax = smin + d.cs[c];
+            const double denom = smax - smin;
+            c0[c] = 2.0 / denom;
+            loc[c] = c0[c] * xi[c] - (smax + smin) / denom;
+        }
+        cell = ((unsigned long long)mi[2] * d.res[1] + mi[1]) * d.res[0] + mi[0];
+        shape32(loc, N, dN);
+        const float4 *rec = reinterpret_cast<const float4 *>(d.sdf + 32ull * cell);
+        double phi = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float4 c4 = rec[q];
+            const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int m = 4 * q + u;
+                phi += (double)cv[u] * N[m];
+                gx += (double)cv[u] * dN[m][0];
+                gy += (double)cv[u] * dN[m][1];
+                gz += (double)cv[u] * dN[m][2];
+            }
+        }
+        dist = phi;
+        normal[0] = gx * c0[0];
+        normal[1] = gy * c0[1];
+        normal[2] = gz * c0[2];
+    }
+    bool animate = false;
+    if ((dist > 0.0) && (dist < supportRadius)) {
+        const float4 *rec = reinterpret_cast<const float4 *>(d.vol + 32ull * cell);
+        double volume = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float4 c4 = rec[q];
+            volume += (double)c4.x * N[4 * q] + (double)c4.y * N[4 * q + 1] + (double)c4.z * N[4 * q + 2] +
+                      (double)c4.w * N[4 * q + 3];
+        }
+        if (volume > 0.0) {
+            const double nl = sqrt((normal[0] * normal[0] + normal[1] * normal[1]) + normal[2] * normal[2]);
+            if (nl > 1.0e-9) {
+                const double dd = fmax(dist + 0.5 * particleRadius, 2.0 * particleRadius);
+                out = make_float4((float)(xi[0] - dd * normal[0] / nl), (float)(xi[1] - dd * normal[1] / nl),
+                                  (float)(xi[2] - dd * normal[2] / nl), (float)volume);
+            }
+        }
+    } else if (dist <= 0.0) {
+        animate = true;
+    }
+    if (animate) {
+        const double nl = sqrt((normal[0] * normal[0] + normal[1] * normal[1]) + normal[2] * normal[2]);
+        if (nl > 1.0e-5) {
+            double delta = 2.0 * particleRadius - dist;
+            delta = fmin(delta, 0.1 * particleRadius);
+            x[i] = make_float4((float)(xi[0] + delta * normal[0] / nl), (float)(xi[1] + delta * normal[1] / nl),
+                               (float)(xi[2] + delta * normal[2] / nl), xf.w);
+            v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            xc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    vmb[i] = out;
+}
+
+void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
+                       double supportRadius, double particleRadius, KernelTimer &T) {
+    if (!n) return;
+    TimedScope ts(T, s, "volume_map_sample");
+    k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, supportRadius, particleRadius);
+    SPH_HIP(hipGetLastError());
+}
+
+}  // namespace sph

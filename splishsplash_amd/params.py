@@ -75,7 +75,8 @@ class DFSPHCounters(C.Structure):
 
 
 def default_params(**kw):
-    """Reference constructor defaults (Simulation.cpp:66-89,
+    """Reference constructor defaults (Simulation.cpp:66-89, boundary
+    handling Bender2019 at Simulation.cpp:88,
     TimeStepDFSPH.cpp:29-41, TimeManager.cpp:12, Viscosity_Weiler2018.cpp:21-29),
     overridden by keyword arguments."""
     p = DFSPHParams()
@@ -100,7 +101,7 @@ def default_params(**kw):
     p.viscosity_boundary = 0.0
     p.visco_max_iter = 100
     p.visco_max_error = 0.01
-    p.boundary_handling_method = BOUNDARY_AKINCI2012
+    p.boundary_handling_method = BOUNDARY_BENDER2019
     for k, v in kw.items():
         if k == "gravity":
             p.gravity[:] = tuple(v)

@@ -1,0 +1,217 @@
+"""DFSPH stages on the device against the FP64 oracle from identical states.
+
+Before every stage the oracle is given the device's current state (FP32
+values, exact in FP64), both run the stage, and the stage's outputs are
+compared normwise: max|gpu - oracle| / max|oracle| <= 1e-5 (FP32 bulk
+arithmetic against the FP64 reference; the kernel-table index is decided in
+FP64 near step boundaries so both read the same table entries)."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from splishsplash_amd import capi, scenes
+from splishsplash_amd.params import BOUNDARY_BENDER2019, VISCOSITY_WEILER2018, default_params
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+STATE = ["position", "velocity", "acceleration", "density", "factor", "advected density", "p / rho^2",
+         "p_v / rho^2", "pressure acceleration", "velocity difference"]
+
+OUTPUTS = {
+    "densities": ["density"],
+    "dfsph_factor": ["factor"],
+    "divergence_setup": ["advected density", "factor", "p_v / rho^2"],
+    "divergence_iter": ["pressure acceleration", "p_v / rho^2"],
+    "divergence_finish": ["velocity", "factor", "p_v / rho^2"],
+    "clear_accel": ["acceleration"],
+    "xsph": ["acceleration"],
+    "update_velocity": ["velocity"],
+    "pressure_setup": ["advected density", "factor", "p / rho^2"],
+    "pressure_iter": ["pressure acceleration", "p / rho^2"],
+    "pressure_finish": ["velocity", "p / rho^2"],
+    "update_position": ["position"],
+}
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def sync_state(g, o, extra=()):
+    for nm in STATE + list(extra):
+        o.set(nm, g.get(nm).astype(np.float64))
+    c = g.counters()
+    o.set_time(c["time"], c["time_step_size"])
+
+
+def make_pair(params, x, v, boundary_x=None, vm=None):
+    g = capi.TimeStepDFSPH(params)
+    g.set_fluid(x, v)
+    o = oracle.OracleDFSPH(params)
+    o.set_fluid(x, v)
+    if boundary_x is not None:
+        g.add_boundary_particles(boundary_x)
+        o.add_boundary_particles(boundary_x)
+    if vm is not None:
+        g.set_volume_map_box(*vm)
+        o.set_volume_map_box(*vm)
+    o.init()
+    g.init()
+    return g, o
+
+
+def perturbed_state(x, seed=0):
+    rng = np.random.default_rng(seed)
+    v = scenes.f32(rng.normal(0.0, 0.3, x.shape))
+    p = scenes.f32(rng.uniform(0.0, 3e-4, len(x)))
+    pv = scenes.f32(rng.uniform(0.0, 0.6, len(x)))
+    return v, p, pv
+
+
+def run_stage_sequence(g, o, seq, extra_state=()):
+    report = {}
+    for st in seq:
+        sync_state(g, o, extra_state)
+        g.stage(st)
+        o.stage(st)
+        for nm in OUTPUTS.get(st, []):
+            e = rel(g.get(nm), o.get(nm))
+            report[(st, nm)] = e
+            assert e <= TOL, f"{st}: {nm} rel err {e:.3e}"
+        if st == "cfl":
+            hg, ho = g.counters()["time_step_size"], o.counters()["time_step_size"]
+            assert hg == pytest.approx(ho, rel=1e-6)
+        if st in ("divergence_iter", "pressure_iter"):
+            k = "avg_divergence_err" if st == "divergence_iter" else "avg_density_err"
+            eg, eo = g.counters()[k], o.counters()[k]
+            assert eg == pytest.approx(eo, rel=1e-4, abs=1e-9 * max(1.0, abs(eo)))
+    return report
+
+
+STEP_SEQ = ["densities", "dfsph_factor", "divergence_setup", "divergence_iter", "divergence_iter",
+            "divergence_finish", "clear_accel", "xsph", "cfl", "update_velocity", "pressure_setup",
+            "pressure_iter", "pressure_iter", "pressure_finish", "update_position"]
+
+
+def test_kernel_table_on_device():
+    rng = np.random.default_rng(1)
+    h = 0.1
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = scenes.f32(d * rng.uniform(0, 1.02 * h, (20000, 1)))
+    w, gw = capi.kernel_eval(h, r)
+    wo, go = oracle.kernel_eval(h, r)
+    assert rel(w, wo) <= 2e-6
+    assert rel(gw, go) <= 2e-6
+
+
+def test_akinci_stages_match_oracle():
+    sc = scenes.dam_break_small(per_side=30)
+    v, p, pv = perturbed_state(sc["x"], seed=1)
+    g, o = make_pair(sc["params"], sc["x"], v, boundary_x=sc["boundary_x"])
+    assert rel(g.get("boundary particle volume"), o.get_boundary_volumes(len(sc["boundary_x"]))) <= TOL
+    g.set_time(0.0, 0.004)
+    o.set_time(0.0, 0.004)
+    g.stage("neighborhood_search")
+    o.stage("neighborhood_search")
+    g.set("p / rho^2", p)
+    g.set("p_v / rho^2", pv)
+    run_stage_sequence(g, o, STEP_SEQ)
+
+
+def vm_scene(n=16, corner=0.06):
+    x = scenes.lattice_block((n, n, n), 0.05, (corner, corner, corner), jitter=0.002, seed=5)
+    # three particles inside the wall (projected back by the volume-map stage)
+    x = np.concatenate([x, scenes.f32([[-0.01, 0.5, 0.5], [0.5, -0.004, 0.4], [0.3, 0.3, -0.02]])])
+    lo, hi = (0.0, 0.0, 0.0), (2.0, 1.5, 1.5)
+    res = scenes.volume_map_resolution(lo, hi, 0.1, 0.05)
+    params = default_params(boundary_handling_method=BOUNDARY_BENDER2019, xsph=0.01, xsph_boundary=0.02,
+                            cfl_factor=1.0, time_step_size=0.004)
+    return x, params, (lo, hi, res)
+
+
+def test_volume_map_stages_match_oracle():
+    x, params, vm = vm_scene()
+    v, p, pv = perturbed_state(x, seed=2)
+    g, o = make_pair(params, x, v, vm=vm)
+    g.stage("neighborhood_search")
+    o.stage("neighborhood_search")
+    sync_state(g, o)
+    g.stage("volume_maps")
+    o.stage("volume_maps")
+    for nm in ("boundary volume", "boundary x", "position", "velocity"):
+        e = rel(g.get(nm), o.get(nm))
+        assert e <= TOL, f"volume_maps: {nm} rel err {e:.3e}"
+    vb = o.get("boundary volume")
+    assert (vb > 0).sum() > 500  # the wall is seen by the near-wall layers
+    xg = g.get("position")
+    assert (xg[-3:] != x[-3:]).any(axis=1).all()  # wall particles were projected
+    g.set("p / rho^2", p)
+    g.set("p_v / rho^2", pv)
+    run_stage_sequence(g, o, STEP_SEQ, extra_state=("boundary volume", "boundary x"))
+
+
+def test_weiler_viscosity_matches_oracle():
+    sc = scenes.dam_break_small(per_side=20)
+    prm = sc["params"]
+    prm.viscosity_method = VISCOSITY_WEILER2018
+    prm.viscosity = 0.05
+    prm.visco_max_error = 1e-5
+    prm.visco_max_iter = 200
+    rng = np.random.default_rng(3)
+    v = scenes.f32(rng.normal(0.0, 0.5, sc["x"].shape))
+    g, o = make_pair(prm, sc["x"], v, boundary_x=sc["boundary_x"])
+    g.set_time(0.0, 0.004)
+    o.set_time(0.0, 0.004)
+    for st in ("neighborhood_search", "densities"):
+        g.stage(st)
+        o.stage(st)
+    g.set("velocity difference", scenes.f32(rng.normal(0.0, 0.01, v.shape)))
+    sync_state(g, o)
+    g.stage("clear_accel")
+    o.stage("clear_accel")
+    g.stage("viscosity")
+    o.stage("viscosity")
+    vn_g = g.get("velocity") + g.get("velocity difference")
+    vn_o = o.get("velocity") + o.get("velocity difference")
+    assert rel(vn_g, vn_o) <= 1e-4
+    dv_o = o.get("velocity difference")
+    assert np.abs(g.get("velocity difference") - dv_o).max() <= 1e-4 * np.abs(vn_o).max()
+    ig, io = g.counters()["visco_iterations"], o.counters()["visco_iterations"]
+    assert abs(ig - io) <= 2, (ig, io)
+    assert io > 2
+
+
+def test_full_step_matches_oracle_and_converges():
+    sc = scenes.dam_break_small(per_side=20)
+    g, o = make_pair(sc["params"], sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    eta = sc["params"].max_error * 0.01 * 1000.0
+    for step in range(12):
+        sync_state(g, o)
+        g.step(1)
+        o.step(1)
+        cg, co = g.counters(), o.counters()
+        assert cg["iterations"] == co["iterations"], (step, cg["iterations"], co["iterations"])
+        assert cg["avg_density_err"] <= eta
+        assert cg["time_step_size"] == pytest.approx(co["time_step_size"], rel=1e-6)
+        for nm in ("position", "velocity"):
+            e = rel(g.get(nm), o.get(nm))
+            assert e <= 1e-4, f"step {step}: {nm} rel err {e:.3e}"
+    assert cg["flags"] == 0
+
+
+def test_dam_break_runs_stably():
+    sc = scenes.dam_break_small(per_side=30)
+    g = capi.TimeStepDFSPH(sc["params"])
+    g.set_fluid(sc["x"], sc["v"])
+    g.add_boundary_particles(sc["boundary_x"])
+    eta = sc["params"].max_error * 0.01 * 1000.0
+    for _ in range(40):
+        g.step(1)
+        c = g.counters()
+        assert c["avg_density_err"] <= eta or c["iterations"] == 100
+    x = g.get("position")
+    assert np.isfinite(x).all() and c["flags"] == 0
+    assert (x > -0.03).all() and (x[:, 0] < 4.03).all() and (x[:, 1] < 3.03).all() and (x[:, 2] < 2.03).all()
+    assert x[:, 1].mean() < sc["x"][:, 1].mean()

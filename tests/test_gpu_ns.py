@@ -1,0 +1,143 @@
+"""Neighbour search on the device against the oracle (bit-exact sets).
+
+The oracle evaluates the same FP32 inclusion test (strict d^2 < r^2 with
+((dx dx + dy dy) + dz dz), self excluded) on the same FP32 positions, by grid
+and by brute force (which agree, tests/test_oracle_ns.py)."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from splishsplash_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def csr_to_sets(off, idx):
+    return [frozenset(idx[off[i]:off[i + 1]].tolist()) for i in range(len(off) - 1)]
+
+
+def assert_same_csr(off_g, idx_g, off_o, idx_o):
+    np.testing.assert_array_equal(np.diff(off_g.astype(np.int64)), np.diff(off_o.astype(np.int64)))
+    for i in range(len(off_g) - 1):
+        a = np.sort(idx_g[off_g[i]:off_g[i + 1]])
+        b = np.sort(idx_o[off_o[i]:off_o[i + 1]])
+        if not np.array_equal(a, b):
+            raise AssertionError(f"point {i}: {a} vs {b}")
+
+
+def radius_for(n, k=35.0):
+    # ~k neighbours for n uniform points in the unit cube
+    return float((k / (4.0 / 3.0 * np.pi * n)) ** (1 / 3))
+
+
+@pytest.mark.parametrize("kind", ["uniform", "clustered"])
+def test_drop_in_search_matches_oracle(kind):
+    n = 60000
+    x = scenes.uniform_points(n, seed=3) if kind == "uniform" else scenes.clustered_points(n, seed=3)
+    x32 = np.ascontiguousarray(x, np.float32)
+    r = radius_for(n)
+    ns = capi.NeighborhoodSearch(r)
+    i = ns.add_point_set(x32)
+    ns.find_neighbors()
+    off_g, idx_g = ns.neighbors_csr(i, i)
+    off_o, idx_o = oracle.neighbors(x32, radius=np.float32(r))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    if kind == "uniform":
+        assert 20 < np.diff(off_o).mean() < 60
+    else:  # the clusters hold 80 % of the points in a small volume
+        assert np.diff(off_o).mean() > 60
+
+
+def test_drop_in_two_sets_activation_and_queries():
+    rng = np.random.default_rng(9)
+    fluid = np.ascontiguousarray(scenes.f32(rng.uniform(0.2, 0.8, (5000, 3))), np.float32)
+    bnd = np.ascontiguousarray(scenes.box_wall_particles((0, 0, 0), (1, 1, 1), 0.05), np.float32)
+    r = 0.1
+    ns = capi.NeighborhoodSearch(r)
+    f = ns.add_point_set(fluid, True, True, True)
+    b = ns.add_point_set(bnd, False, False, True)  # Akinci: not searching, found
+    assert ns.is_active(f, b) and ns.is_active(f, f)
+    assert not ns.is_active(b, f) and not ns.is_active(b, b)
+    ns.find_neighbors()
+    off_g, idx_g = ns.neighbors_csr(f, b)
+    off_o, idx_o = oracle.neighbors(fluid, bnd, radius=np.float32(r))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    # per-point accessors agree with the CSR view
+    ps = ns.point_set(f)
+    for k in (0, 17, 4999):
+        lst = ps.neighbor_list(f, k)
+        assert len(lst) == ps.n_neighbors(f, k)
+        if len(lst):
+            assert ps.neighbor(f, k, 0) == lst[0]
+    # boundary-only activation (Simulation::updateBoundaryVolume pattern)
+    ns.set_active(False)
+    ns.set_active(b, True, True)
+    ns.find_neighbors()
+    off_g, idx_g = ns.neighbors_csr(b, b)
+    off_o, idx_o = oracle.neighbors(bnd, radius=np.float32(r))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    # point query
+    q = np.array([0.5, 0.5, 0.5])
+    got = np.sort(ns.find_neighbors_point(q, f))
+    d2 = ((fluid.astype(np.float32) - q.astype(np.float32)) ** 2).sum(1)
+    assert set(got.tolist()) == set(np.nonzero(d2 < np.float32(r) * np.float32(r))[0].tolist())
+
+
+def test_z_sort_and_sort_field():
+    rng = np.random.default_rng(10)
+    x = np.ascontiguousarray(scenes.f32(rng.uniform(0, 1, (3000, 3))), np.float32)
+    ids = np.arange(len(x), dtype=np.uint32)
+    ns = capi.NeighborhoodSearch(0.08)
+    s = ns.add_point_set(x)
+    ns.z_sort()
+    ps = ns.point_set(s)
+    x0 = x.copy()
+    ps.sort_field(x)
+    ps.sort_field(ids)
+    np.testing.assert_array_equal(x, x0[ids])           # same permutation on both fields
+    assert sorted(ids.tolist()) == list(range(len(x)))
+    ns.find_neighbors()                                   # lists refer to the new order
+    off_g, idx_g = ns.neighbors_csr(s, s)
+    off_o, idx_o = oracle.neighbors(x, radius=np.float32(0.08))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+
+
+def test_dfsph_internal_lists_match_oracle():
+    sc = scenes.dam_break_small(per_side=30)
+    g = capi.TimeStepDFSPH(sc["params"])
+    g.set_fluid(sc["x"], sc["v"])
+    g.add_boundary_particles(sc["boundary_x"])
+    g.stage("neighborhood_search")
+    x32 = np.ascontiguousarray(sc["x"], np.float32)
+    off_g, idx_g = g.neighbors(boundary=False)
+    off_o, idx_o = oracle.neighbors(x32, radius=np.float32(0.1))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    off_g, idx_g = g.neighbors(boundary=True)
+    off_o, idx_o = oracle.neighbors(x32, np.ascontiguousarray(sc["boundary_x"], np.float32), radius=np.float32(0.1))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    cnt = g.get("fluid neighbor count")
+    np.testing.assert_array_equal(cnt, np.diff(off_o))
+
+
+def test_microbench_uniform_literal_radius_small():
+    # literal support 0.0375 at reduced n (dense neighbourhoods, ~100+)
+    n = 100000
+    x = np.ascontiguousarray(scenes.uniform_points(n, seed=3), np.float32)
+    ns = capi.NeighborhoodSearch(0.0375)
+    s = ns.add_point_set(x)
+    ns.find_neighbors()
+    off_g, idx_g = ns.neighbors_csr(s, s)
+    off_o, idx_o = oracle.neighbors(x, radius=np.float32(0.0375))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+
+
+def test_empty_and_tiny_sets():
+    ns = capi.NeighborhoodSearch(0.1)
+    e = np.zeros((0, 3), np.float32)
+    one = np.array([[0.5, 0.5, 0.5]], np.float32)
+    a = ns.add_point_set(e)
+    b = ns.add_point_set(one)
+    ns.find_neighbors()
+    assert ns.point_set(b).n_neighbors(b, 0) == 0
+    assert ns.point_set(b).n_neighbors(a, 0) == 0
+    assert ns.point_set(a).n_points() == 0
