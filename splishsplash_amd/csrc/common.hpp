@@ -143,21 +143,11 @@ __device__ inline double block_max(double v, double *lds) {
     return r;
 }
 
-// Last-block detection for grid reductions: every block publishes its
-// partial, fences, and takes a ticket; the block drawing the final ticket
-// sees all partials and finishes the reduction. Returns true in that block
-// (all threads).
-__device__ inline bool last_block(unsigned *counter) {
-    __shared__ bool amLast;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        const unsigned t = atomicAdd(counter, 1u);
-        amLast = (t == gridDim.x - 1u);
-    }
-    __syncthreads();
-    if (amLast) __threadfence();
-    return amLast;
-}
+// Grid reductions finish in a separate one-block kernel that reads the
+// per-block partials: on the MI355X the eight XCDs have separate L2 caches,
+// so a "last block" scheme needs a device-scope release fence in every block
+// (an L2 write-back each), which costs several times the sweep itself. The
+// kernel boundary provides the same visibility once per dispatch.
 
 // ------------------------------------------------------------------ timing
 // Per-kernel HIP-event timing on one stream (enabled on demand).

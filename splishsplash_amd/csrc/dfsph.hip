@@ -21,6 +21,8 @@
 
 namespace sph {
 
+constexpr int RBLOCK = 1024;  // one-block reductions
+
 __device__ __forceinline__ unsigned gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 __device__ __forceinline__ float3 f3(const float4 &a) { return make_float3(a.x, a.y, a.z); }
@@ -138,21 +140,39 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
 // ------------------------------------------------- pressure accelerations
 // computePressureAccel (TimeStepDFSPH.cpp:1299-1367) for particle i with
 // pressure field P.
+__device__ __forceinline__ void accel_pair(const Dev &D, const float4 &xi, float pi, unsigned j, float pj, float &ax,
+                                           float &ay, float &az) {
+    const float ps = pi + pj;
+    if (fabsf(ps) > D.eps) {
+        float3 r;
+        const float g = kernel_G(xi, D.x[j], D.K, r);
+        ax += ps * (-D.V * (g * r.x));
+        ay += ps * (-D.V * (g * r.y));
+        az += ps * (-D.V * (g * r.z));
+    }
+}
+
 __device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float *__restrict__ P) {
     const float4 xi = D.x[i];
     const float pi = P[i];
     float ax = 0.f, ay = 0.f, az = 0.f;
     const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float ps = pi + P[j];
-        if (fabsf(ps) > D.eps) {
-            float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
-            ax += ps * (-D.V * (g * r.x));
-            ay += ps * (-D.V * (g * r.y));
-            az += ps * (-D.V * (g * r.z));
-        }
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    unsigned k = 0;
+    // four neighbours per trip: the index and pressure loads of all four are
+    // issued before the first use, so each wave keeps more gathers in flight
+    for (; k + 4 <= cnt; k += 4) {
+        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
+        accel_pair(D, xi, pi, j0, p0, ax, ay, az);
+        accel_pair(D, xi, pi, j1, p1, ax, ay, az);
+        accel_pair(D, xi, pi, j2, p2, ax, ay, az);
+        accel_pair(D, xi, pi, j3, p3, ax, ay, az);
+    }
+    for (; k < cnt; k++) {
+        const unsigned j = L[k * S];
+        accel_pair(D, xi, pi, j, P[j], ax, ay, az);
     }
     if (fabsf(pi) > D.eps) {
         if (D.bmode == 1) {
@@ -188,6 +208,13 @@ __global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
     D.ap[i] = make_float4(a.x, a.y, a.z, 0.f);
 }
 
+__device__ __forceinline__ float aij_term(const float4 &xi, const float4 &ai, const float4 &xj, const float4 &aj,
+                                          const KernelConst &K) {
+    float3 r;
+    const float g = kernel_G(xi, xj, K, r);
+    return ((ai.x - aj.x) * (g * r.x) + (ai.y - aj.y) * (g * r.y)) + (ai.z - aj.z) * (g * r.z);
+}
+
 // compute_aij_pj (TimeStepDFSPH.cpp:1370-1420) and the Jacobi update of
 // pressureSolveIteration / divergenceSolveIteration (:544-706), FP64 error
 // sum finished in the last block.
@@ -204,12 +231,21 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
         const float4 ai = D.ap[i];
         float s = 0.f;
         const unsigned cnt = D.cf[i];
-        for (unsigned k = 0; k < cnt; k++) {
-            const unsigned j = D.nf[(size_t)k * D.stride + i];
-            const float4 aj = D.ap[j];
-            float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
-            s += ((ai.x - aj.x) * (g * r.x) + (ai.y - aj.y) * (g * r.y)) + (ai.z - aj.z) * (g * r.z);
+        const unsigned *__restrict__ L = D.nf + i;
+        const size_t S = D.stride;
+        unsigned k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+            const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+            const float4 a0 = D.ap[j0], a1 = D.ap[j1], a2 = D.ap[j2], a3 = D.ap[j3];
+            s += aij_term(xi, ai, x0, a0, D.K);
+            s += aij_term(xi, ai, x1, a1, D.K);
+            s += aij_term(xi, ai, x2, a2, D.K);
+            s += aij_term(xi, ai, x3, a3, D.K);
+        }
+        for (; k < cnt; k++) {
+            const unsigned j = L[k * S];
+            s += aij_term(xi, ai, D.x[j], D.ap[j], D.K);
         }
         s *= D.V;
         unsigned nnb = cnt;
@@ -248,28 +284,36 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
     }
     const double bs = block_sum(err, lds);
     if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
-    if (last_block(&C->ticket)) {
-        double t = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += D.partial[b];
-        t = block_sum(t, lds);
-        if (threadIdx.x == 0) {
-            const double avg = D.n ? t / (double)D.n : 0.0;
-            const bool finite = isfinite(avg);
-            if (!finite) C->flags |= 1;
-            if (PRESSURE) {
-                C->avg_err = avg;
-                C->it += 1;
-                const double eta = D.maxError * 0.01 * (double)D.rho0;
-                const bool chk = avg <= eta;
-                C->done = ((chk && C->it >= D.minIt) || C->it >= D.maxIt || !finite) ? 1 : 0;
-            } else {
-                C->avg_err_v = avg;
-                C->it_v += 1;
-                const double eta = (1.0 / h) * D.maxErrorV * 0.01 * (double)D.rho0;
-                const bool chk = avg <= eta;
-                C->done_v = ((chk && C->it_v >= 1) || C->it_v >= D.maxItV || !finite) ? 1 : 0;
-            }
-            C->ticket = 0;
+}
+
+// Sums the per-block error partials of k_jacobi in a fixed order (one block)
+// and decides convergence (TimeStepDFSPH.cpp:324-341, 477-495). The kernel
+// boundary makes every partial visible without device-scope fences.
+template <bool PRESSURE>
+__global__ __launch_bounds__(RBLOCK) void k_jacobi_reduce(Dev D, unsigned nparts, int check_done) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    if (check_done && (PRESSURE ? C->done : C->done_v)) return;
+    double t = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t += D.partial[b];
+    t = block_sum(t, lds);
+    if (threadIdx.x == 0) {
+        const double h = C->h;
+        const double avg = D.n ? t / (double)D.n : 0.0;
+        const bool finite = isfinite(avg);
+        if (!finite) C->flags |= 1;
+        if (PRESSURE) {
+            C->avg_err = avg;
+            C->it += 1;
+            const double eta = D.maxError * 0.01 * (double)D.rho0;
+            const bool chk = avg <= eta;
+            C->done = ((chk && C->it >= D.minIt) || C->it >= D.maxIt || !finite) ? 1 : 0;
+        } else {
+            C->avg_err_v = avg;
+            C->it_v += 1;
+            const double eta = (1.0 / h) * D.maxErrorV * 0.01 * (double)D.rho0;
+            const bool chk = avg <= eta;
+            C->done_v = ((chk && C->it_v >= 1) || C->it_v >= D.maxItV || !finite) ? 1 : 0;
         }
     }
 }
@@ -378,30 +422,35 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
     if (CFL) {
         const double bm = block_max(vm, lds);
         if (threadIdx.x == 0) D.partial[blockIdx.x] = bm;
-        if (last_block(&C->ticket)) {
-            double t = 0.0;
-            for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t = fmax(t, D.partial[b]);
-            t = block_max(t, lds);
-            if (threadIdx.x == 0) {
-                double maxVel = t;
-                if (!isfinite(maxVel)) C->flags |= 1;
-                if (maxVel < 1.0e-9) maxVel = 1.0e-9;
-                const double diameter = 2.0 * D.radius;
-                double hn = D.cflFactor * 0.4 * (diameter / sqrt(maxVel));
-                hn = fmin(hn, D.cflMax);
-                hn = fmax(hn, D.cflMin);
-                if (D.cflMethod == 2 && C->it != 0) {
-                    double hh = h;
-                    if (C->it > 10)
-                        hh *= 0.9;
-                    else if (C->it < 5)
-                        hh *= 1.1;
-                    hn = fmin(hh, hn);
-                }
-                C->h = hn;
-                C->ticket = 0;
-            }
+    }
+}
+
+// CFL time step from the per-block maxima of |v + a h|^2
+// (Simulation.cpp:415-493, cflMethod 2 at :399-412).
+__global__ __launch_bounds__(RBLOCK) void k_cfl_reduce(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    double t = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t = fmax(t, D.partial[b]);
+    t = block_max(t, lds);
+    if (threadIdx.x == 0) {
+        const double h = C->h;
+        double maxVel = t;
+        if (!isfinite(maxVel)) C->flags |= 1;
+        if (maxVel < 1.0e-9) maxVel = 1.0e-9;
+        const double diameter = 2.0 * D.radius;
+        double hn = D.cflFactor * 0.4 * (diameter / sqrt(maxVel));
+        hn = fmin(hn, D.cflMax);
+        hn = fmax(hn, D.cflMin);
+        if (D.cflMethod == 2 && C->it != 0) {
+            double hh = h;
+            if (C->it > 10)
+                hh *= 0.9;
+            else if (C->it < 5)
+                hh *= 1.1;
+            hn = fmin(hh, hn);
         }
+        C->h = hn;
     }
 }
 
@@ -653,10 +702,10 @@ __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, const floa
 }
 
 // Initial residual r = b - A x (b = v for static boundaries), z = M^-1 r,
-// p = z; last block sets up Eigen's CG state (ConjugateGradient.h:43-70).
+// p = z; k_cg_init_reduce then sets up Eigen's CG state
+// (ConjugateGradient.h:43-70).
 __global__ __launch_bounds__(BLOCK) void k_cg_init(Dev D) {
     __shared__ double lds[BLOCK / 64];
-    Ctl *C = D.ctl;
     const unsigned i = gtid();
     double b2 = 0.0, r2 = 0.0, rz = 0.0;
     if (i < D.n) {
@@ -677,29 +726,31 @@ __global__ __launch_bounds__(BLOCK) void k_cg_init(Dev D) {
         D.partial[3 * blockIdx.x + 1] = s1;
         D.partial[3 * blockIdx.x + 2] = s2;
     }
-    if (last_block(&C->ticket)) {
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            t0 += D.partial[3 * b];
-            t1 += D.partial[3 * b + 1];
-            t2 += D.partial[3 * b + 2];
-        }
-        t0 = block_sum(t0, lds);
-        t1 = block_sum(t1, lds);
-        t2 = block_sum(t2, lds);
-        if (threadIdx.x == 0) {
-            C->cg_rhs2 = t0;
-            C->cg_r2 = t1;
-            C->cg_absNew = t2;
-            C->cg_it = 0;
-            if (t0 == 0.0) {
-                C->cg_done = 2;  // x = 0, no iteration
-            } else {
-                const double thr = fmax(D.viscoTol * D.viscoTol * t0, DBL_MIN);
-                C->cg_thresh = thr;
-                C->cg_done = (t1 < thr || D.viscoMaxIt <= 0) ? 1 : 0;
-            }
-            C->ticket = 0;
+}
+
+__global__ __launch_bounds__(RBLOCK) void k_cg_init_reduce(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
+        t0 += D.partial[3 * b];
+        t1 += D.partial[3 * b + 1];
+        t2 += D.partial[3 * b + 2];
+    }
+    t0 = block_sum(t0, lds);
+    t1 = block_sum(t1, lds);
+    t2 = block_sum(t2, lds);
+    if (threadIdx.x == 0) {
+        C->cg_rhs2 = t0;
+        C->cg_r2 = t1;
+        C->cg_absNew = t2;
+        C->cg_it = 0;
+        if (t0 == 0.0) {
+            C->cg_done = 2;  // x = 0, no iteration
+        } else {
+            const double thr = fmax(D.viscoTol * D.viscoTol * t0, DBL_MIN);
+            C->cg_thresh = thr;
+            C->cg_done = (t1 < thr || D.viscoMaxIt <= 0) ? 1 : 0;
         }
     }
 }
@@ -719,15 +770,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_matvec(Dev D) {
     }
     const double bs = block_sum(pt, lds);
     if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
-    if (last_block(&C->ticket)) {
-        double t = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += D.partial[b];
-        t = block_sum(t, lds);
-        if (threadIdx.x == 0) {
-            C->cg_alpha = C->cg_absNew / t;
-            C->ticket = 0;
-        }
-    }
+}
+
+__global__ __launch_bounds__(RBLOCK) void k_cg_alpha(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->cg_done) return;
+    double t = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t += D.partial[b];
+    t = block_sum(t, lds);
+    if (threadIdx.x == 0) C->cg_alpha = C->cg_absNew / t;
 }
 
 // x += alpha p, r -= alpha tmp, z = M^-1 r; convergence test on |r|^2 and
@@ -760,27 +812,30 @@ __global__ __launch_bounds__(BLOCK) void k_cg_update(Dev D) {
         D.partial[2 * blockIdx.x] = s1;
         D.partial[2 * blockIdx.x + 1] = s2;
     }
-    if (last_block(&C->ticket)) {
-        double t1 = 0.0, t2 = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            t1 += D.partial[2 * b];
-            t2 += D.partial[2 * b + 1];
-        }
-        t1 = block_sum(t1, lds);
-        t2 = block_sum(t2, lds);
-        if (threadIdx.x == 0) {
-            C->cg_r2 = t1;
-            if (!isfinite(t1)) C->flags |= 1;
-            if (t1 < C->cg_thresh || !isfinite(t1)) {
-                C->cg_done = 1;
-            } else {
-                const double absOld = C->cg_absNew;
-                C->cg_absNew = t2;
-                C->cg_beta = t2 / absOld;
-                C->cg_it += 1;
-                if (C->cg_it >= D.viscoMaxIt) C->cg_done = 3;  // p no longer needed
-            }
-            C->ticket = 0;
+}
+
+__global__ __launch_bounds__(RBLOCK) void k_cg_update_reduce(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->cg_done) return;
+    double t1 = 0.0, t2 = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
+        t1 += D.partial[2 * b];
+        t2 += D.partial[2 * b + 1];
+    }
+    t1 = block_sum(t1, lds);
+    t2 = block_sum(t2, lds);
+    if (threadIdx.x == 0) {
+        C->cg_r2 = t1;
+        if (!isfinite(t1)) C->flags |= 1;
+        if (t1 < C->cg_thresh || !isfinite(t1)) {
+            C->cg_done = 1;
+        } else {
+            const double absOld = C->cg_absNew;
+            C->cg_absNew = t2;
+            C->cg_beta = t2 / absOld;
+            C->cg_it += 1;
+            if (C->cg_it >= D.viscoMaxIt) C->cg_done = 3;  // p no longer needed
         }
     }
 }
@@ -1097,6 +1152,13 @@ void DFSPH::volume_maps() {
         kern<<<nblocks(n_), BLOCK, 0, stream>>>(__VA_ARGS__);     \
         SPH_HIP(hipGetLastError());                               \
     } while (0)
+// One-block kernels finishing a grid reduction.
+#define REDUCE(name, kern, ...)                                   \
+    do {                                                          \
+        TimedScope ts_(timer, stream, name);                      \
+        kern<<<1, RBLOCK, 0, stream>>>(__VA_ARGS__);              \
+        SPH_HIP(hipGetLastError());                               \
+    } while (0)
 
 void DFSPH::sweep_init(bool dens, bool fact, bool div) {
     const Dev D = dev();
@@ -1116,12 +1178,14 @@ void DFSPH::div_iteration(bool force) {
     const Dev D = dev();
     LAUNCH("pressure_accel_v", k_accel<false>, D, force ? 0 : 1);
     LAUNCH("jacobi_v", k_jacobi<false>, D, force ? 0 : 1);
+    REDUCE("jacobi_reduce", k_jacobi_reduce<false>, D, nblocks(n_), force ? 0 : 1);
 }
 
 void DFSPH::pressure_iteration(bool force) {
     const Dev D = dev();
     LAUNCH("pressure_accel", k_accel<true>, D, force ? 0 : 1);
     LAUNCH("jacobi", k_jacobi<true>, D, force ? 0 : 1);
+    REDUCE("jacobi_reduce", k_jacobi_reduce<true>, D, nblocks(n_), force ? 0 : 1);
 }
 
 void DFSPH::div_finish(bool scale_pv) {
@@ -1188,6 +1252,7 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
         }
 #undef POSTV
     }
+    if (cfl) REDUCE("cfl_reduce", k_cfl_reduce, D, nblocks(n_));
     if (vupd) v_.swap(v2_);
 }
 
@@ -1215,11 +1280,14 @@ int DFSPH::run_cg() {
     LAUNCH("weiler_diag", k_weiler_diag, D);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 3);
     LAUNCH("cg_init", k_cg_init, D);
+    REDUCE("cg_reduce", k_cg_init_reduce, D, nblocks(n_));
     int launched = 0, chunk = std::max(1, lastCg_ + 1);
     for (;;) {
         for (int c = 0; c < chunk && launched < P.visco_max_iter; c++, launched++) {
             LAUNCH("cg_matvec", k_cg_matvec, D);
+            REDUCE("cg_reduce", k_cg_alpha, D, nblocks(n_));
             LAUNCH("cg_update", k_cg_update, D);
+            REDUCE("cg_reduce", k_cg_update_reduce, D, nblocks(n_));
             LAUNCH("cg_pupdate", k_cg_pupdate, D);
         }
         read_ctl();

@@ -168,13 +168,19 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const unsigned *__restrict__ 
     if (k < n) dst[k] = src[perm[k]];
 }
 
-// 27-cell sweep, one thread per query, column-major output.
+// 27-cell sweep, one thread per query, column-major output. Each wave
+// appends its lanes' neighbours into its own LDS tile [cap][64] and then
+// writes the tile row by row: row k of a wave is 64 consecutive words, one
+// coalesced store, instead of scattered 4-byte appends that leave partial
+// cache lines behind. The candidates of the 9 row ranges are read straight
+// from the cell-sorted array (neighbouring queries share them in L1/L2).
 template <bool SAME>
-__global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q, unsigned nq,
-                                                    const float4 *__restrict__ src,
-                                                    const unsigned *__restrict__ start, GridDesc g, float r2,
-                                                    unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
-                                                    unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+__global__ void k_lists_cm(const float4 *__restrict__ q, unsigned nq, const float4 *__restrict__ src,
+                           const unsigned *__restrict__ start, GridDesc g, float r2, unsigned *__restrict__ nbr,
+                           unsigned stride, unsigned cap, unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+    extern __shared__ unsigned tile[];
+    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    unsigned *__restrict__ wt = tile + (size_t)wid * cap * 64u;
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned k = 0;
     if (i < nq) {
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q
                     const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
                     const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
                     if (d2 < r2) {
-                        if (k < cap) nbr[(size_t)k * stride + i] = j;
+                        if (k < cap) wt[k * 64u + lane] = j;
                         k++;
                     }
                 }
@@ -203,8 +209,12 @@ __global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q
         }
         cnt[i] = k;
     }
-    const unsigned m = wave_max_u(k);
-    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(maxcnt, m);
+    const unsigned kk = k < cap ? k : cap;
+    const unsigned m = wave_max_u(kk);
+    for (unsigned r = 0; r < m; r++)
+        if (r < kk) nbr[(size_t)r * stride + i] = wt[r * 64u + lane];
+    const unsigned mk = wave_max_u(k);
+    if (lane == 0 && mk > 0) atomicMax(maxcnt, mk);
 }
 
 template <bool SAME, bool FILL>
@@ -310,11 +320,18 @@ void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const flo
                           const GridDesc &g, float r2, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
                           unsigned *cnt, unsigned *maxcnt, KernelTimer &T, const char *name) {
     if (!nq) return;
+    SPH_REQUIRE(cap >= 1, SPH_ERR_INVALID, "neighbour capacity out of range");
     TimedScope ts(T, s, name);
+    // waves per block so that the LDS tiles stay within 64 KiB per block
+    const unsigned waves = cap <= 64 ? 4u : (cap <= 128 ? 2u : 1u);
+    const unsigned threads = 64u * waves;
+    const size_t lds = (size_t)waves * cap * 64u * sizeof(unsigned);
+    SPH_REQUIRE(lds <= 65536, SPH_ERR_INVALID, "neighbour capacity above 256 does not fit the LDS tile");
+    const unsigned grid = (nq + threads - 1) / threads;
     if (same_set)
-        k_lists_cm<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<true><<<grid, threads, lds, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
     else
-        k_lists_cm<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<false><<<grid, threads, lds, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
     SPH_HIP(hipGetLastError());
 }
 

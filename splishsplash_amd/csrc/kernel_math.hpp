@@ -7,11 +7,11 @@
 //   p        = min(floor(|r| / s), 9998),      s = h / 9999
 //
 // The table entries are evaluated in closed form from q = i / 9999 instead of
-// being read from memory, so the gather-bound sweeps spend no bandwidth on
-// the table. The index p is taken from an FP32 distance; when |r|/s lies
-// within GUARD of an integer the FP32 value could round to the other side of
-// the step, and p is recomputed from the FP64 distance of the original FP32
-// coordinates -- the value an FP64 build computes from the same positions.
+// being read from memory, so the sweeps spend no bandwidth on the table. The
+// index p is taken from an FP32 distance; when |r|/s lies close enough to an
+// integer that the FP32 value could round to the other side of the step, p
+// is recomputed from the FP64 distance of the original FP32 coordinates --
+// the value an FP64 build computes from the same positions.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,19 +23,24 @@ struct KernelConst {
     float k, l;         // 8/(pi h^3), 48/(pi h^3)
     float W0;           // W(0) = k
     float invH2;        // 1 / h^2
+    float lh2;          // l / h^2
     float stepq;        // 1 / 9999
     double invStep_d;   // 1 / (h / 9999) in double, as the reference computes it
+    double invStep2_d;  // invStep_d^2
     double h_d;
 };
 
 KernelConst make_kernel_const(double h);
 
 constexpr unsigned KRES = 10000u;
-constexpr float GUARD = 2.5e-3f;
 
-// T[i] = W_cubic(q_i), SPHKernels.h:36-54
-__device__ __forceinline__ float table_T(unsigned i, const KernelConst &K) {
-    const float q = (float)i * K.stepq;
+// The table average 0.5 (T[p] + T[p+1]) is evaluated as the smooth kernel
+// at the step midpoint q = (p + 1/2) s / h: both W and G = W'(r)/r are C1
+// across q = 1/2 and smooth on each side, so the two differ by
+// f'' s^2 / 8, about 1e-8 of the value for s = h / 9999. G[0] = 0 (the
+// reference's special case at r = 0) is kept exactly.
+__device__ __forceinline__ float table_W_avg(unsigned p, const KernelConst &K) {
+    const float q = ((float)p + 0.5f) * K.stepq;
     if (q <= 0.5f) {
         const float q2 = q * q;
         return K.k * (6.0f * q2 * q - 6.0f * q2 + 1.0f);
@@ -43,29 +48,39 @@ __device__ __forceinline__ float table_T(unsigned i, const KernelConst &K) {
     const float f = 1.0f - q;
     return K.k * (2.0f * f * f * f);
 }
-
-// G[i] = gradW_cubic((i s, 0, 0))_x / (i s), SPHKernels.h:61-85 and :637-641
-__device__ __forceinline__ float table_G(unsigned i, const KernelConst &K) {
-    if (i == 0u) return 0.0f;
-    const float q = (float)i * K.stepq;
-    if (q <= 0.5f) return K.l * (3.0f * q - 2.0f) * K.invH2;
-    const float f = 1.0f - q;
-    return -K.l * f * f * K.invH2 / q;
+__device__ __forceinline__ float table_G_avg(unsigned p, const KernelConst &K) {
+    const float q = (p == 0u) ? K.stepq : ((float)p + 0.5f) * K.stepq;
+    float g;
+    if (q <= 0.5f) {
+        g = K.lh2 * (3.0f * q - 2.0f);
+    } else {
+        const float f = 1.0f - q;
+        g = -K.lh2 * f * f * __builtin_amdgcn_rcpf(q);
+    }
+    return (p == 0u) ? 0.5f * g : g;
 }
 
 __device__ __forceinline__ float sqn(float x, float y, float z) { return (x * x + y * y) + z * z; }
 
-// Table position for the pair (xi, xj) with FP32 squared distance d2.
+// Table position for the pair (xi, xj) with FP32 squared distance d2. The
+// FP32 value t = |r| / s is within 3.9e-7 t of the FP64 one (coordinate
+// differences, squares and sum: 3e-7 on d2, halved by the square root;
+// 1-ulp square root; invStep rounding; product). Inside a band of 5e-7 t
+// around an integer m the step is decided in FP64 from the original FP32
+// coordinates: floor(|r| / s) >= m  <=>  |r|^2 / s^2 >= m^2, which is what an
+// FP64 build computes from the same positions up to its last-bit rounding.
 __device__ __forceinline__ unsigned table_pos(const float4 &xi, const float4 &xj, float d2, const KernelConst &K) {
-    const float t = sqrtf(d2) * K.invStep_f;
+    const float t = __builtin_amdgcn_sqrtf(d2) * K.invStep_f;
     unsigned pos = (unsigned)t;
-    const float fr = t - (float)pos;
-    if (fr < GUARD || fr > 1.0f - GUARD) {
+    const float m = rintf(t);
+    if (fabsf(t - m) < 5.0e-7f * t + 2.0e-6f) {
         const double dx = (double)xi.x - (double)xj.x;
         const double dy = (double)xi.y - (double)xj.y;
         const double dz = (double)xi.z - (double)xj.z;
         const double d2d = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-        pos = (unsigned)__dmul_rn(__dsqrt_rn(d2d), K.invStep_d);
+        const double md = (double)m;
+        const unsigned mi = (unsigned)m;
+        pos = (__dmul_rn(d2d, K.invStep2_d) >= md * md) ? mi : mi - 1u;
     }
     return pos < KRES - 2u ? pos : KRES - 2u;
 }
@@ -75,8 +90,7 @@ __device__ __forceinline__ float kernel_W(const float4 &xi, const float4 &xj, co
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float d2 = sqn(dx, dy, dz);
     if (!(d2 <= K.h2)) return 0.0f;
-    const unsigned p = table_pos(xi, xj, d2, K);
-    return 0.5f * (table_T(p, K) + table_T(p + 1u, K));
+    return table_W_avg(table_pos(xi, xj, d2, K), K);
 }
 
 // Scalar g with gradW(xi - xj) = g (xi - xj); zero outside the support.
@@ -85,8 +99,7 @@ __device__ __forceinline__ float kernel_G(const float4 &xi, const float4 &xj, co
     r = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
     const float d2 = sqn(r.x, r.y, r.z);
     if (!(d2 <= K.h2)) return 0.0f;
-    const unsigned p = table_pos(xi, xj, d2, K);
-    return 0.5f * (table_G(p, K) + table_G(p + 1u, K));
+    return table_G_avg(table_pos(xi, xj, d2, K), K);
 }
 
 // Both W and the gradient scalar for one pair.
@@ -100,8 +113,8 @@ __device__ __forceinline__ void kernel_WG(const float4 &xi, const float4 &xj, co
         return;
     }
     const unsigned p = table_pos(xi, xj, d2, K);
-    w = 0.5f * (table_T(p, K) + table_T(p + 1u, K));
-    g = 0.5f * (table_G(p, K) + table_G(p + 1u, K));
+    w = table_W_avg(p, K);
+    g = table_G_avg(p, K);
 }
 
 __device__ __forceinline__ float dot3(const float3 &a, const float3 &b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }

@@ -45,10 +45,12 @@ KernelConst make_kernel_const(double h) {
     const double stepSize = h / (double)(KRES - 1);
     K.invStep_d = 1.0 / stepSize;
     K.invStep_f = (float)K.invStep_d;
+    K.invStep2_d = K.invStep_d * K.invStep_d;
     K.k = (float)(8.0 / (pi * h3));
     K.l = (float)(48.0 / (pi * h3));
     K.W0 = K.k;
     K.invH2 = (float)(1.0 / (h * h));
+    K.lh2 = (float)(48.0 / (pi * h3) / (h * h));
     K.stepq = (float)(1.0 / (double)(KRES - 1));
     K.h_d = h;
     return K;

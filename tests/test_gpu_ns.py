@@ -112,15 +112,16 @@ def test_dfsph_internal_lists_match_oracle():
     off_g, idx_g = g.neighbors(boundary=False)
     off_o, idx_o = oracle.neighbors(x32, radius=np.float32(0.1))
     assert_same_csr(off_g, idx_g, off_o, idx_o)
+    fluid_counts = np.diff(off_o)
     off_g, idx_g = g.neighbors(boundary=True)
     off_o, idx_o = oracle.neighbors(x32, np.ascontiguousarray(sc["boundary_x"], np.float32), radius=np.float32(0.1))
     assert_same_csr(off_g, idx_g, off_o, idx_o)
     cnt = g.get("fluid neighbor count")
-    np.testing.assert_array_equal(cnt, np.diff(off_o))
+    np.testing.assert_array_equal(cnt, fluid_counts)
 
 
 def test_microbench_uniform_literal_radius_small():
-    # literal support 0.0375 at reduced n (dense neighbourhoods, ~100+)
+    # the microbenchmark's literal support radius 0.0375 at a reduced n
     n = 100000
     x = np.ascontiguousarray(scenes.uniform_points(n, seed=3), np.float32)
     ns = capi.NeighborhoodSearch(0.0375)
