@@ -49,6 +49,23 @@ BYTES = {
 }
 
 
+# bench kernel names -> device kernel symbols in the rocprofv3 traffic record
+SYMBOL = {"jacobi": "k_jacobi", "jacobi_v": "k_jacobi", "pressure_accel": "k_accel", "pressure_accel_v": "k_accel",
+          "fluid_lists": "k_lists_cm"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    record (FETCH_SIZE/WRITE_SIZE scaled by a copy calibration; the commands
+    are in profiles/dam1m_traffic_k_jacobi.md), or None."""
+    path = os.path.join(ROOT, "profiles", "dam1m_traffic.json")
+    try:
+        rec = json.load(open(path))
+        return rec["kernels"][SYMBOL[kernel]]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -149,7 +166,7 @@ def main():
             bpp = BYTES[best](K)
             achieved = n * bpp / (kernels[best]["avg_us"] * 1e-6) / 1e9
             roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": best,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(best), "kernel": best,
                         "bytes_per_particle": bpp, "avg_launch_us": round(kernels[best]["avg_us"], 2)}
             for name, s in kernels.items():
                 if name in BYTES:
