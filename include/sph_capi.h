@@ -265,6 +265,24 @@ int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name,
 int sph_dfsph_reset_kernel_stats(sph_dfsph_t s);
 int sph_dfsph_synchronize(sph_dfsph_t s);
 
+/* ------------------------------------------------------------------------ */
+/* Learned warm start (no counterpart in the reference): a per-particle MLP   */
+/* 6 -> 64 -> 64 -> 1 with SiLU, BF16 MFMA with FP32 accumulation. theta is  */
+/* the flat parameter vector W1 (6x64 [in][out]), b1 (64), W2 (64x64), b2     */
+/* (64), W3 (64), b3 (1); inputs are rows of 8 floats (6 features + 2 zeros). */
+/* ------------------------------------------------------------------------ */
+int sph_mlp_num_params(void);
+/* y[i] = net(x[i]), host buffers. */
+int sph_mlp_forward(const float *theta, const float *x, size_t n, float *y);
+/* Mean squared error over the n samples and its gradient (host buffers). */
+int sph_mlp_loss_grad(const float *theta, const float *x, const float *t,
+                      size_t n, float *grad, float *loss);
+/* `steps` Adam steps (lr, beta 0.9/0.999) on minibatches of `batch` rows of
+ * a seeded shuffle of the samples; theta is updated in place and the final
+ * loss over all samples is returned in *loss. */
+int sph_mlp_train(float *theta, const float *x, const float *t, size_t n,
+                  int steps, int batch, double lr, unsigned seed, float *loss);
+
 /* Cubic-spline table kernel (PrecomputedKernel<CubicKernel,10000>,
  * SPHKernels.h:614-691) evaluated on the device for n FP32 offsets r (3n
  * floats): W into w[n], gradW into g[3n]. For kernel checks. */

@@ -83,6 +83,10 @@ _SIGS = {
     "sph_dfsph_reset_kernel_stats": (C.c_int, [_vp]),
     "sph_dfsph_synchronize": (C.c_int, [_vp]),
     "sph_kernel_eval": (C.c_int, [C.c_double, _fp, C.c_size_t, _fp, _fp]),
+    "sph_mlp_num_params": (C.c_int, []),
+    "sph_mlp_forward": (C.c_int, [_fp, _fp, C.c_size_t, _fp]),
+    "sph_mlp_loss_grad": (C.c_int, [_fp, _fp, _fp, C.c_size_t, _fp, _fp]),
+    "sph_mlp_train": (C.c_int, [_fp, _fp, _fp, C.c_size_t, C.c_int, C.c_int, C.c_double, C.c_uint, _fp]),
 }
 
 
@@ -126,6 +130,48 @@ def kernel_eval(support_radius, r):
     _chk(lib().sph_kernel_eval(support_radius, r.ctypes.data_as(_fp), n, w.ctypes.data_as(_fp),
                                g.ctypes.data_as(_fp)))
     return w, g
+
+
+MLP_NPARAM = 4673
+
+
+def _mlp_inputs(x):
+    x = np.asarray(x, np.float32).reshape(len(x), -1)
+    out = np.zeros((len(x), 8), np.float32)
+    out[:, :min(6, x.shape[1])] = x[:, :6]
+    return out
+
+
+def mlp_forward(theta, x):
+    """Device BF16-MFMA evaluation of the warm-start network on rows of 6
+    features."""
+    th = np.ascontiguousarray(theta, np.float32)
+    xx = _mlp_inputs(x)
+    y = np.zeros(len(xx), np.float32)
+    _chk(lib().sph_mlp_forward(th.ctypes.data_as(_fp), xx.ctypes.data_as(_fp), len(xx), y.ctypes.data_as(_fp)))
+    return y
+
+
+def mlp_loss_grad(theta, x, t):
+    th = np.ascontiguousarray(theta, np.float32)
+    xx = _mlp_inputs(x)
+    tt = np.ascontiguousarray(t, np.float32)
+    g = np.zeros(MLP_NPARAM, np.float32)
+    loss = C.c_float(0)
+    _chk(lib().sph_mlp_loss_grad(th.ctypes.data_as(_fp), xx.ctypes.data_as(_fp), tt.ctypes.data_as(_fp), len(xx),
+                                 g.ctypes.data_as(_fp), C.byref(loss)))
+    return loss.value, g
+
+
+def mlp_train(theta, x, t, steps=1000, batch=4096, lr=1e-3, seed=0):
+    """Adam on the device; returns (new theta, final loss)."""
+    th = np.ascontiguousarray(theta, np.float32).copy()
+    xx = _mlp_inputs(x)
+    tt = np.ascontiguousarray(t, np.float32)
+    loss = C.c_float(0)
+    _chk(lib().sph_mlp_train(th.ctypes.data_as(_fp), xx.ctypes.data_as(_fp), tt.ctypes.data_as(_fp), len(xx),
+                             steps, batch, lr, seed, C.byref(loss)))
+    return th, loss.value
 
 
 class PointSet:

@@ -1,0 +1,455 @@
+// mlp.hip -- BF16 MFMA inference and training kernels of the warm-start
+// network (see mlp.hpp).
+//
+// One wave computes a tile of 32 rows with v_mfma_f32_32x32x8bf16_1k
+// (32x8 A, 8x32 B, 32x32 FP32 accumulator). Operand layouts of that
+// instruction, lane l = 32 g + r:
+//   A[m][k]: lane holds m = r, k = 4g .. 4g+3
+//   B[k][n]: lane holds n = r, k = 4g .. 4g+3
+//   C[m][n]: lane holds n = r, element e at m = 8 (e / 4) + 4 g + e % 4
+// Between layers the activations go through LDS ([32][SP] BF16 per wave) to
+// turn the C layout into the next A layout.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "mlp.hpp"
+
+namespace sph {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int SP = 68;        // LDS row pitch (BF16 elements)
+constexpr int MWAVES = 4;     // waves per block
+
+__device__ __forceinline__ short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int crow(int e, int g) { return 8 * (e >> 2) + 4 * g + (e & 3); }
+__device__ __forceinline__ f32x16 zero16() {
+    f32x16 z;
+#pragma unroll
+    for (int e = 0; e < 16; e++) z[e] = 0.0f;
+    return z;
+}
+
+struct Wregs {
+    bf16x4 B1[2];     // layer 1: K = features (6 + 2 zero), two 32-column halves
+    bf16x4 B2[8][2];  // layer 2: 8 K-chunks, two halves
+    bf16x4 B3[8];     // layer 3: only column 0 non-zero
+    float b1[2], b2[2], w3[2], b3;
+};
+
+__device__ __forceinline__ void load_w(const float *__restrict__ th, Wregs &R, int r, int g) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * g + j;
+            R.B1[h][j] = (k < MLP_NIN) ? f2bf(th[MLP_OFF_W1 + k * 64 + 32 * h + r]) : (short)0;
+        }
+#pragma unroll
+    for (int kc = 0; kc < 8; kc++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 8 * kc + 4 * g + j;
+            R.B2[kc][0][j] = f2bf(th[MLP_OFF_W2 + k * 64 + r]);
+            R.B2[kc][1][j] = f2bf(th[MLP_OFF_W2 + k * 64 + 32 + r]);
+            R.B3[kc][j] = (r == 0) ? f2bf(th[MLP_OFF_W3 + k]) : (short)0;
+        }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        R.b1[h] = th[MLP_OFF_B1 + 32 * h + r];
+        R.b2[h] = th[MLP_OFF_B2 + 32 * h + r];
+        R.w3[h] = th[MLP_OFF_W3 + 32 * h + r];
+    }
+    R.b3 = th[MLP_OFF_B3];
+}
+
+// Forward pass of one 32-row tile. Returns the pre-activations z1, z2 (with
+// bias) in C layout and the output (valid in lanes r == 0). Leaves SiLU(z2)
+// in S. Block-uniform: contains __syncthreads.
+__device__ __forceinline__ void fwd_tile(const Wregs &R, const float *__restrict__ X, unsigned n, unsigned row0,
+                                         short *S, int r, int g, f32x16 &z1a, f32x16 &z1b, f32x16 &z2a, f32x16 &z2b,
+                                         f32x16 &yv) {
+    const unsigned row = row0 + (unsigned)r;
+    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < n) xv = *reinterpret_cast<const float4 *>(X + (size_t)row * MLP_XSTRIDE + 4 * g);
+    bf16x4 a;
+    a[0] = f2bf(xv.x);
+    a[1] = f2bf(xv.y);
+    a[2] = f2bf(xv.z);
+    a[3] = f2bf(xv.w);
+    z1a = mfma(a, R.B1[0], zero16());
+    z1b = mfma(a, R.B1[1], zero16());
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        z1a[e] += R.b1[0];
+        z1b[e] += R.b1[1];
+        S[crow(e, g) * SP + r] = f2bf(silu(z1a[e]));
+        S[crow(e, g) * SP + 32 + r] = f2bf(silu(z1b[e]));
+    }
+    __syncthreads();
+    z2a = zero16();
+    z2b = zero16();
+#pragma unroll
+    for (int kc = 0; kc < 8; kc++) {
+        const bf16x4 ak = *reinterpret_cast<const bf16x4 *>(S + r * SP + 8 * kc + 4 * g);
+        z2a = mfma(ak, R.B2[kc][0], z2a);
+        z2b = mfma(ak, R.B2[kc][1], z2b);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        z2a[e] += R.b2[0];
+        z2b[e] += R.b2[1];
+        S[crow(e, g) * SP + r] = f2bf(silu(z2a[e]));
+        S[crow(e, g) * SP + 32 + r] = f2bf(silu(z2b[e]));
+    }
+    __syncthreads();
+    yv = zero16();
+#pragma unroll
+    for (int kc = 0; kc < 8; kc++) {
+        const bf16x4 ak = *reinterpret_cast<const bf16x4 *>(S + r * SP + 8 * kc + 4 * g);
+        yv = mfma(ak, R.B3[kc], yv);
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) yv[e] += R.b3;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64 * MWAVES) void k_mlp_fwd(const float *__restrict__ th, const float *__restrict__ X,
+                                                        unsigned n, float *__restrict__ y) {
+    __shared__ short sm[MWAVES][32 * SP];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, g = lane >> 5;
+    Wregs R;
+    load_w(th, R, r, g);
+    const unsigned ntiles = (n + 31u) / 32u;
+    for (unsigned t0 = blockIdx.x * MWAVES; t0 < ntiles; t0 += gridDim.x * MWAVES) {
+        const unsigned row0 = (t0 + wid) * 32u;
+        f32x16 z1a, z1b, z2a, z2b, yv;
+        fwd_tile(R, X, n, row0, sm[wid], r, g, z1a, z1b, z2a, z2b, yv);
+        if (r == 0)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const unsigned row = row0 + crow(e, g);
+                if (row < n) y[row] = yv[e];
+            }
+    }
+}
+
+// Gradients of sum_i (y_i - t_i)^2 / n, one partial vector per wave.
+__global__ __launch_bounds__(64 * MWAVES) void k_mlp_grad(const float *__restrict__ th, const float *__restrict__ X,
+                                                         const float *__restrict__ T, unsigned n,
+                                                         float *__restrict__ partial, float *__restrict__ lossp) {
+    __shared__ short sm[MWAVES][32 * SP];
+    __shared__ short sd[MWAVES][32 * SP];
+    __shared__ short sx[MWAVES][32 * MLP_XSTRIDE];
+    __shared__ float sy[MWAVES][32];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, g = lane >> 5;
+    Wregs R;
+    load_w(th, R, r, g);
+    bf16x4 B2T[8][2];  // W2^T as B operand: B[k = n'][col = k'] = W2[k'][n']
+#pragma unroll
+    for (int kc = 0; kc < 8; kc++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) B2T[kc][h][j] = f2bf(th[MLP_OFF_W2 + (32 * h + r) * 64 + 8 * kc + 4 * g + j]);
+    f32x16 gW2[2][2], gW1[2];
+    gW2[0][0] = gW2[0][1] = gW2[1][0] = gW2[1][1] = gW1[0] = gW1[1] = zero16();
+    float gW3[2] = {0.f, 0.f}, gb1[2] = {0.f, 0.f}, gb2[2] = {0.f, 0.f}, gb3 = 0.f, L = 0.f;
+    const float invn = 1.0f / (float)n;
+    short *S = sm[wid], *D = sd[wid], *SX = sx[wid];
+    float *SY = sy[wid];
+    const unsigned ntiles = (n + 31u) / 32u;
+    for (unsigned t0 = blockIdx.x * MWAVES; t0 < ntiles; t0 += gridDim.x * MWAVES) {
+        const unsigned row0 = (t0 + wid) * 32u;
+        f32x16 z1a, z1b, z2a, z2b, yv;
+        fwd_tile(R, X, n, row0, S, r, g, z1a, z1b, z2a, z2b, yv);
+        {
+            const unsigned row = row0 + (unsigned)r;
+            float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (row < n) xv = *reinterpret_cast<const float4 *>(X + (size_t)row * MLP_XSTRIDE + 4 * g);
+            SX[r * MLP_XSTRIDE + 4 * g + 0] = f2bf(xv.x);
+            SX[r * MLP_XSTRIDE + 4 * g + 1] = f2bf(xv.y);
+            SX[r * MLP_XSTRIDE + 4 * g + 2] = f2bf(xv.z);
+            SX[r * MLP_XSTRIDE + 4 * g + 3] = f2bf(xv.w);
+        }
+        if (r == 0)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const unsigned row = row0 + crow(e, g);
+                float d = 0.f;
+                if (row < n) {
+                    const float diff = yv[e] - T[row];
+                    L += diff * diff;
+                    d = 2.0f * diff * invn;
+                }
+                SY[crow(e, g)] = d;
+                gb3 += d;
+            }
+        __syncthreads();
+        // dW3, db2 and dz2 (into D); h1 back into S
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const float d = SY[crow(e, g)];
+            const float dza = d * R.w3[0] * dsilu(z2a[e]);
+            const float dzb = d * R.w3[1] * dsilu(z2b[e]);
+            gW3[0] += silu(z2a[e]) * d;
+            gW3[1] += silu(z2b[e]) * d;
+            gb2[0] += dza;
+            gb2[1] += dzb;
+            D[crow(e, g) * SP + r] = f2bf(dza);
+            D[crow(e, g) * SP + 32 + r] = f2bf(dzb);
+            S[crow(e, g) * SP + r] = f2bf(silu(z1a[e]));
+            S[crow(e, g) * SP + 32 + r] = f2bf(silu(z1b[e]));
+        }
+        __syncthreads();
+        // dW2 += h1^T dz2 (M = 64 inputs, N = 64 outputs, K = 32 rows)
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            bf16x4 A0, A1, Bn0, Bn1;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = 8 * kc + 4 * g + j;
+                A0[j] = S[row * SP + r];
+                A1[j] = S[row * SP + 32 + r];
+                Bn0[j] = D[row * SP + r];
+                Bn1[j] = D[row * SP + 32 + r];
+            }
+            gW2[0][0] = mfma(A0, Bn0, gW2[0][0]);
+            gW2[0][1] = mfma(A0, Bn1, gW2[0][1]);
+            gW2[1][0] = mfma(A1, Bn0, gW2[1][0]);
+            gW2[1][1] = mfma(A1, Bn1, gW2[1][1]);
+        }
+        // dh1 = dz2 W2^T (M = 32 rows, N = 64, K = 64)
+        f32x16 dh0 = zero16(), dh1 = zero16();
+#pragma unroll
+        for (int kc = 0; kc < 8; kc++) {
+            const bf16x4 ak = *reinterpret_cast<const bf16x4 *>(D + r * SP + 8 * kc + 4 * g);
+            dh0 = mfma(ak, B2T[kc][0], dh0);
+            dh1 = mfma(ak, B2T[kc][1], dh1);
+        }
+        __syncthreads();
+        // dz1 (into D), db1
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const float da = dh0[e] * dsilu(z1a[e]);
+            const float db = dh1[e] * dsilu(z1b[e]);
+            gb1[0] += da;
+            gb1[1] += db;
+            D[crow(e, g) * SP + r] = f2bf(da);
+            D[crow(e, g) * SP + 32 + r] = f2bf(db);
+        }
+        __syncthreads();
+        // dW1 += x^T dz1 (M = features, padded to 32; N = 64; K = 32 rows)
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            bf16x4 A, Bn0, Bn1;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = 8 * kc + 4 * g + j;
+                A[j] = (r < MLP_NIN) ? SX[row * MLP_XSTRIDE + r] : (short)0;
+                Bn0[j] = D[row * SP + r];
+                Bn1[j] = D[row * SP + 32 + r];
+            }
+            gW1[0] = mfma(A, Bn0, gW1[0]);
+            gW1[1] = mfma(A, Bn1, gW1[1]);
+        }
+        __syncthreads();
+    }
+    float *P = partial + (size_t)(blockIdx.x * MWAVES + wid) * MLP_NPARAM;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int m = crow(e, g);
+            if (m < MLP_NIN) P[MLP_OFF_W1 + m * 64 + 32 * h + r] = gW1[h][e];
+        }
+#pragma unroll
+    for (int mh = 0; mh < 2; mh++)
+#pragma unroll
+        for (int nh = 0; nh < 2; nh++)
+#pragma unroll
+            for (int e = 0; e < 16; e++)
+                P[MLP_OFF_W2 + (32 * mh + crow(e, g)) * 64 + 32 * nh + r] = gW2[mh][nh][e];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const float w3 = gW3[h] + __shfl_xor(gW3[h], 32, 64);
+        const float b1 = gb1[h] + __shfl_xor(gb1[h], 32, 64);
+        const float b2 = gb2[h] + __shfl_xor(gb2[h], 32, 64);
+        if (g == 0) {
+            P[MLP_OFF_W3 + 32 * h + r] = w3;
+            P[MLP_OFF_B1 + 32 * h + r] = b1;
+            P[MLP_OFF_B2 + 32 * h + r] = b2;
+        }
+    }
+    const float b3 = gb3 + __shfl_xor(gb3, 32, 64);
+    const float l2 = L + __shfl_xor(L, 32, 64);
+    if (lane == 0) {
+        P[MLP_OFF_B3] = b3;
+        lossp[blockIdx.x * MWAVES + wid] = l2;
+    }
+}
+
+// grad[k] = sum over waves of partial[w][k]; loss = sum(lossp) / n.
+__global__ void k_mlp_grad_reduce(const float *__restrict__ partial, const float *__restrict__ lossp, unsigned waves,
+                                  unsigned n, float *__restrict__ grad, float *__restrict__ loss) {
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < (unsigned)MLP_NPARAM) {
+        float s = 0.f;
+        for (unsigned w = 0; w < waves; w++) s += partial[(size_t)w * MLP_NPARAM + k];
+        grad[k] = s;
+    }
+    if (k == 0) {
+        float s = 0.f;
+        for (unsigned w = 0; w < waves; w++) s += lossp[w];
+        *loss = s / (float)n;
+    }
+}
+
+__global__ void k_adam(float *__restrict__ th, const float *__restrict__ grad, float *__restrict__ m,
+                       float *__restrict__ v, float b1t, float b2t, float lr) {
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (unsigned)MLP_NPARAM) return;
+    const float g = grad[k];
+    const float mk = 0.9f * m[k] + 0.1f * g;
+    const float vk = 0.999f * v[k] + 0.001f * g * g;
+    m[k] = mk;
+    v[k] = vk;
+    th[k] -= lr * (mk / (1.0f - b1t)) / (sqrtf(vk / (1.0f - b2t)) + 1e-8f);
+}
+
+static unsigned mlp_blocks(unsigned n) {
+    const unsigned tiles = (n + 31u) / 32u;
+    return std::max(1u, std::min(1024u, (tiles + MWAVES - 1) / MWAVES));
+}
+
+void mlp_forward(hipStream_t s, const float *theta, const float *x, unsigned n, float *y) {
+    if (!n) return;
+    k_mlp_fwd<<<mlp_blocks(n), 64 * MWAVES, 0, s>>>(theta, x, n, y);
+    SPH_HIP(hipGetLastError());
+}
+
+void MLPTrainer::loss_grad(hipStream_t s, const float *theta, const float *x, const float *t, unsigned n) {
+    SPH_REQUIRE(n > 0, SPH_ERR_INVALID, "empty training batch");
+    const unsigned blocks = std::min(256u, mlp_blocks(n));
+    waves = blocks * MWAVES;
+    partial.reserve((size_t)waves * MLP_NPARAM);
+    loss_part.reserve(waves);
+    grad.reserve(MLP_NPARAM);
+    loss.reserve(1);
+    k_mlp_grad<<<blocks, 64 * MWAVES, 0, s>>>(theta, x, t, n, partial.get(), loss_part.get());
+    k_mlp_grad_reduce<<<(MLP_NPARAM + 255) / 256, 256, 0, s>>>(partial.get(), loss_part.get(), waves, n, grad.get(),
+                                                              loss.get());
+    SPH_HIP(hipGetLastError());
+}
+
+void MLPTrainer::adam(hipStream_t s, float *theta, int step, float lr) {
+    if (adam_m.n < (size_t)MLP_NPARAM) {
+        adam_m.reserve(MLP_NPARAM);
+        adam_v.reserve(MLP_NPARAM);
+        std::vector<float> z(MLP_NPARAM, 0.f);
+        SPH_HIP(hipMemcpyAsync(adam_m.get(), z.data(), MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, s));
+        SPH_HIP(hipMemcpyAsync(adam_v.get(), z.data(), MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, s));
+        SPH_HIP(hipStreamSynchronize(s));
+    }
+    const float b1t = (float)std::pow(0.9, step), b2t = (float)std::pow(0.999, step);
+    k_adam<<<(MLP_NPARAM + 255) / 256, 256, 0, s>>>(theta, grad.get(), adam_m.get(), adam_v.get(), b1t, b2t, lr);
+    SPH_HIP(hipGetLastError());
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+extern "C" {
+
+int sph_mlp_num_params(void) { return MLP_NPARAM; }
+
+int sph_mlp_forward(const float *theta, const float *x, size_t n, float *y) {
+    return guarded([&] {
+        SPH_REQUIRE(theta && x && y, SPH_ERR_INVALID, "null argument");
+        require_device();
+        if (!n) return;
+        DBuf<float> dt, dx, dy;
+        dt.reserve(MLP_NPARAM);
+        dx.reserve(n * MLP_XSTRIDE);
+        dy.reserve(n);
+        SPH_HIP(hipMemcpy(dt.get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(dx.get(), x, n * MLP_XSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+        mlp_forward(nullptr, dt.get(), dx.get(), (unsigned)n, dy.get());
+        SPH_HIP(hipMemcpy(y, dy.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int sph_mlp_loss_grad(const float *theta, const float *x, const float *t, size_t n, float *grad, float *loss) {
+    return guarded([&] {
+        SPH_REQUIRE(theta && x && t && grad && loss && n > 0, SPH_ERR_INVALID, "bad arguments");
+        require_device();
+        DBuf<float> dt, dx, dtt;
+        dt.reserve(MLP_NPARAM);
+        dx.reserve(n * MLP_XSTRIDE);
+        dtt.reserve(n);
+        SPH_HIP(hipMemcpy(dt.get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(dx.get(), x, n * MLP_XSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(dtt.get(), t, n * sizeof(float), hipMemcpyHostToDevice));
+        MLPTrainer tr;
+        tr.loss_grad(nullptr, dt.get(), dx.get(), dtt.get(), (unsigned)n);
+        SPH_HIP(hipMemcpy(grad, tr.grad.get(), MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+        SPH_HIP(hipMemcpy(loss, tr.loss.get(), sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int sph_mlp_train(float *theta, const float *x, const float *t, size_t n, int steps, int batch, double lr,
+                  unsigned seed, float *loss_out) {
+    return guarded([&] {
+        SPH_REQUIRE(theta && x && t && n > 0 && steps >= 0 && batch > 0, SPH_ERR_INVALID, "bad arguments");
+        require_device();
+        // shuffled copy of the samples; minibatches are consecutive windows
+        std::vector<unsigned> order(n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::mt19937 rng(seed);
+        std::shuffle(order.begin(), order.end(), rng);
+        std::vector<float> xs(n * MLP_XSTRIDE), ts(n);
+        for (size_t i = 0; i < n; i++) {
+            std::copy(x + (size_t)order[i] * MLP_XSTRIDE, x + (size_t)order[i] * MLP_XSTRIDE + MLP_XSTRIDE,
+                      xs.begin() + i * MLP_XSTRIDE);
+            ts[i] = t[order[i]];
+        }
+        DBuf<float> dt, dx, dtt;
+        dt.reserve(MLP_NPARAM);
+        dx.reserve(n * MLP_XSTRIDE);
+        dtt.reserve(n);
+        SPH_HIP(hipMemcpy(dt.get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(dx.get(), xs.data(), n * MLP_XSTRIDE * sizeof(float), hipMemcpyHostToDevice));
+        SPH_HIP(hipMemcpy(dtt.get(), ts.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        MLPTrainer tr;
+        const size_t b = std::min<size_t>((size_t)batch, n);
+        const size_t nbatches = std::max<size_t>(1, n / b);
+        for (int it = 0; it < steps; it++) {
+            const size_t o = (size_t)(it % nbatches) * b;
+            tr.loss_grad(nullptr, dt.get(), dx.get() + o * MLP_XSTRIDE, dtt.get() + o, (unsigned)b);
+            tr.adam(nullptr, dt.get(), it + 1, (float)lr);
+        }
+        tr.loss_grad(nullptr, dt.get(), dx.get(), dtt.get(), (unsigned)n);
+        SPH_HIP(hipMemcpy(theta, dt.get(), MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss_out) SPH_HIP(hipMemcpy(loss_out, tr.loss.get(), sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
