@@ -181,7 +181,10 @@ typedef struct sph_dfsph_counters {
     int    max_fluid_neighbors;
     int    max_boundary_neighbors;
     int    flags;            /* bit0: NaN/Inf seen in a reduction           */
-    int    reserved[7];
+    int    learned_particles;   /* particles that kept the learned start in
+                                   the last pressure solve                  */
+    int    learned_particles_v; /* same, last divergence solve              */
+    int    reserved[5];
 } sph_dfsph_counters_t;
 
 /* Sub-steps of TimeStepDFSPH::step() (TimeStepDFSPH.cpp:117-249), callable
@@ -217,15 +220,26 @@ int sph_dfsph_set_fluid(sph_dfsph_t s, const float *x, const float *v,
 /* BoundaryModel_Akinci2012::initModel (BoundaryModel_Akinci2012.cpp:77-110)
  * for a static body: n boundary particles, 3n floats. */
 int sph_dfsph_add_boundary_particles(sph_dfsph_t s, const float *x, size_t n);
-/* BoundaryModel_Bender2019 volume map of an axis-aligned box wall
- * (SimulatorBase::initVolumeMap, SimulatorBase.cpp:2566-2704): the signed
- * distance of the box surface (inverted when `invert`, i.e. fluid inside),
- * minus `thickness`, discretised on resolution[0..2] cubic cells over the
- * box grown by 8h + thickness, and the volume field integrated from it. */
+/* Adds a BoundaryModel_Bender2019 boundary object: the volume map of an
+ * axis-aligned box wall (SimulatorBase::initVolumeMap,
+ * SimulatorBase.cpp:2566-2704): the signed distance of the box surface
+ * (inverted when `invert`, i.e. fluid inside), minus `thickness`, discretised
+ * on resolution[0..2] cubic cells over the box grown by 8h + thickness, and
+ * the volume field integrated from it. Boundary objects are sampled and
+ * summed in the order they were added (TimeStep.cpp:177-222). */
 int sph_dfsph_set_volume_map_box(sph_dfsph_t s, const double box_min[3],
                                  const double box_max[3], int invert,
                                  double thickness,
                                  const unsigned resolution[3]);
+/* Adds a Bender2019 boundary object for a static sphere: the analytic
+ * signed distance |x - centre| - radius (negated when `invert`: fluid
+ * inside), minus `thickness`, on resolution[0..2] cells over the sphere's
+ * bounding box grown by 8h + thickness. The reference builds the same map
+ * from a sphere mesh's distance field (SimulatorBase.cpp:2566-2704). */
+int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3],
+                                    double radius, int invert,
+                                    double thickness,
+                                    const unsigned resolution[3]);
 /* Deferred initialisation (SimulatorBase::deferredInit,
  * StaticBoundarySimulator.cpp:181-199): Akinci boundary volumes
  * (Simulation::updateBoundaryVolume) and device buffers. Called by the first
@@ -282,6 +296,26 @@ int sph_mlp_loss_grad(const float *theta, const float *x, const float *t,
  * loss over all samples is returned in *loss. */
 int sph_mlp_train(float *theta, const float *x, const float *t, size_t n,
                   int steps, int batch, double lr, unsigned seed, float *loss);
+
+/* Learned warm start inside the time step (no counterpart in the reference;
+ * the hook replaces the warm starts of TimeStepDFSPH.cpp:292-296 and
+ * :449-453). solver: 0 pressure solve, 1 divergence solve. With a model set,
+ * the solve starts from the network's prediction wherever its initial
+ * residual is smaller than that of the plain previous-step warm start, and
+ * then iterates to the same tolerance. theta == NULL restores the plain
+ * start. */
+int sph_dfsph_set_warm_start_model(sph_dfsph_t s, int solver,
+                                   const float *theta);
+/* Training-sample collection: after each solve, every `stride`-th particle
+ * (rotating offset) contributes its 6 features (rows of 8 floats) and its
+ * converged stiffness relative to the step's scale; up to `capacity`
+ * samples are kept on the device. stride 0 turns collection off. */
+int sph_dfsph_collect_training(sph_dfsph_t s, int solver, unsigned stride,
+                               size_t capacity);
+/* Copies the collected samples (x: n rows of 8 floats, t: n targets) and
+ * clears the buffer; *n receives the number copied. */
+int sph_dfsph_get_training(sph_dfsph_t s, int solver, float *x, float *t,
+                           size_t capacity, size_t *n);
 
 /* Cubic-spline table kernel (PrecomputedKernel<CubicKernel,10000>,
  * SPHKernels.h:614-691) evaluated on the device for n FP32 offsets r (3n

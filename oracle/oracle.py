@@ -42,6 +42,7 @@ def lib():
             "orc_set_fluid": (C.c_int, [C.c_void_p, _dp, _dp, C.c_size_t]),
             "orc_add_boundary_particles": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
             "orc_set_volume_map_box": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _up]),
+            "orc_add_volume_map_sphere": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int, C.c_double, _up]),
             "orc_init": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
@@ -61,6 +62,9 @@ def lib():
             "orc_serendipity": (C.c_int, [_dp, _dp, _dp]),
             "orc_serendipity_node": (C.c_int, [C.c_int, _dp]),
             "orc_box_sdf": (C.c_int, [_dp, _dp, C.c_int, C.c_double, _dp, C.c_size_t, _dp]),
+            "orc_shape_sdf": (C.c_int, [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, C.c_size_t, _dp]),
+            "orc_shape_volume_map_eval": (C.c_int, [C.c_int, _dp, _dp, C.c_int, C.c_double, _up, C.c_double, C.c_int,
+                                                    C.c_int, _dp, C.c_size_t, _dp, _dp, _ip]),
             "orc_volume_map_eval": (C.c_int, [_dp, _dp, C.c_int, C.c_double, _up, C.c_double, C.c_int,
                                               C.c_int, _dp, C.c_size_t, _dp, _dp, _ip]),
             "orc_volume_map_domain": (C.c_int, [_dp, _dp, C.c_double, _up, C.c_double, _dp, _dp, _dp]),
@@ -169,6 +173,29 @@ def volume_map_eval(lo, hi, res, support_radius, x, field, interpolate, invert=T
     return val, grad, ok.astype(bool)
 
 
+def sphere_sdf(centre, radius, x, invert=False, thickness=0.0):
+    x = _d(x).reshape(-1, 3)
+    out = np.zeros(len(x))
+    _chk(lib().orc_shape_sdf(1, _ptr(_d(centre), _dp), _ptr(_d([radius, 0.0, 0.0]), _dp), int(invert), thickness,
+                             _ptr(x, _dp), len(x), _ptr(out, _dp)))
+    return out
+
+
+def sphere_volume_map_eval(centre, radius, res, support_radius, x, field, interpolate, invert=False,
+                           thickness=0.0):
+    """As volume_map_eval, for the map of a sphere."""
+    x = _d(x).reshape(-1, 3)
+    n = len(x)
+    val = np.zeros(n)
+    grad = np.zeros((n, 3))
+    ok = np.zeros(n, np.int32)
+    r = np.ascontiguousarray(res, dtype=np.uint32)
+    _chk(lib().orc_shape_volume_map_eval(1, _ptr(_d(centre), _dp), _ptr(_d([radius, 0.0, 0.0]), _dp), int(invert),
+                                         thickness, _ptr(r, _up), support_radius, field, int(interpolate),
+                                         _ptr(x, _dp), n, _ptr(val, _dp), _ptr(grad, _dp), _ptr(ok, _ip)))
+    return val, grad, ok.astype(bool)
+
+
 def volume_map_domain(lo, hi, res, support_radius, thickness=0.0):
     a, b, c = np.zeros(3), np.zeros(3), np.zeros(3)
     r = np.ascontiguousarray(res, dtype=np.uint32)
@@ -218,6 +245,16 @@ class OracleDFSPH:
         _chk(lib().orc_set_volume_map_box(self.h, _ptr(_d(lo), _dp), _ptr(_d(hi), _dp), int(invert),
                                           thickness, _ptr(r, _up)))
 
+    def add_volume_map_sphere(self, centre, radius, res, invert=False, thickness=0.0):
+        """Static sphere obstacle; `res` is cells per axis or, as a float, the
+        cell size in metres (as in the device library's binding)."""
+        if np.ndim(res) == 0:
+            ext = 2.0 * (radius + 8.0 * 4.0 * self.params.particle_radius + thickness)
+            res = [max(1, int(round(ext / float(res))))] * 3
+        r = np.ascontiguousarray(res, dtype=np.uint32)
+        _chk(lib().orc_add_volume_map_sphere(self.h, _ptr(_d(centre), _dp), float(radius), int(invert), thickness,
+                                             _ptr(r, _up)))
+
     def init(self):
         _chk(lib().orc_init(self.h))
 
@@ -230,7 +267,7 @@ class OracleDFSPH:
     def get(self, name, comps=None):
         if comps is None:
             from splishsplash_amd.params import FIELD_COMPONENTS
-            comps = FIELD_COMPONENTS.get(name, 1)
+            comps = 3 if name.startswith("boundary x") else FIELD_COMPONENTS.get(name, 1)
         count = self.n if name != "boundary particle volume" else None
         if count is None:
             raise ValueError("use get_boundary_volumes")

@@ -232,19 +232,32 @@ void gauss_legendre(int n, double *x, double *w) {
     }
 }
 
-// ------------------------------------------------------------ box SDF
-double BoxSDF::eval(const double *p) const {
-    bool in = true;
-    double dmin = std::numeric_limits<double>::max(), q2 = 0.0;
-    for (int c = 0; c < 3; c++) {
-        if (p[c] < lo[c] || p[c] > hi[c]) in = false;
-        dmin = std::min(dmin, std::min(p[c] - lo[c], hi[c] - p[c]));
-        const double q = std::max(std::max(lo[c] - p[c], 0.0), p[c] - hi[c]);
-        q2 += q * q;
+// ---------------------------------------------------------- shape SDF
+double ShapeSDF::eval(const double *p) const {
+    double dmesh;
+    if (kind == 1) {
+        const double d[3] = {p[0] - centre[0], p[1] - centre[1], p[2] - centre[2]};
+        dmesh = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - radius;
+    } else {
+        bool in = true;
+        double dmin = std::numeric_limits<double>::max(), q2 = 0.0;
+        for (int c = 0; c < 3; c++) {
+            if (p[c] < lo[c] || p[c] > hi[c]) in = false;
+            dmin = std::min(dmin, std::min(p[c] - lo[c], hi[c] - p[c]));
+            const double q = std::max(std::max(lo[c] - p[c], 0.0), p[c] - hi[c]);
+            q2 += q * q;
+        }
+        dmesh = in ? -dmin : std::sqrt(q2);
     }
-    const double dmesh = in ? -dmin : std::sqrt(q2);
     const double sign = invert ? -1.0 : 1.0;
     return sign * (dmesh - thickness);
+}
+
+void ShapeSDF::aabb(double *blo, double *bhi) const {
+    for (int c = 0; c < 3; c++) {
+        blo[c] = kind == 1 ? centre[c] - radius : lo[c];
+        bhi[c] = kind == 1 ? centre[c] + radius : hi[c];
+    }
 }
 
 // -------------------------------------------------- serendipity element
@@ -298,17 +311,19 @@ void serendipity(const double *xi, double *N, double *dN) {
 }
 
 // ---------------------------------------------------------- volume map
-// Domain = box grown by 8h + thickness (SimulatorBase.cpp:2596-2600); cell
-// size = domain diagonal / resolution.
-void VolumeMap::init(const BoxSDF &s, double h, const unsigned *r) {
+// Domain = the shape's bounding box grown by 8h + thickness
+// (SimulatorBase.cpp:2596-2600); cell size = domain extent / resolution.
+void VolumeMap::init(const ShapeSDF &s, double h, const unsigned *r) {
     sdf = s;
     supportRadius = h;
     cubic.set(h);
     gauss_legendre(16, gx, gw);  // p = 30 -> 16 points (GaussQuadrature.cpp:102-103)
     const double pad = 8.0 * h + s.thickness;
+    double blo[3], bhi[3];
+    s.aabb(blo, bhi);
     for (int c = 0; c < 3; c++) {
-        lo[c] = s.lo[c] - pad;
-        hi[c] = s.hi[c] + pad;
+        lo[c] = blo[c] - pad;
+        hi[c] = bhi[c] + pad;
         res[c] = r[c];
         cs[c] = (hi[c] - lo[c]) / (double)res[c];
         c0[c] = 2.0 / cs[c];
@@ -520,8 +535,8 @@ void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
     pv.assign(n, 0.0);
     ap.assign(3 * n, 0.0);
     vdiff.assign(3 * n, 0.0);
-    vmV.assign(n, 0.0);
-    vmX.assign(3 * n, 0.0);
+    vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
+    vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
     initialized = false;
 }
 
@@ -532,9 +547,11 @@ void Sim::add_boundary(const double *xx, size_t nn) {
     initialized = false;
 }
 
-void Sim::set_volume_map(const BoxSDF &s, const unsigned *res) {
-    vm.reset(new VolumeMap());
-    vm->init(s, hs, res);
+void Sim::add_volume_map(const ShapeSDF &s, const unsigned *res) {
+    vms.emplace_back(new VolumeMap());
+    vms.back()->init(s, hs, res);
+    vmV.assign(vms.size() * n, 0.0);
+    vmX.assign(3 * vms.size() * n, 0.0);
 }
 
 // Simulation::updateBoundaryVolume + BoundaryModel_Akinci2012::computeBoundaryVolume
@@ -571,9 +588,18 @@ void Sim::neighborhood_search() {
     }
 }
 
-// TimeStep::computeVolumeAndBoundaryX (TimeStep.cpp:177-346), static box.
+// TimeStep::computeVolumeAndBoundaryX (TimeStep.cpp:177-346), static shapes,
+// one boundary object after the other: a particle moved out of one object is
+// sampled against the next at its new position.
 void Sim::volume_maps() {
     if (!has_volume_map()) return;
+    for (size_t o = 0; o < vms.size(); o++) volume_map(o);
+}
+
+void Sim::volume_map(size_t o) {
+    VolumeMap *vm = vms[o].get();
+    double *vmVo = &vmV[o * n];
+    double *vmXo = &vmX[3 * o * n];
     std::vector<uint64_t> cells;
     cells.reserve(n);
     for (size_t i = 0; i < n; i++) {
@@ -586,9 +612,9 @@ void Sim::volume_maps() {
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
         double *xi = &x[3 * i];
-        double *xb = &vmX[3 * i];
+        double *xb = &vmXo[3 * i];
         xb[0] = xb[1] = xb[2] = 0.0;
-        vmV[i] = 0.0;
+        vmVo[i] = 0.0;
         double normal[3] = {0, 0, 0};
         double dist = std::numeric_limits<double>::max();
         double val;
@@ -599,23 +625,23 @@ void Sim::volume_maps() {
             double volume;
             vm->interpolate(1, xi, &volume, nullptr);
             if ((volume > 0.0) && (volume != std::numeric_limits<double>::max())) {
-                vmV[i] = volume;
+                vmVo[i] = volume;
                 const double nl = std::sqrt(sqn3(normal));
                 if (nl > 1.0e-9) {
                     for (int c = 0; c < 3; c++) normal[c] /= nl;
                     const double d = std::max(dist + 0.5 * particleRadius, 2.0 * particleRadius);
                     for (int c = 0; c < 3; c++) xb[c] = xi[c] - d * normal[c];
                 } else {
-                    vmV[i] = 0.0;
+                    vmVo[i] = 0.0;
                 }
             } else {
-                vmV[i] = 0.0;
+                vmVo[i] = 0.0;
             }
         } else if (dist <= 0.0) {
             animate = true;
-            vmV[i] = 0.0;
+            vmVo[i] = 0.0;
         } else {
-            vmV[i] = 0.0;
+            vmVo[i] = 0.0;
         }
         if (animate) {
             if (dist != std::numeric_limits<double>::max()) {
@@ -628,7 +654,7 @@ void Sim::volume_maps() {
                     v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0;
                 }
             }
-            vmV[i] = 0.0;
+            vmVo[i] = 0.0;
         }
     }
 }
@@ -653,11 +679,13 @@ void Sim::densities() {
                 density += bV[j] * K.W(d);
             }
         } else if (has_volume_map()) {
-            const double Vj = vmV[i];
-            if (Vj > 0.0) {
-                const double *xj = &vmX[3 * i];
-                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                density += Vj * K.W(d);
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (Vj > 0.0) {
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    density += Vj * K.W(d);
+                }
             }
         }
         rho[i] = density * density0;
@@ -691,13 +719,15 @@ void Sim::dfsph_factor() {
                 for (int c = 0; c < 3; c++) gpi[c] -= -bV[j] * g[c];
             }
         } else if (has_volume_map()) {
-            const double Vj = vmV[i];
-            if (Vj > 0.0) {
-                const double *xj = &vmX[3 * i];
-                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                double g[3];
-                K.gradW(d, g);
-                for (int c = 0; c < 3; c++) gpi[c] -= -Vj * g[c];
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (Vj > 0.0) {
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double g[3];
+                    K.gradW(d, g);
+                    for (int c = 0; c < 3; c++) gpi[c] -= -Vj * g[c];
+                }
             }
         }
         sum_grad_p_k += sqn3(gpi);
@@ -731,13 +761,15 @@ static inline double div_sum(const Sim &S, size_t i) {
             delta += S.bV[j] * dot3(vi, g);
         }
     } else if (S.has_volume_map()) {
-        const double Vj = S.vmV[i];
-        if (Vj > 0.0) {
-            const double *xj = &S.vmX[3 * i];
-            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-            double g[3];
-            S.K.gradW(d, g);
-            delta += Vj * dot3(vi, g);
+        for (size_t o = 0; o < S.vms.size(); o++) {
+            const double Vj = S.vmV[o * S.n + i];
+            if (Vj > 0.0) {
+                const double *xj = &S.vmX[3 * (o * S.n + i)];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                S.K.gradW(d, g);
+                delta += Vj * dot3(vi, g);
+            }
         }
     }
     return delta;
@@ -791,13 +823,15 @@ void Sim::pressure_accel(const std::vector<double> &prho2, size_t i, double *ai)
                 for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-bV[j] * g[c]);
             }
         } else if (has_volume_map()) {
-            const double Vj = vmV[i];
-            if (Vj > 0.0) {
-                const double *xj = &vmX[3 * i];
-                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                double g[3];
-                K.gradW(d, g);
-                for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-Vj * g[c]);
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (Vj > 0.0) {
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double g[3];
+                    K.gradW(d, g);
+                    for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-Vj * g[c]);
+                }
             }
         }
     }
@@ -827,13 +861,15 @@ double Sim::aij_pj(size_t i) const {
             s += bV[j] * dot3(ai, g);
         }
     } else if (has_volume_map()) {
-        const double Vj = vmV[i];
-        if (Vj > 0.0) {
-            const double *xj = &vmX[3 * i];
-            const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-            double g[3];
-            K.gradW(d, g);
-            s += Vj * dot3(ai, g);
+        for (size_t o = 0; o < vms.size(); o++) {
+            const double Vj = vmV[o * n + i];
+            if (Vj > 0.0) {
+                const double *xj = &vmX[3 * (o * n + i)];
+                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(d, g);
+                s += Vj * dot3(ai, g);
+            }
         }
     }
     return s;
@@ -953,30 +989,33 @@ void Sim::weiler_matvec(const std::vector<double> &vec, std::vector<double> &res
                     const double s = d * mub * (density0 * bV[j] / density_i) * dot3(vi, xixj) / (sqn3(xixj) + 0.01 * h2);
                     for (int c = 0; c < 3; c++) ai[c] += s * g[c];
                 }
-            } else if (has_volume_map() && vmV[i] > 0.0) {
-                const double Vj = vmV[i];
-                const double *xj = &vmX[3 * i];
-                const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
-                const double nl = std::sqrt(sqn3(normal));
-                if (nl > 0.0001) {
-                    for (int c = 0; c < 3; c++) normal[c] /= nl;
-                    double t1[3], t2[3];
-                    orthogonal_vectors(normal, t1, t2);
-                    const double dist = 0.5 * h;  // m_tangentialDistanceFactor * h
-                    const double vol = 0.25 * Vj;
-                    for (int sidx = 0; sidx < 4; sidx++) {
-                        const double *t = (sidx < 2) ? t1 : t2;
-                        const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
-                        double xs[3], xix[3];
-                        for (int c = 0; c < 3; c++) {
-                            xs[c] = xj[c] + sg * t[c] * dist;
-                            xix[c] = xi[c] - xs[c];
+            } else if (has_volume_map()) {
+                for (size_t o = 0; o < vms.size(); o++) {
+                    const double Vj = vmV[o * n + i];
+                    if (!(Vj > 0.0)) continue;
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
+                    const double nl = std::sqrt(sqn3(normal));
+                    if (nl > 0.0001) {
+                        for (int c = 0; c < 3; c++) normal[c] /= nl;
+                        double t1[3], t2[3];
+                        orthogonal_vectors(normal, t1, t2);
+                        const double dist = 0.5 * h;  // m_tangentialDistanceFactor * h
+                        const double vol = 0.25 * Vj;
+                        for (int sidx = 0; sidx < 4; sidx++) {
+                            const double *t = (sidx < 2) ? t1 : t2;
+                            const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
+                            double xs[3], xix[3];
+                            for (int c = 0; c < 3; c++) {
+                                xs[c] = xj[c] + sg * t[c] * dist;
+                                xix[c] = xi[c] - xs[c];
+                            }
+                            double g[3];
+                            K.gradW(xix, g);
+                            const double s = d * mub * vol * dot3(vi, xix) / (sqn3(xix) + 0.01 * h2);
+                            for (int c = 0; c < 3; c++) ai[c] += s * g[c];
                         }
-                        double g[3];
-                        K.gradW(xix, g);
-                        const double s = d * mub * vol * dot3(vi, xix) / (sqn3(xix) + 0.01 * h2);
-                        for (int c = 0; c < 3; c++) ai[c] += s * g[c];
                     }
                 }
             }
@@ -1016,31 +1055,34 @@ void Sim::weiler_diag(size_t i, double *m) const {
                 for (int r0 = 0; r0 < 3; r0++)
                     for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xixj[c]);
             }
-        } else if (has_volume_map() && vmV[i] > 0.0) {
-            const double Vj = vmV[i];
-            const double *xj = &vmX[3 * i];
-            const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-            double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
-            const double nl = std::sqrt(sqn3(normal));
-            if (nl > 0.0001) {
-                for (int c = 0; c < 3; c++) normal[c] /= nl;
-                double t1[3], t2[3];
-                orthogonal_vectors(normal, t1, t2);
-                const double dist = 0.5 * h;
-                const double vol = 0.25 * Vj;
-                for (int sidx = 0; sidx < 4; sidx++) {
-                    const double *t = (sidx < 2) ? t1 : t2;
-                    const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
-                    double xs[3], xix[3];
-                    for (int c = 0; c < 3; c++) {
-                        xs[c] = xj[c] + sg * t[c] * dist;
-                        xix[c] = xi[c] - xs[c];
+        } else if (has_volume_map()) {
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (!(Vj > 0.0)) continue;
+                const double *xj = &vmX[3 * (o * n + i)];
+                const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
+                const double nl = std::sqrt(sqn3(normal));
+                if (nl > 0.0001) {
+                    for (int c = 0; c < 3; c++) normal[c] /= nl;
+                    double t1[3], t2[3];
+                    orthogonal_vectors(normal, t1, t2);
+                    const double dist = 0.5 * h;
+                    const double vol = 0.25 * Vj;
+                    for (int sidx = 0; sidx < 4; sidx++) {
+                        const double *t = (sidx < 2) ? t1 : t2;
+                        const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
+                        double xs[3], xix[3];
+                        for (int c = 0; c < 3; c++) {
+                            xs[c] = xj[c] + sg * t[c] * dist;
+                            xix[c] = xi[c] - xs[c];
+                        }
+                        double g[3];
+                        K.gradW(xix, g);
+                        const double s = d * mub * vol / (sqn3(xix) + 0.01 * h2);
+                        for (int r0 = 0; r0 < 3; r0++)
+                            for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xix[c]);
                     }
-                    double g[3];
-                    K.gradW(xix, g);
-                    const double s = d * mub * vol / (sqn3(xix) + 0.01 * h2);
-                    for (int r0 = 0; r0 < 3; r0++)
-                        for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xix[c]);
                 }
             }
         }
@@ -1115,13 +1157,16 @@ void Sim::xsph() {
                     const double s = -invH * cb * (density0 * bV[j] / density_i);
                     for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
                 }
-            } else if (has_volume_map() && vmV[i] > 0.0) {
-                const double Vj = vmV[i];
-                const double *xj = &vmX[3 * i];
-                const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                const double w = K.W(d);
-                const double s = -invH * cb * (density0 * Vj / density_i);
-                for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
+            } else if (has_volume_map()) {
+                for (size_t o = 0; o < vms.size(); o++) {
+                    const double Vj = vmV[o * n + i];
+                    if (!(Vj > 0.0)) continue;
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    const double w = K.W(d);
+                    const double s = -invH * cb * (density0 * Vj / density_i);
+                    for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
+                }
             }
         }
     }
@@ -1316,11 +1361,27 @@ bool field_of(Sim *s, const char *name, FieldRef &f) {
     else if (n == "p_v / rho^2") f = {&s->pv, 1};
     else if (n == "pressure acceleration") f = {&s->ap, 3};
     else if (n == "velocity difference") f = {&s->vdiff, 3};
-    else if (n == "boundary volume") f = {&s->vmV, 1};
-    else if (n == "boundary x") f = {&s->vmX, 3};
     else if (n == "boundary particle volume") f = {&s->bV, 1};
     else return false;
     return true;
+}
+// "boundary volume" / "boundary x" of volume-map object 0, or with a
+// trailing " <o>" of object o: sets `out` to the slice (null if the name is
+// another field).
+int vm_field(Sim *s, const char *name, size_t count, double *&out) {
+    const std::string n(name);
+    out = nullptr;
+    const bool isx = n.rfind("boundary x", 0) == 0;
+    if (!isx && n.rfind("boundary volume", 0) != 0) return SPH_OK;
+    const std::string rest = n.substr(isx ? 10 : 15);
+    if (!rest.empty() && (rest[0] != ' ' || rest.size() < 2 || rest.find_first_not_of("0123456789", 1) != std::string::npos))
+        return SPH_OK;  // not a volume-map field
+    const size_t o = rest.empty() ? 0 : (size_t)std::atoi(rest.c_str() + 1);
+    if (o >= std::max<size_t>(s->vms.size(), 1)) return fail("no such volume map");
+    const size_t k = isx ? 3 : 1;
+    if (count != k * s->n) return fail("size mismatch");
+    out = (isx ? s->vmX.data() : s->vmV.data()) + k * o * s->n;
+    return SPH_OK;
 }
 }  // namespace
 
@@ -1347,14 +1408,25 @@ int orc_add_boundary_particles(void *h, const double *x, size_t n) {
 }
 int orc_set_volume_map_box(void *h, const double *lo, const double *hi, int invert, double thickness,
                            const unsigned *res) {
-    orc::BoxSDF s;
+    orc::ShapeSDF s;
     for (int c = 0; c < 3; c++) {
         s.lo[c] = lo[c];
         s.hi[c] = hi[c];
     }
     s.invert = invert != 0;
     s.thickness = thickness;
-    ((Sim *)h)->set_volume_map(s, res);
+    ((Sim *)h)->add_volume_map(s, res);
+    return SPH_OK;
+}
+int orc_add_volume_map_sphere(void *h, const double *centre, double radius, int invert, double thickness,
+                              const unsigned *res) {
+    orc::ShapeSDF s;
+    s.kind = 1;
+    for (int c = 0; c < 3; c++) s.centre[c] = centre[c];
+    s.radius = radius;
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    ((Sim *)h)->add_volume_map(s, res);
     return SPH_OK;
 }
 int orc_init(void *h) {
@@ -1379,6 +1451,12 @@ int orc_get_field(void *h, const char *name, double *out, size_t count) {
         for (size_t i = 0; i < s->n; i++) out[i] = c.off.empty() ? 0.0 : (double)c.count(i);
         return SPH_OK;
     }
+    double *vm = nullptr;
+    if (const int rc = vm_field(s, name, count, vm)) return rc;
+    if (vm) {
+        std::copy(vm, vm + count, out);
+        return SPH_OK;
+    }
     FieldRef f;
     if (!field_of(s, name, f)) return fail("unknown field");
     if (count != f.vec->size()) return fail("size mismatch");
@@ -1387,6 +1465,12 @@ int orc_get_field(void *h, const char *name, double *out, size_t count) {
 }
 int orc_set_field(void *h, const char *name, const double *in, size_t count) {
     Sim *s = (Sim *)h;
+    double *vm = nullptr;
+    if (const int rc = vm_field(s, name, count, vm)) return rc;
+    if (vm) {
+        std::copy(in, in + count, vm);
+        return SPH_OK;
+    }
     FieldRef f;
     if (!field_of(s, name, f)) return fail("unknown field");
     if (count != f.vec->size()) return fail("size mismatch");
@@ -1487,9 +1571,33 @@ int orc_serendipity_node(int k, double *xi) {
     orc::serendipity_node(k, xi);
     return SPH_OK;
 }
+static orc::ShapeSDF make_shape(int kind, const double *a, const double *b, int invert, double thickness) {
+    orc::ShapeSDF s;
+    s.kind = kind;
+    for (int c = 0; c < 3; c++) {
+        if (kind == 1) {
+            s.centre[c] = a[c];
+        } else {
+            s.lo[c] = a[c];
+            s.hi[c] = b[c];
+        }
+    }
+    if (kind == 1) s.radius = b[0];
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    return s;
+}
+// Signed distance of a shape at n points: kind 0 box (a = lo, b = hi),
+// kind 1 sphere (a = centre, b[0] = radius).
+int orc_shape_sdf(int kind, const double *a, const double *b, int invert, double thickness, const double *x,
+                  size_t n, double *out) {
+    const orc::ShapeSDF s = make_shape(kind, a, b, invert, thickness);
+    for (size_t i = 0; i < n; i++) out[i] = s.eval(x + 3 * i);
+    return SPH_OK;
+}
 int orc_box_sdf(const double *lo, const double *hi, int invert, double thickness, const double *x, size_t n,
                 double *out) {
-    orc::BoxSDF s;
+    orc::ShapeSDF s;
     for (int c = 0; c < 3; c++) {
         s.lo[c] = lo[c];
         s.hi[c] = hi[c];
@@ -1501,16 +1609,19 @@ int orc_box_sdf(const double *lo, const double *hi, int invert, double thickness
 }
 // Node values of the volume map of a box (field 0 sdf, field 1 volume) at n
 // points, and interpolated values + gradients at m points.
+int orc_shape_volume_map_eval(int kind, const double *a, const double *b, int invert, double thickness,
+                              const unsigned *res, double support_radius, int field, int interpolate,
+                              const double *x, size_t n, double *val, double *grad, int *ok);
 int orc_volume_map_eval(const double *lo, const double *hi, int invert, double thickness, const unsigned *res,
                         double support_radius, int field, int interpolate, const double *x, size_t n,
                         double *val, double *grad, int *ok) {
-    orc::BoxSDF s;
-    for (int c = 0; c < 3; c++) {
-        s.lo[c] = lo[c];
-        s.hi[c] = hi[c];
-    }
-    s.invert = invert != 0;
-    s.thickness = thickness;
+    return orc_shape_volume_map_eval(0, lo, hi, invert, thickness, res, support_radius, field, interpolate, x, n,
+                                     val, grad, ok);
+}
+int orc_shape_volume_map_eval(int kind, const double *a, const double *b, int invert, double thickness,
+                              const unsigned *res, double support_radius, int field, int interpolate,
+                              const double *x, size_t n, double *val, double *grad, int *ok) {
+    const orc::ShapeSDF s = make_shape(kind, a, b, invert, thickness);
     orc::VolumeMap vm;
     vm.init(s, support_radius, res);
     if (interpolate) {
@@ -1535,7 +1646,7 @@ int orc_volume_map_eval(const double *lo, const double *hi, int invert, double t
 // Volume map domain: lo[3], hi[3], cell size[3].
 int orc_volume_map_domain(const double *blo, const double *bhi, double thickness, const unsigned *res,
                           double support_radius, double *lo, double *hi, double *cs) {
-    orc::BoxSDF s;
+    orc::ShapeSDF s;
     for (int c = 0; c < 3; c++) {
         s.lo[c] = blo[c];
         s.hi[c] = bhi[c];

@@ -61,14 +61,18 @@ void ns_brute(const R *xq, size_t nq, const R *xs, size_t ns, R radius, bool sam
 // Legendre recurrence.
 void gauss_legendre(int n, double *x, double *w);
 
-// Signed distance of an axis-aligned box surface as used for a wall
-// (SimulatorBase.cpp:2609-2611): sign * (d_mesh(x) - thickness), sign = -1
-// when inverted. d_mesh is negative inside the box.
-struct BoxSDF {
-    double lo[3], hi[3];
+// Signed distance of an analytic boundary shape as used for a volume map
+// (SimulatorBase.cpp:2609-2611): sign * (d_shape(x) - thickness), sign = -1
+// when inverted, d_shape < 0 inside the shape. Kinds: 0 axis-aligned box
+// [lo, hi] (a wall when inverted), 1 sphere (centre, radius).
+struct ShapeSDF {
+    int kind = 0;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double centre[3] = {0, 0, 0}, radius = 0;
     bool invert = true;
     double thickness = 0;
     double eval(const double *x) const;
+    void aabb(double *blo, double *bhi) const;
 };
 
 // 32-node cubic serendipity cell (the element of Discregrid's
@@ -77,17 +81,17 @@ struct BoxSDF {
 void serendipity_node(int n, double *xi);
 void serendipity(const double *xi, double *N, double *dN /* 32x3 or null */);
 
-// Volume map of a box wall, Bender2019 (SimulatorBase.cpp:2566-2704,
+// Volume map of one static shape, Bender2019 (SimulatorBase.cpp:2566-2704,
 // TimeStep.cpp:223-346). Cell values are evaluated on demand and cached.
 struct VolumeMap {
     double lo[3], hi[3], cs[3], c0[3];
     unsigned res[3];
-    BoxSDF sdf;
+    ShapeSDF sdf;
     double supportRadius = 0;
     Cubic cubic;
     double gx[16], gw[16];
     std::unordered_map<uint64_t, std::vector<double>> cache;  // 64 values per cell
-    void init(const BoxSDF &s, double supportRadius, const unsigned *res);
+    void init(const ShapeSDF &s, double supportRadius, const unsigned *res);
     bool locate(const double *x, uint64_t &cell, double *xi) const;
     double node_sdf(const double *x) const;
     double node_volume(const double *x) const;
@@ -115,8 +119,8 @@ public:
     size_t n = 0, nb = 0;
     std::vector<double> x, v, a, rho, factor, rho_adv, p, pv, ap, vdiff;
     std::vector<double> bx, bV;         // Akinci particles and volumes
-    std::unique_ptr<VolumeMap> vm;
-    std::vector<double> vmV, vmX;       // per fluid particle boundary volume / x_b
+    std::vector<std::unique_ptr<VolumeMap>> vms;  // boundary objects in insertion order
+    std::vector<double> vmV, vmX;       // per object and fluid particle: V_b [o * n + i], x_b
     CSR nf, nbd;                        // fluid->fluid, fluid->boundary
     double time = 0, dt = 0, h_step = 0;
     int iterations = 0, iterationsV = 0, viscoIterations = 0, steps = 0;
@@ -126,13 +130,14 @@ public:
 
     void set_fluid(const double *x, const double *v, size_t n);
     void add_boundary(const double *x, size_t n);
-    void set_volume_map(const BoxSDF &s, const unsigned *res);
+    void add_volume_map(const ShapeSDF &s, const unsigned *res);
     void init();
     void step();
     int stage(int s);
 
     void neighborhood_search();
     void volume_maps();
+    void volume_map(size_t o);
     void densities();
     void dfsph_factor();
     void divergence_setup();
@@ -156,7 +161,7 @@ public:
     void weiler_matvec(const std::vector<double> &vec, std::vector<double> &res) const;
     void weiler_diag(size_t i, double *m) const;
     bool has_boundary_particles() const { return P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb > 0; }
-    bool has_volume_map() const { return P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 && vm; }
+    bool has_volume_map() const { return P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 && !vms.empty(); }
 };
 
 }  // namespace orc

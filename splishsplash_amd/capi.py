@@ -68,6 +68,7 @@ _SIGS = {
     "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_boundary_particles": (C.c_int, [_vp, _fp, C.c_size_t]),
     "sph_dfsph_set_volume_map_box": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _up]),
+    "sph_dfsph_add_volume_map_sphere": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_double, _up]),
     "sph_dfsph_init": (C.c_int, [_vp]),
     "sph_dfsph_step": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_stage": (C.c_int, [_vp, C.c_int]),
@@ -76,6 +77,9 @@ _SIGS = {
     "sph_dfsph_set_field": (C.c_int, [_vp, C.c_char_p, _vp, C.c_size_t]),
     "sph_dfsph_get_neighbors": (C.c_int, [_vp, C.c_int, _up, _up, C.c_size_t, _szp]),
     "sph_dfsph_get_counters": (C.c_int, [_vp, C.POINTER(DFSPHCounters)]),
+    "sph_dfsph_set_warm_start_model": (C.c_int, [_vp, C.c_int, _fp]),
+    "sph_dfsph_collect_training": (C.c_int, [_vp, C.c_int, C.c_uint, C.c_size_t]),
+    "sph_dfsph_get_training": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_size_t, _szp]),
     "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_kernel_stats": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_longlong),
@@ -340,6 +344,17 @@ class TimeStepDFSPH:
         _chk(lib().sph_dfsph_set_volume_map_box(self.h, lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp),
                                                 int(invert), thickness, r.ctypes.data_as(_up)))
 
+    def add_volume_map_sphere(self, centre, radius, res, invert=False, thickness=0.0):
+        """Static sphere obstacle as a Bender2019 volume map; `res` is the
+        cells per axis or, as a float, the cell size in metres."""
+        c = np.ascontiguousarray(centre, np.float64)
+        if np.ndim(res) == 0:
+            ext = 2.0 * (radius + 8.0 * 4.0 * self.params.particle_radius + thickness)
+            res = [max(1, int(round(ext / float(res))))] * 3
+        r = np.ascontiguousarray(res, np.uint32)
+        _chk(lib().sph_dfsph_add_volume_map_sphere(self.h, c.ctypes.data_as(_dp), float(radius), int(invert),
+                                                   thickness, r.ctypes.data_as(_up)))
+
     def init(self):
         _chk(lib().sph_dfsph_init(self.h))
 
@@ -353,7 +368,10 @@ class TimeStepDFSPH:
         if name == "boundary particle volume":
             out = np.zeros(self.nb, np.float32)
         else:
-            comps = FIELD_COMPONENTS[name]
+            if name.startswith("boundary x ") or name.startswith("boundary volume "):
+                comps = 3 if name.startswith("boundary x") else 1  # per volume-map object
+            else:
+                comps = FIELD_COMPONENTS[name]
             dt = np.uint32 if name in ("id", "fluid neighbor count", "boundary neighbor count") else np.float32
             out = np.zeros((self.n, comps) if comps > 1 else self.n, dt)
         _chk(lib().sph_dfsph_get_field(self.h, name.encode(), out.ctypes.data, out.nbytes))
@@ -379,6 +397,42 @@ class TimeStepDFSPH:
 
     def set_time(self, t, dt):
         _chk(lib().sph_dfsph_set_time(self.h, t, dt))
+
+    _SOLVERS = {"pressure": 0, "divergence": 1}
+
+    def set_warm_start_model(self, solver, theta):
+        """Start `solver` ("pressure" or "divergence") from the network with
+        parameters `theta` (per particle, where its initial residual beats
+        the reference's warm start); None restores the reference's start."""
+        s = self._SOLVERS[solver]
+        if theta is None:
+            _chk(lib().sph_dfsph_set_warm_start_model(self.h, s, None))
+            return
+        th = np.ascontiguousarray(theta, np.float32)
+        if th.size != MLP_NPARAM:
+            raise ValueError(f"theta must hold {MLP_NPARAM} floats")
+        self._theta = getattr(self, "_theta", {})
+        self._theta[solver] = th
+        _chk(lib().sph_dfsph_set_warm_start_model(self.h, s, th.ctypes.data_as(_fp)))
+
+    def collect_training(self, solver, stride, capacity):
+        """Records (features, converged stiffness / scale) of every
+        `stride`-th particle after each solve, up to `capacity` samples;
+        stride 0 stops recording."""
+        self._cap = getattr(self, "_cap", {})
+        self._cap[solver] = int(capacity)
+        _chk(lib().sph_dfsph_collect_training(self.h, self._SOLVERS[solver], int(stride), int(capacity)))
+
+    def get_training(self, solver):
+        """Returns (x [k, 8], t [k]) recorded since the last call and clears
+        the record."""
+        cap = getattr(self, "_cap", {}).get(solver, 0)
+        x = np.zeros((max(cap, 1), 8), np.float32)
+        t = np.zeros(max(cap, 1), np.float32)
+        n = C.c_size_t(0)
+        _chk(lib().sph_dfsph_get_training(self.h, self._SOLVERS[solver], x.ctypes.data_as(_fp),
+                                          t.ctypes.data_as(_fp), cap, C.byref(n)))
+        return x[: n.value].copy(), t[: n.value].copy()
 
     def enable_timing(self, on=True):
         _chk(lib().sph_dfsph_enable_timing(self.h, int(on)))

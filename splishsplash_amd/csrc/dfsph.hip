@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "dfsph.hpp"
+#include "mlp.hpp"
 
 namespace sph {
 
@@ -36,9 +37,11 @@ __global__ void k_reset(Ctl *C, int which) {
     if (which == 0) {
         C->it = 0;
         C->done = 0;
+        C->nlearn = 0;
     } else if (which == 1) {
         C->it_v = 0;
         C->done_v = 0;
+        C->nlearn_v = 0;
     } else if (which == 2) {
         C->maxcnt_f = 0;
         C->maxcnt_b = 0;
@@ -100,8 +103,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
             if (DIV) divb += xb.w * (((vi.x * gx + vi.y * gy) + vi.z * gz));
         }
     } else if (D.bmode == 2) {
-        const float4 xb = D.vmb[i];
-        if (xb.w > 0.f) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
             float3 r;
             float w, g;
             kernel_WG(xi, xb, D.K, r, w, g);
@@ -132,7 +136,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
             D.radv[i] = rd;
             float dAdv = fmaxf(rd, 0.0f);
             if (nnb < 20u) dAdv = 0.0f;
-            D.pv[i] = (dAdv > 0.0f) ? 0.5f * fminf(D.pv[i], 0.5f) * invH : 0.0f;
+            const float pv0 = D.pv[i];
+            if (D.kprev) D.kprev[i] = pv0;
+            D.pv[i] = (dAdv > 0.0f) ? 0.5f * fminf(pv0, 0.5f) * invH : 0.0f;
         }
     }
 }
@@ -186,8 +192,9 @@ __device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float 
                 az += pi * (-xb.w * (g * r.z));
             }
         } else if (D.bmode == 2) {
-            const float4 xb = D.vmb[i];
-            if (xb.w > 0.f) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
                 float3 r;
                 const float g = kernel_G(xi, xb, D.K, r);
                 ax += pi * (-xb.w * (g * r.x));
@@ -259,8 +266,9 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
                 s += xb.w * (((ai.x * (g * r.x) + ai.y * (g * r.y)) + ai.z * (g * r.z)));
             }
         } else if (D.bmode == 2) {
-            const float4 xb = D.vmb[i];
-            if (xb.w > 0.f) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
                 float3 r;
                 const float g = kernel_G(xi, xb, D.K, r);
                 s += xb.w * (((ai.x * (g * r.x) + ai.y * (g * r.y)) + ai.z * (g * r.z)));
@@ -396,8 +404,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
                         ai.z += s * vi.z * w;
                     }
                 } else if (D.bmode == 2) {
-                    const float4 xb = D.vmb[i];
-                    if (xb.w > 0.f) {
+                    for (unsigned o = 0; o < D.nvm; o++) {
+                        const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                        if (!(xb.w > 0.f)) break;
                         const float w = kernel_W(xi, xb, D.K);
                         const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
                         ai.x += s * vi.x * w;
@@ -480,8 +489,9 @@ __global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
             delta += xb.w * ((vi.x * (g * r.x) + vi.y * (g * r.y)) + vi.z * (g * r.z));
         }
     } else if (D.bmode == 2) {
-        const float4 xb = D.vmb[i];
-        if (xb.w > 0.f) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
             float3 r;
             const float g = kernel_G(xi, xb, D.K, r);
             delta += xb.w * ((vi.x * (g * r.x) + vi.y * (g * r.y)) + vi.z * (g * r.z));
@@ -491,7 +501,9 @@ __global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
     D.radv[i] = radv;
     const float invH2 = (float)(1.0 / (h * h));
     D.alpha[i] *= invH2;
-    D.p[i] = (radv > 1.0f) ? 0.5f * fminf(D.p[i], 0.00025f) * invH2 : 0.0f;
+    const float p0 = D.p[i];
+    if (D.kprev) D.kprev[i] = p0;
+    D.p[i] = (radv > 1.0f) ? 0.5f * fminf(p0, 0.00025f) * invH2 : 0.0f;
 }
 
 // p *= h^2 and the compensated position update x += h_step v (Kahan), then
@@ -593,8 +605,9 @@ __global__ __launch_bounds__(BLOCK) void k_weiler_diag(Dev D) {
                 m[5] += s * (gz * r.z);
             }
         } else if (D.bmode == 2) {
-            const float4 xb = D.vmb[i];
-            if (xb.w > 0.f) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
                 float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
                 const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
                 if (nl > 0.0001f) {
@@ -671,8 +684,9 @@ __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, const floa
                 az += s * (g * r.z);
             }
         } else if (D.bmode == 2) {
-            const float4 xb = D.vmb[i];
-            if (xb.w > 0.f) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
                 float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
                 const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
                 if (nl > 0.0001f) {
@@ -866,6 +880,247 @@ __global__ __launch_bounds__(BLOCK) void k_boundary_volume(float4 *bx, unsigned 
     bx[i].w = 1.0f / delta;
 }
 
+// ----------------------------------------------------- learned warm start
+// Block maxima of the previous-step stiffness, then the step's scale.
+__global__ __launch_bounds__(BLOCK) void k_max_partial(Dev D, const float *__restrict__ q) {
+    __shared__ double lds[BLOCK / 64];
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = (i < D.n) ? (double)fmaxf(q[i], 0.0f) : 0.0;
+    const double m = block_max(v, lds);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = m;
+}
+template <bool PRESSURE>
+__global__ __launch_bounds__(RBLOCK) void k_kref_reduce(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    double t = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t = fmax(t, D.partial[b]);
+    t = block_max(t, lds);
+    if (threadIdx.x == 0) {
+        if (PRESSURE)
+            D.ctl->kref = t;
+        else
+            D.ctl->kref_v = t;
+    }
+}
+
+// Per-particle inputs of the warm-start network: the solve's source term
+// (density deviation 1 - rho_adv, or the divergence rho_dot h), its kernel-
+// weighted neighbourhood mean, the previous-step stiffness and its mean (both
+// relative to the step's largest), the boundary volume (relative to V) and
+// the neighbour count / 40.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_features(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const double h = D.ctl->h;
+    const float sh = PRESSURE ? 1.0f : (float)h;
+    const float off = PRESSURE ? 1.0f : 0.0f;
+    const float kref = (float)(PRESSURE ? D.ctl->kref : D.ctl->kref_v);
+    const float ik = kref > 0.f ? 1.0f / kref : 0.0f;
+    const float4 xi = D.x[i];
+    const float src_i = (D.radv[i] - off) * sh;
+    const float w0 = D.V * D.K.W0;
+    float wsum = w0, ssrc = w0 * src_i, sk = w0 * D.kprev[i];
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float w = D.V * kernel_W(xi, D.x[j], D.K);
+        wsum += w;
+        ssrc += w * ((D.radv[j] - off) * sh);
+        sk += w * D.kprev[j];
+    }
+    float bvol = 0.f;
+    unsigned nnb = cnt;
+    if (D.bmode == 1) {
+        const unsigned cb = D.cb[i];
+        nnb += cb;
+        for (unsigned k = 0; k < cb; k++) bvol += D.bx[D.nb[(size_t)k * D.stride + i]].w;
+    } else if (D.bmode == 2) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float w = D.vmb[(size_t)o * D.stride + i].w;
+            if (!(w > 0.f)) break;
+            bvol += w;
+        }
+    }
+    float *F = (PRESSURE ? D.feats : D.feats_v) + (size_t)i * 8;
+    reinterpret_cast<float4 *>(F)[0] =
+        make_float4(100.0f * src_i, 100.0f * ssrc / wsum, D.kprev[i] * ik, (sk / wsum) * ik);
+    reinterpret_cast<float4 *>(F)[1] = make_float4(bvol / D.V, (float)nnb * (1.0f / 40.0f), 0.f, 0.f);
+}
+
+// Network output -> start value of the solve: p = max(y, 0) kref / h^2, or
+// p_v = max(y, 0) kref_v / h where the reference's divergence warm start
+// applies (rho_dot > 0 and at least 20 neighbours, TimeStepDFSPH.cpp:418-453).
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_pred_start(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const double h = D.ctl->h;
+    const float sc = PRESSURE ? (float)(D.ctl->kref / (h * h)) : (float)(D.ctl->kref_v / h);
+    float q = fmaxf(D.pred[i], 0.0f) * sc;
+    if (!PRESSURE) {
+        const unsigned nnb = D.cf[i] + (D.bmode == 1 ? D.cb[i] : 0u);
+        if (!(D.radv[i] > 0.0f) || nnb < 20u) q = 0.0f;
+    }
+    D.pred[i] = q;
+}
+
+// Pressure accelerations of both candidate starts (plain P, predicted Q).
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_accel2(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    const float *__restrict__ P = PRESSURE ? D.p : D.pv;
+    const float *__restrict__ Q = D.pred;
+    const float4 xi = D.x[i];
+    const float pi = P[i], qi = Q[i];
+    float ax = 0.f, ay = 0.f, az = 0.f, bxx = 0.f, byy = 0.f, bzz = 0.f;
+    const unsigned cnt = D.cf[i];
+    for (unsigned k = 0; k < cnt; k++) {
+        const unsigned j = D.nf[(size_t)k * D.stride + i];
+        const float ps = pi + P[j], qs = qi + Q[j];
+        if (fabsf(ps) > D.eps || fabsf(qs) > D.eps) {
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float gx = -D.V * (g * r.x), gy = -D.V * (g * r.y), gz = -D.V * (g * r.z);
+            if (fabsf(ps) > D.eps) {
+                ax += ps * gx;
+                ay += ps * gy;
+                az += ps * gz;
+            }
+            if (fabsf(qs) > D.eps) {
+                bxx += qs * gx;
+                byy += qs * gy;
+                bzz += qs * gz;
+            }
+        }
+    }
+    const bool bp = fabsf(pi) > D.eps, bq = fabsf(qi) > D.eps;
+    if (bp || bq) {
+        auto bterm = [&](const float4 &xb) {
+            float3 r;
+            const float g = kernel_G(xi, xb, D.K, r);
+            const float gx = -xb.w * (g * r.x), gy = -xb.w * (g * r.y), gz = -xb.w * (g * r.z);
+            if (bp) {
+                ax += pi * gx;
+                ay += pi * gy;
+                az += pi * gz;
+            }
+            if (bq) {
+                bxx += qi * gx;
+                byy += qi * gy;
+                bzz += qi * gz;
+            }
+        };
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned k = 0; k < cb; k++) bterm(D.bx[D.nb[(size_t)k * D.stride + i]]);
+        } else if (D.bmode == 2) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
+                bterm(xb);
+            }
+        }
+    }
+    D.ap[i] = make_float4(ax, ay, az, 0.f);
+    D.ap2[i] = make_float4(bxx, byy, bzz, 0.f);
+}
+
+// First Jacobi iteration from the per-particle better of the two starts: the
+// one with the smaller initial residual |s - (A p)_i| is kept and updated as
+// in pressureSolveIteration / divergenceSolveIteration.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    const unsigned i = gtid();
+    const double h = C->h;
+    double err = 0.0;
+    bool took = false;
+    if (i < D.n) {
+        const float4 xi = D.x[i], ai = D.ap[i], bi = D.ap2[i];
+        float s1 = 0.f, s2 = 0.f;
+        const unsigned cnt = D.cf[i];
+        for (unsigned k = 0; k < cnt; k++) {
+            const unsigned j = D.nf[(size_t)k * D.stride + i];
+            const float4 aj = D.ap[j], bj = D.ap2[j];
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            s1 += ((ai.x - aj.x) * gx + (ai.y - aj.y) * gy) + (ai.z - aj.z) * gz;
+            s2 += ((bi.x - bj.x) * gx + (bi.y - bj.y) * gy) + (bi.z - bj.z) * gz;
+        }
+        s1 *= D.V;
+        s2 *= D.V;
+        unsigned nnb = cnt;
+        auto bterm = [&](const float4 &xb) {
+            float3 r;
+            const float g = kernel_G(xi, xb, D.K, r);
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            s1 += xb.w * ((ai.x * gx + ai.y * gy) + ai.z * gz);
+            s2 += xb.w * ((bi.x * gx + bi.y * gy) + bi.z * gz);
+        };
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            nnb += cb;
+            for (unsigned k = 0; k < cb; k++) bterm(D.bx[D.nb[(size_t)k * D.stride + i]]);
+        } else if (D.bmode == 2) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
+                bterm(xb);
+            }
+        }
+        const float sc = PRESSURE ? (float)(h * h) : (float)h;
+        const float si = PRESSURE ? 1.0f - D.radv[i] : -D.radv[i];
+        const float r1 = si - s1 * sc, r2 = si - s2 * sc;
+        took = fabsf(r2) < fabsf(r1);
+        const float start = took ? D.pred[i] : (PRESSURE ? D.p[i] : D.pv[i]);
+        const float rr = took ? r2 : r1;
+        float res = fminf(rr, 0.0f);
+        if (!PRESSURE && nnb < 20u) res = 0.0f;
+        const float pn = fmaxf(start - 0.5f * rr * D.alpha[i], 0.0f);
+        if (PRESSURE)
+            D.p[i] = pn;
+        else
+            D.pv[i] = pn;
+        err = -(double)D.rho0 * (double)res;
+    }
+    const unsigned long long m = __ballot(took);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(PRESSURE ? &C->nlearn : &C->nlearn_v, (unsigned)__popcll(m));
+    const double bs = block_sum(err, lds);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+}
+
+// Training samples: features of every stride-th particle and its converged
+// stiffness (the stored p h^2 or p_v h) relative to the step's scale.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_collect(Dev D) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    Ctl *C = D.ctl;
+    const unsigned stride = PRESSURE ? D.sstride : D.sstride_v;
+    const double kref = PRESSURE ? C->kref : C->kref_v;
+    const bool take = (i < D.n) && stride && (i % stride) == (D.soff % stride) && kref > 0.0;
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const unsigned lane = threadIdx.x & 63u;
+    unsigned base = 0;
+    if (lane == (unsigned)__ffsll((long long)m) - 1u)
+        base = atomicAdd(PRESSURE ? &C->nsamp : &C->nsamp_v, (unsigned)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1, 64);
+    if (!take) return;
+    const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    const unsigned cap = PRESSURE ? D.scap : D.scap_v;
+    if (slot >= cap) return;
+    const float *F = (PRESSURE ? D.feats : D.feats_v) + (size_t)i * 8;
+    float *X = (PRESSURE ? D.sx : D.sx_v) + (size_t)slot * 8;
+    reinterpret_cast<float4 *>(X)[0] = reinterpret_cast<const float4 *>(F)[0];
+    reinterpret_cast<float4 *>(X)[1] = reinterpret_cast<const float4 *>(F)[1];
+    const float kf = PRESSURE ? D.p[i] : D.pv[i];
+    (PRESSURE ? D.st : D.st_v)[slot] = (float)(kf / kref);
+}
+
 // ==================================================================== host
 DFSPH::DFSPH(const sph_dfsph_params_t &p) : P(p) {
     require_device();
@@ -911,14 +1166,22 @@ void DFSPH::add_boundary(const float *x, size_t n) {
 
 void DFSPH::set_volume_map_box(const double lo[3], const double hi[3], int invert, double thickness,
                                const unsigned res[3]) {
+    Shape sh;
+    sh.kind = SHAPE_BOX;
     for (int c = 0; c < 3; c++) {
-        vm_lo_[c] = lo[c];
-        vm_hi_[c] = hi[c];
-        vm_res_[c] = res[c];
+        sh.box_lo[c] = lo[c];
+        sh.box_hi[c] = hi[c];
     }
-    vm_invert_ = invert;
-    vm_thick_ = thickness;
-    vm_requested_ = true;
+    sh.invert = invert;
+    sh.thickness = thickness;
+    add_volume_map(sh, res);
+}
+
+void DFSPH::add_volume_map(const Shape &shape, const unsigned res[3]) {
+    SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
+    SPH_REQUIRE(vm_shapes_.size() < 64, SPH_ERR_INVALID, "volume map: at most 64 objects");
+    vm_shapes_.push_back(shape);
+    vm_res_.push_back({res[0], res[1], res[2]});
     initialized_ = false;
 }
 
@@ -934,6 +1197,7 @@ Dev DFSPH::dev() const {
     D.vdiff = vdiff_.get();
     D.xc = xc_.get();
     D.vmb = vmb_.get();
+    D.nvm = nvm_;
     D.rho = rho_.get();
     D.alpha = alpha_.get();
     D.radv = radv_.get();
@@ -945,7 +1209,7 @@ Dev DFSPH::dev() const {
     D.cb = cb_.get();
     D.bx = bx_.get();
     const int bm = P.boundary_handling_method;
-    D.bmode = (bm == SPH_BOUNDARY_AKINCI2012 && nb_ > 0) ? 1 : ((bm == SPH_BOUNDARY_BENDER2019 && vmap.built) ? 2 : 0);
+    D.bmode = (bm == SPH_BOUNDARY_AKINCI2012 && nb_ > 0) ? 1 : ((bm == SPH_BOUNDARY_BENDER2019 && nvm_ > 0) ? 2 : 0);
     D.V = (float)V;
     D.rho0 = (float)P.density0;
     D.eps = 1.0e-5f;
@@ -977,6 +1241,20 @@ Dev DFSPH::dev() const {
     D.hs2 = (float)(hs * hs);
     D.viscoMaxIt = P.visco_max_iter;
     D.viscoTol = P.visco_max_error;
+    D.feats = feats_[0].get();
+    D.feats_v = feats_[1].get();
+    D.pred = pred_.get();
+    D.kprev = nullptr;  // set per solve by the callers that save it
+    D.ap2 = ap2_.get();
+    D.sx = sx_[0].get();
+    D.st = st_[0].get();
+    D.sx_v = sx_[1].get();
+    D.st_v = st_[1].get();
+    D.scap = (unsigned)cap_s_[0];
+    D.scap_v = (unsigned)cap_s_[1];
+    D.sstride = stride_s_[0];
+    D.sstride_v = stride_s_[1];
+    D.soff = soff_;
     return D;
 }
 
@@ -998,7 +1276,11 @@ void DFSPH::init() {
     const size_t n = n_;
     stride_ = (unsigned)((n + 255) / 256 * 256);
     const size_t S = stride_;
-    for (DBuf<float4> *b : {&x_, &v_, &v2_, &a_, &ap_, &vdiff_, &xc_, &vmb_, &x2_, &vdiff2_, &xc2_}) b->reserve(S);
+    for (DBuf<float4> *b : {&x_, &v_, &v2_, &a_, &ap_, &vdiff_, &xc_, &x2_, &vdiff2_, &xc2_}) b->reserve(S);
+    const bool bender = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019;
+    const size_t nobj = bender ? vm_shapes_.size() : 0;
+    vmb_.reserve(std::max<size_t>(nobj, 1) * S);
+    if (nobj > 1) vmraw_.reserve(nobj * S);
     for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_, &p2_, &pv2_}) b->reserve(S);
     id_.reserve(S);
     id2_.reserve(S);
@@ -1016,8 +1298,10 @@ void DFSPH::init() {
         std::vector<float> zf(n, 0.f);
         SPH_HIP(hipMemcpy(x_.get(), hx.data(), n * sizeof(float4), hipMemcpyHostToDevice));
         SPH_HIP(hipMemcpy(v_.get(), hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        for (DBuf<float4> *b : {&a_, &ap_, &vdiff_, &xc_, &vmb_})
+        for (DBuf<float4> *b : {&a_, &ap_, &vdiff_, &xc_})
             SPH_HIP(hipMemcpy(b->get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        for (size_t o = 0; o < std::max<size_t>(nobj, 1); o++)
+            SPH_HIP(hipMemcpy(vmb_.get() + o * S, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
         for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_})
             SPH_HIP(hipMemcpy(b->get(), zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
         SPH_HIP(hipMemcpy(id_.get(), ids.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
@@ -1033,10 +1317,11 @@ void DFSPH::init() {
     };
     ext(hx_);
     ext(hbx_);
-    if (vm_requested_)
+    for (const Shape &sh : vm_shapes_)
         for (int c = 0; c < 3; c++) {
-            lo[c] = std::min(lo[c], vm_lo_[c]);
-            hi[c] = std::max(hi[c], vm_hi_[c]);
+            const bool sp = sh.kind == SHAPE_SPHERE;
+            lo[c] = std::min(lo[c], sp ? sh.centre[c] - sh.radius : sh.box_lo[c]);
+            hi[c] = std::max(hi[c], sp ? sh.centre[c] + sh.radius : sh.box_hi[c]);
         }
     for (int c = 0; c < 3; c++) {
         lo[c] -= 2.0 * hs;
@@ -1078,8 +1363,14 @@ void DFSPH::init() {
         SPH_HIP(hipGetLastError());
         SPH_HIP(hipStreamSynchronize(stream));
     }
-    if (vm_requested_ && P.boundary_handling_method == SPH_BOUNDARY_BENDER2019)
-        vmap.build(stream, vm_lo_, vm_hi_, vm_invert_, vm_thick_, vm_res_, hs);
+    vmaps.clear();
+    for (size_t o = 0; o < nobj; o++) {
+        vmaps.emplace_back(new VolumeMap());
+        vmaps.back()->build(stream, vm_shapes_[o], vm_res_[o].data(), hs);
+    }
+    nvm_ = (unsigned)nobj;
+    initialized_ = true;
+    if (learned_[0] || learned_[1] || stride_s_[0] || stride_s_[1]) ensure_learn_buffers();
     if (P.viscosity_method == SPH_VISCOSITY_WEILER2018) {
         minv_.reserve(2 * S);
         for (DBuf<float4> *b : {&cgx_, &cgr_, &cgp_, &cgz_, &cgt_}) b->reserve(S);
@@ -1142,8 +1433,17 @@ void DFSPH::neighborhood_search() {
 }
 
 void DFSPH::volume_maps() {
-    if (P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 || !vmap.built) return;
-    sample_volume_map(stream, vmap.d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(), hs, r, timer);
+    if (P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 || nvm_ == 0) return;
+    // objects in insertion order: a particle projected out of one object is
+    // sampled against the next at its new position (TimeStep.cpp:177-222)
+    if (nvm_ == 1) {
+        sample_volume_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(), hs, r, timer);
+        return;
+    }
+    for (unsigned o = 0; o < nvm_; o++)
+        sample_volume_map(stream, vmaps[o]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(),
+                          vmraw_.get() + (size_t)o * stride_, hs, r, timer);
+    pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer);
 }
 
 #define LAUNCH(name, kern, ...)                                   \
@@ -1161,7 +1461,8 @@ void DFSPH::volume_maps() {
     } while (0)
 
 void DFSPH::sweep_init(bool dens, bool fact, bool div) {
-    const Dev D = dev();
+    Dev D = dev();
+    if (div && learn_or_collect(1)) D.kprev = kprev_.get();
     if (dens && fact && div)
         LAUNCH("density_factor_divsetup", (k_sweep_init<true, true, true>), D);
     else if (dens && fact)
@@ -1195,8 +1496,101 @@ void DFSPH::div_finish(bool scale_pv) {
 }
 
 void DFSPH::pressure_setup() {
-    const Dev D = dev();
+    Dev D = dev();
+    if (learn_or_collect(0)) D.kprev = kprev_.get();
     LAUNCH("pressure_setup", k_psetup, D);
+}
+
+// Scale, features and (with a model) the network's start plus the first
+// iteration from the per-particle better start. Returns iterations done.
+void DFSPH::learned_start(bool pressure) {
+    const int solver = pressure ? 0 : 1;
+    if (!learn_or_collect(solver)) return;
+    Dev D = dev();
+    D.kprev = kprev_.get();
+    LAUNCH("learn_scale", k_max_partial, D, kprev_.get());
+    if (pressure)
+        REDUCE("learn_scale", k_kref_reduce<true>, D, nblocks(n_));
+    else
+        REDUCE("learn_scale", k_kref_reduce<false>, D, nblocks(n_));
+    if (pressure)
+        LAUNCH("learn_features", k_features<true>, D);
+    else
+        LAUNCH("learn_features", k_features<false>, D);
+    if (!learned_[solver]) return;
+    {
+        TimedScope ts(timer, stream, "learn_mlp");
+        mlp_forward(stream, theta_[solver].get(), feats_[solver].get(), (unsigned)n_, pred_.get());
+    }
+    if (pressure) {
+        LAUNCH("learn_start", k_pred_start<true>, D);
+        LAUNCH("learn_accel2", k_accel2<true>, D);
+        LAUNCH("learn_jacobi2", k_jacobi2<true>, D);
+        REDUCE("jacobi_reduce", k_jacobi_reduce<true>, D, nblocks(n_), 0);
+    } else {
+        LAUNCH("learn_start", k_pred_start<false>, D);
+        LAUNCH("learn_accel2", k_accel2<false>, D);
+        LAUNCH("learn_jacobi2", k_jacobi2<false>, D);
+        REDUCE("jacobi_reduce", k_jacobi_reduce<false>, D, nblocks(n_), 0);
+    }
+}
+
+void DFSPH::collect(bool pressure) {
+    const int solver = pressure ? 0 : 1;
+    if (stride_s_[solver] == 0) return;
+    const Dev D = dev();
+    if (pressure)
+        LAUNCH("learn_collect", k_collect<true>, D);
+    else
+        LAUNCH("learn_collect", k_collect<false>, D);
+}
+
+void DFSPH::set_warm_start_model(int solver, const float *theta) {
+    SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
+    if (!theta) {
+        learned_[solver] = false;
+        return;
+    }
+    theta_[solver].reserve(MLP_NPARAM);
+    SPH_HIP(hipMemcpy(theta_[solver].get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    learned_[solver] = true;
+    ensure_learn_buffers();
+}
+
+void DFSPH::collect_training(int solver, unsigned stride, size_t capacity) {
+    SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
+    SPH_REQUIRE(capacity < (size_t)0xffffffffu, SPH_ERR_INVALID, "capacity too large");
+    stride_s_[solver] = stride;
+    cap_s_[solver] = stride ? capacity : 0;
+    if (stride) {
+        sx_[solver].reserve(std::max<size_t>(capacity, 1) * 8);
+        st_[solver].reserve(std::max<size_t>(capacity, 1));
+        ensure_learn_buffers();
+    }
+    const unsigned zero = 0;
+    upload_ctl_field(solver == 0 ? offsetof(Ctl, nsamp) : offsetof(Ctl, nsamp_v), &zero, sizeof(unsigned));
+}
+
+void DFSPH::get_training(int solver, float *x, float *t, size_t capacity, size_t *n) {
+    SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
+    read_ctl();
+    const size_t have = std::min<size_t>(solver == 0 ? hctl_->nsamp : hctl_->nsamp_v, cap_s_[solver]);
+    const size_t k = std::min(have, capacity);
+    if (k && x) SPH_HIP(hipMemcpy(x, sx_[solver].get(), k * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    if (k && t) SPH_HIP(hipMemcpy(t, st_[solver].get(), k * sizeof(float), hipMemcpyDeviceToHost));
+    if (n) *n = k;
+    const unsigned zero = 0;
+    upload_ctl_field(solver == 0 ? offsetof(Ctl, nsamp) : offsetof(Ctl, nsamp_v), &zero, sizeof(unsigned));
+}
+
+void DFSPH::ensure_learn_buffers() {
+    if (!initialized_) return;  // init() allocates them
+    const size_t S = stride_;
+    feats_[0].reserve(8 * S);
+    feats_[1].reserve(8 * S);
+    pred_.reserve(S);
+    kprev_.reserve(S);
+    ap2_.reserve(S);
 }
 
 void DFSPH::pressure_finish(bool finalize_too) {
@@ -1256,9 +1650,9 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
     if (vupd) v_.swap(v2_);
 }
 
-int DFSPH::run_solver(bool pressure) {
+int DFSPH::run_solver(bool pressure, int already) {
     const int maxIt = pressure ? P.max_iterations : P.max_iterations_v;
-    int launched = 0;
+    int launched = already;
     int chunk = std::max(1, pressure ? std::max(lastIt_, P.min_iterations) : lastItV_);
     for (;;) {
         for (int c = 0; c < chunk && launched < maxIt; c++, launched++) {
@@ -1313,7 +1707,8 @@ void DFSPH::step() {
     if (div) k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
     sweep_init(true, true, div);
     if (div) {
-        itVLast_ = run_solver(false);
+        learned_start(false);
+        itVLast_ = run_solver(false, learned_[1] ? 1 : 0);
         lastItV_ = std::max(1, itVLast_);
         totItV_ += itVLast_;
         div_finish(false);
@@ -1324,12 +1719,16 @@ void DFSPH::step() {
     if (weiler) viscosity(false);
     post_forces(true, weiler, P.xsph != 0.0 || P.xsph_boundary != 0.0, P.cfl_method == 1 || P.cfl_method == 2, true,
                 div);
+    if (div) collect(false);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
     pressure_setup();
-    itLast_ = run_solver(true);
+    learned_start(true);
+    itLast_ = run_solver(true, learned_[0] ? 1 : 0);
     lastIt_ = std::max(P.min_iterations, itLast_);
     totIt_ += itLast_;
     pressure_finish(true);
+    collect(true);
+    soff_++;
     steps_++;
 }
 
@@ -1350,7 +1749,8 @@ void DFSPH::stage(int s) {
         case SPH_STAGE_DIVERGENCE_SOLVE:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
             sweep_init(false, false, true);
-            itVLast_ = run_solver(false);
+            learned_start(false);
+            itVLast_ = run_solver(false, learned_[1] ? 1 : 0);
             totItV_ += itVLast_;
             div_finish(true);
             break;
@@ -1375,7 +1775,8 @@ void DFSPH::stage(int s) {
         case SPH_STAGE_PRESSURE_SOLVE:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
             pressure_setup();
-            itLast_ = run_solver(true);
+            learned_start(true);
+            itLast_ = run_solver(true, learned_[0] ? 1 : 0);
             totIt_ += itLast_;
             pressure_finish(false);
             finalize(true, false);
@@ -1426,8 +1827,21 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
     else if (name == "acceleration") f4 = a_.get(), comps = 3;
     else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
     else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
-    else if (name == "boundary x") f4 = vmb_.get(), comps = 3, vmpart = 1;
-    else if (name == "boundary volume") f4 = vmb_.get(), comps = 1, vmpart = 2;
+    else if (name.rfind("boundary x", 0) == 0 || name.rfind("boundary volume", 0) == 0) {
+        // "boundary x" / "boundary volume" of volume-map object 0, or "... <o>"
+        const bool isx = name.rfind("boundary x", 0) == 0;
+        const std::string rest = name.substr(isx ? 10 : 15);
+        unsigned o = 0;
+        if (!rest.empty()) {
+            SPH_REQUIRE(rest[0] == ' ' && rest.size() > 1 && rest.find_first_not_of("0123456789", 1) == std::string::npos,
+                        SPH_ERR_NOTFOUND, "unknown field '" + name + "'");
+            o = (unsigned)std::stoul(rest.substr(1));
+        }
+        SPH_REQUIRE(o < std::max(nvm_, 1u), SPH_ERR_NOTFOUND, "no volume map " + std::to_string(o));
+        f4 = (nvm_ > 1 ? vmraw_.get() : vmb_.get()) + (size_t)o * stride_;
+        comps = isx ? 3 : 1;
+        vmpart = isx ? 1 : 2;
+    }
     else if (name == "density") f1 = rho_.get();
     else if (name == "factor") f1 = alpha_.get();
     else if (name == "advected density") f1 = radv_.get();
@@ -1561,6 +1975,8 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->max_fluid_neighbors = (int)hctl_->maxcnt_f;
     c->max_boundary_neighbors = (int)hctl_->maxcnt_b;
     c->flags = hctl_->flags;
+    c->learned_particles = learned_[0] ? (int)hctl_->nlearn : 0;
+    c->learned_particles_v = learned_[1] ? (int)hctl_->nlearn_v : 0;
 }
 
 void DFSPH::set_time(double t, double h) {

@@ -2,6 +2,7 @@
 // SPlisHSPlasH/DFSPH/TimeStepDFSPH.cpp) with its neighbour search, Akinci2012
 // and Bender2019 boundaries, XSPH and the Weiler2018 implicit viscosity.
 #pragma once
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -28,9 +29,10 @@ struct Ctl {
     int cg_it, cg_done;
     int it_prev;       // pressure iterations of the previous solve (CFL method 2)
     int flags;         // bit0: non-finite value in a reduction
-    unsigned ticket;   // last-block counter (returns to 0 after each use)
     unsigned maxcnt_f, maxcnt_b;
-    unsigned pad;
+    double kref, kref_v;          // stiffness scales of the learned start
+    unsigned nlearn, nlearn_v;    // particles that kept the learned start
+    unsigned nsamp, nsamp_v;      // training samples written
 };
 
 // Everything a sweep kernel needs, passed by value.
@@ -41,6 +43,8 @@ struct Dev {
     const unsigned *nf, *cf, *nb, *cb;
     const float4 *bx;  // Akinci particles (x, y, z, volume), cell-sorted
     int bmode;         // 0 none, 1 Akinci2012, 2 Bender2019
+    unsigned nvm;      // volume-map slots: vmb[o * stride + i], entries with
+                       // V_b > 0 first (object order), then V_b = 0
     float V, rho0, eps, mass;
     float3 grav;
     KernelConst K;
@@ -58,6 +62,12 @@ struct Dev {
     float mu, mub, hs2;
     int viscoMaxIt;
     double viscoTol;
+    // learned warm start
+    float *feats, *feats_v;  // 8 floats per particle
+    float *pred, *kprev;     // network output, previous-step stiffness
+    float4 *ap2;             // pressure accelerations of the predicted start
+    float *sx, *st, *sx_v, *st_v;
+    unsigned scap, scap_v, sstride, sstride_v, soff;
 };
 
 class DFSPH {
@@ -75,6 +85,7 @@ public:
     void add_boundary(const float *x, size_t n);
     void set_volume_map_box(const double lo[3], const double hi[3], int invert, double thickness,
                             const unsigned res[3]);
+    void add_volume_map(const Shape &shape, const unsigned res[3]);
     void init();
     void step();
     void stage(int s);
@@ -82,12 +93,15 @@ public:
     void get_field(const std::string &name, void *host, size_t bytes);
     void set_field(const std::string &name, const void *host, size_t bytes);
     void get_neighbors(int boundary, unsigned *offsets, unsigned *indices, size_t capacity, size_t *nnz);
+    void set_warm_start_model(int solver, const float *theta);
+    void collect_training(int solver, unsigned stride, size_t capacity);
+    void get_training(int solver, float *x, float *t, size_t capacity, size_t *n);
     void counters(sph_dfsph_counters_t *c);
     void set_time(double t, double h);
     void sync();
     size_t n() const { return n_; }
 
-    VolumeMap vmap;
+    std::vector<std::unique_ptr<VolumeMap>> vmaps;  // boundary objects, in insertion order
 
 private:
     size_t n_ = 0;
@@ -97,13 +111,12 @@ private:
     GridDesc grid_{};
     std::vector<float> hx_, hv_, hbx_;  // staged host inputs
     size_t nb_ = 0;
-    bool vm_requested_ = false;
-    double vm_lo_[3], vm_hi_[3], vm_thick_ = 0;
-    unsigned vm_res_[3];
-    int vm_invert_ = 1;
+    std::vector<Shape> vm_shapes_;             // staged volume-map objects
+    std::vector<std::array<unsigned, 3>> vm_res_;
+    unsigned nvm_ = 0;                         // volume maps built
 
     // device state
-    DBuf<float4> x_, v_, v2_, a_, ap_, vdiff_, xc_, vmb_, bx_;
+    DBuf<float4> x_, v_, v2_, a_, ap_, vdiff_, xc_, vmb_, vmraw_, bx_;
     DBuf<float4> x2_, vdiff2_, xc2_;  // permutation targets
     DBuf<float> rho_, alpha_, radv_, p_, pv_, p2_, pv2_;
     DBuf<unsigned> id_, id2_, nf_, cf_, nb_l_, cb_, bperm_, bstart_;
@@ -115,6 +128,17 @@ private:
     long long totIt_ = 0, totItV_ = 0, totVisco_ = 0;
     int steps_ = 0, lastIt_ = 2, lastItV_ = 1, lastCg_ = 4;
     int itLast_ = 0, itVLast_ = 0, cgLast_ = 0;
+    // learned warm start
+    bool learned_[2] = {false, false};
+    unsigned stride_s_[2] = {0, 0};
+    size_t cap_s_[2] = {0, 0};
+    DBuf<float> theta_[2], feats_[2], pred_, kprev_, sx_[2], st_[2];
+    DBuf<float4> ap2_;
+    unsigned soff_ = 0;
+    bool learn_or_collect(int solver) const { return learned_[solver] || stride_s_[solver] > 0; }
+    void learned_start(bool pressure);
+    void collect(bool pressure);
+    void ensure_learn_buffers();
 
     Dev dev() const;
     void upload_ctl_field(size_t off, const void *src, size_t bytes);
@@ -133,7 +157,7 @@ private:
     void pressure_iteration(bool force);
     void pressure_finish(bool finalize);
     void finalize(bool scale_p, bool update_x);
-    int run_solver(bool pressure);
+    int run_solver(bool pressure, int already = 0);
     int run_cg();
     std::vector<unsigned> ids_host();
 };

@@ -81,6 +81,22 @@ int sph_dfsph_set_volume_map_box(sph_dfsph_t s, const double box_min[3], const d
     });
 }
 
+int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3], double radius, int invert,
+                                    double thickness, const unsigned resolution[3]) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(centre && resolution, SPH_ERR_INVALID, "null argument");
+        SPH_REQUIRE(radius > 0, SPH_ERR_INVALID, "radius must be > 0");
+        Shape sh;
+        sh.kind = SHAPE_SPHERE;
+        for (int c = 0; c < 3; c++) sh.centre[c] = centre[c];
+        sh.radius = radius;
+        sh.invert = invert;
+        sh.thickness = thickness;
+        s->s->add_volume_map(sh, resolution);
+    });
+}
+
 int sph_dfsph_init(sph_dfsph_t s) {
     return guarded([&] {
         DCHK(s);
@@ -134,6 +150,27 @@ int sph_dfsph_get_neighbors(sph_dfsph_t s, int boundary, unsigned *offsets, unsi
     return guarded([&] {
         DCHK(s);
         s->s->get_neighbors(boundary, offsets, indices, capacity, nnz);
+    });
+}
+
+int sph_dfsph_set_warm_start_model(sph_dfsph_t s, int solver, const float *theta) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_warm_start_model(solver, theta);
+    });
+}
+
+int sph_dfsph_collect_training(sph_dfsph_t s, int solver, unsigned stride, size_t capacity) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->collect_training(solver, stride, capacity);
+    });
+}
+
+int sph_dfsph_get_training(sph_dfsph_t s, int solver, float *x, float *t, size_t capacity, size_t *n) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->get_training(solver, x, t, capacity, n);
     });
 }
 

@@ -48,18 +48,24 @@ static Gauss16 gauss16() {
     return g;
 }
 
-// Signed distance of the box surface: sign * (d_mesh - thickness) with
-// d_mesh < 0 inside (SimulatorBase.cpp:2609-2611).
-__device__ __forceinline__ double box_sdf(const double *p, const VMapDesc &d) {
-    bool in = true;
-    double dmin = DBL_MAX, q2 = 0.0;
-    for (int c = 0; c < 3; c++) {
-        if (p[c] < d.box_lo[c] || p[c] > d.box_hi[c]) in = false;
-        dmin = fmin(dmin, fmin(p[c] - d.box_lo[c], d.box_hi[c] - p[c]));
-        const double q = fmax(fmax(d.box_lo[c] - p[c], 0.0), p[c] - d.box_hi[c]);
-        q2 += q * q;
+// Signed distance of the shape's surface: sign * (d_shape - thickness) with
+// d_shape < 0 inside (SimulatorBase.cpp:2609-2611).
+__device__ __forceinline__ double shape_sdf(const double *p, const VMapDesc &d) {
+    double dm;
+    if (d.kind == SHAPE_SPHERE) {
+        const double a = p[0] - d.centre[0], b = p[1] - d.centre[1], c = p[2] - d.centre[2];
+        dm = sqrt((a * a + b * b) + c * c) - d.radius;
+    } else {
+        bool in = true;
+        double dmin = DBL_MAX, q2 = 0.0;
+        for (int c = 0; c < 3; c++) {
+            if (p[c] < d.box_lo[c] || p[c] > d.box_hi[c]) in = false;
+            dmin = fmin(dmin, fmin(p[c] - d.box_lo[c], d.box_hi[c] - p[c]));
+            const double q = fmax(fmax(d.box_lo[c] - p[c], 0.0), p[c] - d.box_hi[c]);
+            q2 += q * q;
+        }
+        dm = in ? -dmin : sqrt(q2);
     }
-    const double dm = in ? -dmin : sqrt(q2);
     return (d.invert ? -1.0 : 1.0) * (dm - d.thickness);
 }
 
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, u
     if (g >= total) return;
     double x[3];
     node_pos(g, d, nc, x);
-    const double dist_x = box_sdf(x, d);
+    const double dist_x = shape_sdf(x, d);
     sdf_node[g] = (float)dist_x;
     const double h = q.h;
     double res = 0.0;
@@ -141,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, u
                         for (int c = 0; c < 3; c++)
                             if (!(y[c] >= d.lo[c] && y[c] <= dom_hi[c])) indom = false;
                         if (indom) {
-                            const double dist = box_sdf(y, d);
+                            const double dist = shape_sdf(y, d);
                             if (dist <= 0.0)
                                 f = 1.0;
                             else if (dist < h)
@@ -198,22 +204,30 @@ __global__ __launch_bounds__(BLOCK) void k_vm_cells(VMapDesc d, NodeCounts nc, u
     vol_cell[t] = vol_node[g];
 }
 
-void VolumeMap::build(hipStream_t s, const double box_lo[3], const double box_hi[3], int invert, double thickness,
-                      const unsigned res[3], double h) {
+void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], double h) {
     SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
-    const double pad = 8.0 * h + thickness;
+    SPH_REQUIRE(shape.kind == SHAPE_BOX || shape.kind == SHAPE_SPHERE, SPH_ERR_INVALID, "volume map: unknown shape");
+    const double pad = 8.0 * h + shape.thickness;
+    d.kind = shape.kind;
+    d.radius = shape.radius;
+    if (shape.kind == SHAPE_SPHERE) SPH_REQUIRE(shape.radius > 0, SPH_ERR_INVALID, "volume map: radius must be > 0");
     for (int c = 0; c < 3; c++) {
-        SPH_REQUIRE(box_hi[c] > box_lo[c], SPH_ERR_INVALID, "volume map: empty box");
-        d.box_lo[c] = box_lo[c];
-        d.box_hi[c] = box_hi[c];
-        d.lo[c] = box_lo[c] - pad;
-        const double hi = box_hi[c] + pad;
+        d.box_lo[c] = shape.box_lo[c];
+        d.box_hi[c] = shape.box_hi[c];
+        d.centre[c] = shape.centre[c];
+        double lo = shape.box_lo[c], hi = shape.box_hi[c];
+        if (shape.kind == SHAPE_SPHERE) {
+            lo = shape.centre[c] - shape.radius;
+            hi = shape.centre[c] + shape.radius;
+        }
+        SPH_REQUIRE(hi > lo, SPH_ERR_INVALID, "volume map: empty box");
+        d.lo[c] = lo - pad;
         d.res[c] = res[c];
-        d.cs[c] = (hi - d.lo[c]) / (double)res[c];
+        d.cs[c] = (hi + pad - d.lo[c]) / (double)res[c];
         d.inv[c] = 1.0 / d.cs[c];
     }
-    d.invert = invert;
-    d.thickness = thickness;
+    d.invert = shape.invert;
+    d.thickness = shape.thickness;
     ncells = (size_t)res[0] * res[1] * res[2];
     const NodeCounts nc = node_counts(res);
     const unsigned long long total = nc.nc + nc.nx + nc.ny + nc.nz;
@@ -373,6 +387,26 @@ void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, 
     if (!n) return;
     TimedScope ts(T, s, "volume_map_sample");
     k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, supportRadius, particleRadius);
+    SPH_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(BLOCK) void k_vm_pack(unsigned n, unsigned stride, unsigned nobj,
+                                                   const float4 *__restrict__ raw, float4 *__restrict__ vmb) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned k = 0;
+    for (unsigned o = 0; o < nobj; o++) {
+        const float4 e = raw[(size_t)o * stride + i];
+        if (e.w > 0.f) vmb[(size_t)(k++) * stride + i] = e;
+    }
+    for (; k < nobj; k++) vmb[(size_t)k * stride + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
+                      KernelTimer &T) {
+    if (!n) return;
+    TimedScope ts(T, s, "volume_map_pack");
+    k_vm_pack<<<nblocks(n), BLOCK, 0, s>>>(n, stride, nobj, raw, vmb);
     SPH_HIP(hipGetLastError());
 }
 

@@ -1,11 +1,13 @@
-// volume_map.hpp -- Bender2019 volume map of a box wall on the device.
+// volume_map.hpp -- Bender2019 volume maps of static analytic shapes (a box
+// wall or a sphere) on the device.
 //
 // Discretisation (the scheme of Discregrid's CubicLagrangeDiscreteGrid as
 // used by SimulatorBase::initVolumeMap, SimulatorBase.cpp:2566-2704): the
-// domain is the box grown by 8h + thickness, cut into res[0..2] cells; each
-// cell carries the 32-node cubic serendipity element (8 corners + two nodes
-// on each of the 12 edges at 1/3 and 2/3). Field 0 is the signed distance
-// (positive on the fluid side for an inverted box); field 1 is the volume
+// domain is the shape's bounding box grown by 8h + thickness, cut into
+// res[0..2] cells; each cell carries the 32-node cubic serendipity element
+// (8 corners + two nodes on each of the 12 edges at 1/3 and 2/3). Field 0 is
+// the signed distance (positive on the fluid side: an inverted box holds the
+// fluid inside, a sphere keeps it outside); field 1 is the volume
 //     V(x) = 0.8 * int_{|xi|<h} gamma(sdf(x + xi)) dxi,
 //     gamma = 1 (sdf <= 0), W(sdf)/W(0) (0 < sdf < h), 0 otherwise,
 // integrated with the 16^3-point Gauss-Legendre product rule (p = 30).
@@ -18,12 +20,26 @@
 
 namespace sph {
 
+enum { SHAPE_BOX = 0, SHAPE_SPHERE = 1 };
+
+// Analytic boundary shape: sign * (d_shape(x) - thickness), d_shape < 0
+// inside the shape, sign = -1 when inverted.
+struct Shape {
+    int kind = SHAPE_BOX;
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // box
+    double centre[3] = {0, 0, 0}, radius = 0;              // sphere
+    int invert = 1;
+    double thickness = 0;
+};
+
 struct VMapDesc {
     double lo[3];    // domain min
     double cs[3];    // cell size
     double inv[3];   // 1 / cell size
     unsigned res[3];
+    int kind;
     double box_lo[3], box_hi[3];
+    double centre[3], radius;
     int invert;
     double thickness;
     const float *sdf;  // ncells x 32
@@ -37,18 +53,23 @@ public:
     size_t ncells = 0;
     bool built = false;
     // Builds both fields on the device (synchronous).
-    void build(hipStream_t s, const double box_lo[3], const double box_hi[3], int invert, double thickness,
-               const unsigned res[3], double supportRadius);
+    void build(hipStream_t s, const Shape &shape, const unsigned res[3], double supportRadius);
     // Copies cell records (32 sdf values then 32 volume values, as floats)
     // of the given cells to host memory.
     void cell_values(const uint64_t *cells, size_t n, float *out) const;
 };
 
 // Per-particle sampling, TimeStep::computeVolumeAndBoundaryX
-// (TimeStep.cpp:223-346) for one static box: writes vmb[i] = (x_b, V_b),
+// (TimeStep.cpp:223-346) for one static shape: writes vmb[i] = (x_b, V_b),
 // projects particles with sdf <= 0 back towards the surface and zeroes their
 // velocity; xc (position compensation) is cleared for moved particles.
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
                        double supportRadius, double particleRadius, KernelTimer &T);
+
+// Packs, per particle, the records with V_b > 0 of `nobj` object-major
+// arrays (raw[o * stride + i]) into the leading slots of vmb (same layout),
+// keeping object order, and zeroes the remaining slots.
+void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
+                      KernelTimer &T);
 
 }  // namespace sph

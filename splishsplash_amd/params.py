@@ -67,7 +67,9 @@ class DFSPHCounters(C.Structure):
         ("max_fluid_neighbors", C.c_int),
         ("max_boundary_neighbors", C.c_int),
         ("flags", C.c_int),
-        ("reserved", C.c_int * 7),
+        ("learned_particles", C.c_int),
+        ("learned_particles_v", C.c_int),
+        ("reserved", C.c_int * 5),
     ]
 
     def as_dict(self):

@@ -120,3 +120,38 @@ def clustered_points(n, seed=3, clusters=64, sigma=0.03, background=0.2):
     pts = centres[lab] + rng.normal(0.0, sigma, size=(nc, 3))
     pts = np.concatenate([pts, rng.uniform(0.0, 1.0, size=(nb, 3))])
     return f32(np.clip(pts, 0.0, 1.0))
+
+
+def dam_break_variant(seed, per_side=None, aspect=None, n_spheres=None, spacing=0.05, vm_cell=0.05, **kw):
+    """Seeded dam-break variant for the learned warm start: a cubic fluid
+    block of `per_side` (default U{60..120}) particles per axis at spacing
+    0.05 (r = 0.025) from corner (0.1, 0.1, 0.1), jitter U[-0.001, 0.001]^3;
+    box [0, L] x [0, b + 0.4] x [0, b + 0.2] with b the block edge and
+    L = aspect * b (aspect U[1.5, 3]); `n_spheres` (default U{0..8}) static
+    spheres of radius U[0.1, 0.4] placed in the box clear of the block, each a
+    Bender2019 volume map of its analytic SDF. Boundary, solver and viscosity
+    settings as in the 1M dam break."""
+    rng = np.random.default_rng(7919 * seed + 17)
+    n = int(per_side if per_side is not None else rng.integers(60, 121))
+    a = float(aspect if aspect is not None else rng.uniform(1.5, 3.0))
+    ns = int(n_spheres if n_spheres is not None else rng.integers(0, 9))
+    b = n * spacing
+    hi = (max(a * b, b + 0.4), b + 0.4, b + 0.2)
+    spheres = []
+    tries = 0
+    while len(spheres) < ns and tries < 1000:
+        tries += 1
+        r = rng.uniform(0.1, 0.4)
+        c = np.array([rng.uniform(r, hi[0] - r), rng.uniform(r, hi[1] - r), rng.uniform(r, hi[2] - r)])
+        # clear of the block [0.05, b + 0.15]^3 and of the other spheres
+        q = np.clip(c, 0.05, b + 0.15)
+        if np.linalg.norm(c - q) < r + 0.05:
+            continue
+        if any(np.linalg.norm(c - np.array(s[:3])) < r + s[3] + 0.05 for s in spheres):
+            continue
+        spheres.append((float(c[0]), float(c[1]), float(c[2]), float(r)))
+    sc = dam_break_1m(per_side=n, seed=seed, box_hi=hi, vm_cell=vm_cell, **kw)
+    sc["spheres"] = spheres
+    sc["sphere_res"] = vm_cell
+    sc["variant"] = {"seed": seed, "per_side": n, "aspect": a, "spheres": spheres}
+    return sc

@@ -45,7 +45,7 @@ def sync_state(g, o, extra=()):
     o.set_time(c["time"], c["time_step_size"])
 
 
-def make_pair(params, x, v, boundary_x=None, vm=None):
+def make_pair(params, x, v, boundary_x=None, vm=None, spheres=()):
     g = capi.TimeStepDFSPH(params)
     g.set_fluid(x, v)
     o = oracle.OracleDFSPH(params)
@@ -56,6 +56,9 @@ def make_pair(params, x, v, boundary_x=None, vm=None):
     if vm is not None:
         g.set_volume_map_box(*vm)
         o.set_volume_map_box(*vm)
+    for c, r in spheres:
+        g.add_volume_map_sphere(c, r, 0.05)
+        o.add_volume_map_sphere(c, r, 0.05)
     o.init()
     g.init()
     return g, o
@@ -150,6 +153,35 @@ def test_volume_map_stages_match_oracle():
     g.set("p / rho^2", p)
     g.set("p_v / rho^2", pv)
     run_stage_sequence(g, o, STEP_SEQ, extra_state=("boundary volume", "boundary x"))
+
+
+def test_volume_map_spheres_match_oracle():
+    # Box wall plus two sphere obstacles cutting into the fluid block: the
+    # particles inside a sphere are projected out of it, and near-surface
+    # particles see up to three boundary objects (summed in object order).
+    x, params, vm = vm_scene()
+    spheres = [((0.45, 0.45, 0.45), 0.12), ((0.80, 0.35, 0.60), 0.15)]
+    v, p, pv = perturbed_state(x, seed=4)
+    g, o = make_pair(params, x, v, vm=vm, spheres=spheres)
+    g.stage("neighborhood_search")
+    o.stage("neighborhood_search")
+    sync_state(g, o)
+    g.stage("volume_maps")
+    o.stage("volume_maps")
+    names = ["position", "velocity"]
+    for k in range(3):
+        names += [f"boundary volume {k}", f"boundary x {k}"]
+    for nm in names:
+        e = rel(g.get(nm), o.get(nm))
+        assert e <= TOL, f"volume_maps: {nm} rel err {e:.3e}"
+    for k in (1, 2):
+        assert (o.get(f"boundary volume {k}") > 0).sum() > 50
+    inside = [np.linalg.norm(x - np.array(c), axis=1) < r for c, r in spheres]
+    moved = (g.get("position") != x).any(axis=1)
+    assert moved[inside[0] | inside[1]].all()
+    g.set("p / rho^2", p)
+    g.set("p_v / rho^2", pv)
+    run_stage_sequence(g, o, STEP_SEQ, extra_state=tuple(names[2:]))
 
 
 def test_weiler_viscosity_matches_oracle():

@@ -131,3 +131,46 @@ def test_volume_map_interpolation_exact_where_sdf_linear():
     out = np.array([[-1.0, 1, 1], [3.5, 1, 1]])
     _, _, ok = oracle.volume_map_eval(lo, hi, res, 0.1, out, field=0, interpolate=True)
     assert not ok.any()
+
+
+def test_sphere_sdf_closed_form():
+    c, R = np.array([1.0, 0.5, 0.75]), 0.3
+    pts = np.random.default_rng(5).uniform(0.0, 2.0, (40, 3))
+    d = oracle.sphere_sdf(c, R, pts)
+    np.testing.assert_allclose(d, np.linalg.norm(pts - c, axis=1) - R, atol=1e-15)
+    # inverted and with thickness: sign * (d - thickness)
+    d2 = oracle.sphere_sdf(c, R, pts, invert=True, thickness=0.05)
+    np.testing.assert_allclose(d2, -(np.linalg.norm(pts - c, axis=1) - R - 0.05), atol=1e-15)
+
+
+def test_sphere_volume_tends_to_halfspace():
+    # Near the surface of a sphere of radius 1000 h the volume integrand is
+    # the half-space one up to O(h / R): compare with the map of the face
+    # y = 0 of a (non-inverted) box, which is exactly the half space there.
+    h, Rb = 0.1, 100.0
+    pts = np.array([[0.0, d, 0.0] for d in np.linspace(-0.05, 0.12, 8)])
+    v, _, ok = oracle.sphere_volume_map_eval((0.0, -Rb, 0.0), Rb, (4, 4, 4), h, pts, field=1, interpolate=False)
+    vb, _, _ = oracle.volume_map_eval((-5, -10, -5), (5, 0, 5), (4, 4, 4), h, pts, field=1, interpolate=False,
+                                      invert=False)
+    assert ok.all()
+    np.testing.assert_allclose(v, vb, rtol=2e-3)
+    # monotone in the distance; zero once the ball |xi| < h stays at sdf >= h
+    assert np.all(np.diff(v) < 0)
+    far, _, _ = oracle.sphere_volume_map_eval((0.0, -Rb, 0.0), Rb, (4, 4, 4), h, [[0.0, 0.2001, 0.0]], field=1,
+                                              interpolate=False)
+    assert far[0] == 0.0
+
+
+def test_sphere_volume_map_interpolates_sdf():
+    # The cubic element reproduces the sphere's distance to O(cell^4): at
+    # 0.05 m cells and R = 0.3 m the error is far below the particle radius.
+    c, R = np.array([1.0, 1.0, 1.0]), 0.3
+    res = [int(round(2 * (R + 0.8) / 0.05))] * 3
+    rng = np.random.default_rng(6)
+    dirs = rng.normal(size=(50, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    pts = c + dirs * rng.uniform(R - 0.05, R + 0.1, (50, 1))
+    val, grad, ok = oracle.sphere_volume_map_eval(c, R, res, 0.1, pts, field=0, interpolate=True)
+    assert ok.all()
+    np.testing.assert_allclose(val, np.linalg.norm(pts - c, axis=1) - R, atol=1e-4)
+    np.testing.assert_allclose(grad, dirs, atol=5e-3)
