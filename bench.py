@@ -5,9 +5,12 @@ Workload: the 1,000,000-particle dam break (100^3 lattice at r = 0.025 in a
 0.05 m cells, Weiler2018 viscosity nu = 0.01 with CG tolerance 1e-4, XSPH
 0.01, CFL factor 1 with dt <= 5 ms, max density error 0.01 %, max divergence
 error 0.1 %). A step is one TimeStepDFSPH::step() of the whole scene with all
-data resident in HBM. Prints one JSON line.
+data resident in HBM. The scene is run twice from the same initial state:
+with the reference's warm start ("plain") and with the learned warm start
+(the packaged networks); `value` is the learned run's throughput. Prints one
+JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--plain-only]
 """
 import argparse
 import json
@@ -42,6 +45,11 @@ BYTES = {
     "cg_update": lambda K: 108,
     "cg_pupdate": lambda K: 36,
     "volume_map_sample": lambda K: 156,
+    "learn_features": lambda K: 4 * K + 48,
+    "learn_mlp": lambda K: 36,
+    "learn_accel2": lambda K: 4 * K + 56,
+    "learn_jacobi2": lambda K: 4 * K + 80,
+    "learn_start": lambda K: 28,
     "finalize": lambda K: 68,
     "permute": lambda K: 124,
     "cell_hist": lambda K: 20,
@@ -104,88 +112,128 @@ def cpu_baseline(sim, sc, steps):
                       f"{oc['iterations']}/{oc['iterations_v']}/{oc['visco_iterations']})"}
 
 
+def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None):
+    """Steps one fresh simulation of `sc` (thetas: learned warm-start
+    networks or None for the reference's start): `warmup` untimed steps,
+    `breakdown_steps` steps with every kernel timed by HIP events (the
+    per-kernel table), then the `steps` timed steps, during which only
+    `timed_kernel` (default: the largest in the table) carries events."""
+    from splishsplash_amd import capi, learned
+
+    sim = learned.make_sim(sc, thetas)
+    sim.step(warmup)
+    kernels = {}
+    if breakdown_steps:
+        sim.enable_timing(True)
+        sim.time_only(None)
+        sim.reset_kernel_stats()
+        sim.step(breakdown_steps)
+        for name, (launches, ms) in sim.kernel_stats().items():
+            if launches:
+                kernels[name] = {"launches_per_step": launches / breakdown_steps,
+                                 "avg_us": 1e3 * ms / launches, "ms_per_step": ms / breakdown_steps}
+    if timed_kernel is None and kernels:
+        timed_kernel = max((k for k in kernels if k in BYTES), key=lambda k: kernels[k]["ms_per_step"], default=None)
+    sim.enable_timing(timed_kernel is not None)
+    if timed_kernel:
+        sim.time_only([timed_kernel])
+        sim.reset_kernel_stats()
+    c0 = sim.counters()
+    capi.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    capi.synchronize()
+    elapsed = time.perf_counter() - t0
+    c1 = sim.counters()
+    live = None
+    if timed_kernel:
+        st = sim.kernel_stats().get(timed_kernel)
+        if st and st[0]:
+            live = {"kernel": timed_kernel, "launches": st[0], "avg_us": 1e3 * st[1] / st[0]}
+    n = len(sc["x"])
+    res = {"value": n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "pressure_iterations_per_step": (c1["total_iterations"] - c0["total_iterations"]) / steps,
+           "divergence_iterations_per_step": (c1["total_iterations_v"] - c0["total_iterations_v"]) / steps,
+           "cg_iterations_per_step": (c1["total_visco_iterations"] - c0["total_visco_iterations"]) / steps,
+           "pressure_solves_at_cap": c1["capped_solves"] - c0["capped_solves"],
+           "divergence_solves_at_cap": c1["capped_solves_v"] - c0["capped_solves_v"],
+           "avg_density_err": c1["avg_density_err"], "avg_overpressure_err": c1["avg_overpressure_err"],
+           "time_step_size": c1["time_step_size"], "sim_time": c1["time"], "flags": c1["flags"],
+           "learned_particles": c1["learned_particles"]}
+    K = float(sim.get("fluid neighbor count").mean())
+    del sim
+    return res, kernels, live, K
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--breakdown-steps", type=int, default=10)
     ap.add_argument("--per-side", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP-event timing")
+    ap.add_argument("--plain-only", action="store_true", help="skip the learned warm-start run")
+    ap.add_argument("--weights", default=None, help="warm-start networks (default: the packaged ones)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(a multi-GPU run would shard the particles by slabs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from splishsplash_amd import capi, scenes
+    from splishsplash_amd import learned, scenes
 
     sc = scenes.dam_break_1m(per_side=args.per_side)
     n = len(sc["x"])
-    log(f"bench: {n} particles, volume map {list(sc['vm_res'])} cells")
-    sim = capi.TimeStepDFSPH(sc["params"])
-    sim.set_fluid(sc["x"], sc["v"])
-    sim.set_volume_map_box(sc["box_lo"], sc["box_hi"], sc["vm_res"])
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    log(f"bench: {n} particles, volume map {list(sc['vm_res'])} cells, {args.warmup} warm-up + "
+        f"{args.steps} timed steps per path")
     t0 = time.perf_counter()
-    sim.init()
-    log(f"bench: init (volume map build) {time.perf_counter() - t0:.2f} s")
-    t0 = time.perf_counter()
-    sim.step(args.warmup)
-    log(f"bench: {args.warmup} warm-up steps {time.perf_counter() - t0:.2f} s")
-    c0 = sim.counters()
-    if not args.no_timing:
-        sim.enable_timing(True)
-        sim.reset_kernel_stats()
-    capi.synchronize()
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    capi.synchronize()
-    elapsed = time.perf_counter() - t0
-    c1 = sim.counters()
-    value = n * args.steps / elapsed
-    log(f"bench: {args.steps} timed steps {elapsed:.3f} s -> {value:.4g} particle-steps/s")
-
-    it_p = (c1["total_iterations"] - c0["total_iterations"]) / args.steps
-    it_v = (c1["total_iterations_v"] - c0["total_iterations_v"]) / args.steps
-    it_cg = (c1["total_visco_iterations"] - c0["total_visco_iterations"]) / args.steps
-    K = float(sim.get("fluid neighbor count").mean())
+    plain, k_plain, live_plain, K = run_path(sc, None, args.warmup, args.steps, args.breakdown_steps)
+    log(f"bench: plain {plain['ms_per_step']:.3f} ms/step, {plain['pressure_iterations_per_step']:.2f} pressure "
+        f"iterations/step ({time.perf_counter() - t0:.1f} s)")
+    lrn = k_lrn = live_lrn = None
+    if not args.plain_only:
+        thetas = learned.load(args.weights or learned.WEIGHTS)
+        t0 = time.perf_counter()
+        lrn, k_lrn, live_lrn, _ = run_path(sc, thetas, args.warmup, args.steps, args.breakdown_steps)
+        log(f"bench: learned {lrn['ms_per_step']:.3f} ms/step, {lrn['pressure_iterations_per_step']:.2f} pressure "
+            f"iterations/step ({time.perf_counter() - t0:.1f} s)")
+    head, kernels, live = (lrn, k_lrn, live_lrn) if lrn else (plain, k_plain, live_plain)
 
     roofline = None
-    kernels = {}
-    if not args.no_timing:
-        stats = sim.kernel_stats()
-        for name, (launches, ms) in stats.items():
-            if launches:
-                kernels[name] = {"launches": launches, "avg_us": 1e3 * ms / launches, "total_ms": ms}
-        best = None
-        for name, s in kernels.items():
-            if name in BYTES and (best is None or s["total_ms"] > kernels[best]["total_ms"]):
-                best = name
-        if best:
-            bpp = BYTES[best](K)
-            achieved = n * bpp / (kernels[best]["avg_us"] * 1e-6) / 1e9
-            roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(best), "kernel": best,
-                        "bytes_per_particle": bpp, "avg_launch_us": round(kernels[best]["avg_us"], 2)}
-            for name, s in kernels.items():
-                if name in BYTES:
-                    s["gbs"] = round(n * BYTES[name](K) / (s["avg_us"] * 1e-6) / 1e9, 1)
+    if live:
+        name = live["kernel"]
+        bpp = BYTES[name](K)
+        achieved = n * bpp / (live["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name), "kernel": name,
+                    "bytes_per_particle": round(bpp, 1), "avg_launch_us": round(live["avg_us"], 2),
+                    "launches_timed": live["launches"]}
+    for table in (k_plain, k_lrn):
+        for name, s in (table or {}).items():
+            if name in BYTES:
+                s["gbs"] = round(n * BYTES[name](K) / (s["avg_us"] * 1e-6) / 1e9, 1)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
+        from splishsplash_amd import learned as _l
+        sim = _l.make_sim(sc)
+        sim.step(args.warmup)
         cpu = cpu_baseline(sim, sc, args.cpu_steps)
+        del sim
         log(f"bench: CPU baseline {cpu['value']:.4g} particle-steps/s")
 
     out = {
         "metric": "DFSPH particle-steps/s at 1M particles, 0.01% density err; learned vs plain",
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -193,15 +241,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": "dam break, 1e6 particles (100^3 lattice, r=0.025) in [0,16]x[0,6]x[0,6], "
                                "Bender2019 volume map 0.05 m cells, Weiler2018 nu=0.01 tol 1e-4, XSPH 0.01, "
-                               "CFL 0.4, maxError 0.01%, maxErrorV 0.1%; plain warm start",
-                   "particles": n, "parallelism": "single GPU"},
+                               "CFL 0.4, maxError 0.01%, maxErrorV 0.1%; value = learned warm start",
+                   "particles": n, "parallelism": "single GPU", "tolerance_kg_m3": eta},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "solver": {"pressure_iterations_per_step": it_p, "divergence_iterations_per_step": it_v,
-                   "cg_iterations_per_step": it_cg, "mean_fluid_neighbors": K,
-                   "avg_density_err": c1["avg_density_err"], "time_step_size": c1["time_step_size"],
-                   "sim_time": c1["time"], "flags": c1["flags"]},
+        "plain": plain,
+        "learned": lrn,
+        "learned_vs_plain": ({"speedup": lrn["value"] / plain["value"],
+                              "pressure_iteration_cut": 1.0 - lrn["pressure_iterations_per_step"]
+                              / max(plain["pressure_iterations_per_step"], 1e-30),
+                              "step_time_cut": 1.0 - lrn["ms_per_step"] / plain["ms_per_step"]} if lrn else None),
+        "mean_fluid_neighbors": K,
         "kernels": kernels,
+        "kernels_plain": k_plain if lrn else None,
     }
     print(json.dumps(out))
 

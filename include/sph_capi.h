@@ -184,7 +184,17 @@ typedef struct sph_dfsph_counters {
     int    learned_particles;   /* particles that kept the learned start in
                                    the last pressure solve                  */
     int    learned_particles_v; /* same, last divergence solve              */
-    int    reserved[5];
+    int    capped_solves;    /* pressure solves that stopped at max_iterations
+                                without reaching the tolerance (cumulative)  */
+    int    capped_solves_v;  /* same, divergence solves                     */
+    int    reserved0;
+    /* Excess of the last iteration: rho0 * mean over particles that keep
+     * pressure of the positive residual (s - A p)_i, i.e. expansion the
+     * pressure overshoots. The reference's error (above) counts only
+     * remaining compression; a solve started from the learned network must
+     * bring both below the tolerance. Same units as the errors above. */
+    double avg_overpressure_err;
+    double avg_overpressure_err_v;
 } sph_dfsph_counters_t;
 
 /* Sub-steps of TimeStepDFSPH::step() (TimeStepDFSPH.cpp:117-249), callable
@@ -273,6 +283,10 @@ int sph_dfsph_set_time(sph_dfsph_t s, double time, double time_step_size);
  * enabled, sph_dfsph_kernel_stats reports, for kernel k < *count, its name,
  * launch count and summed milliseconds. */
 int sph_dfsph_enable_timing(sph_dfsph_t s, int enable);
+/* Restricts timing to the kernels named in the comma-separated list (null
+ * or empty: all), so that a timed run records two events per launch of
+ * those kernels only. */
+int sph_dfsph_time_only(sph_dfsph_t s, const char *names);
 int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name,
                            long long *launches, double *total_ms,
                            int *count);

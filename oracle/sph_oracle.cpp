@@ -881,19 +881,25 @@ double Sim::divergence_iteration() {
     const double h = dt, density0 = P.density0;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) pressure_accel(pv, i, &ap[3 * i]);
-    std::vector<double> err(n);
+    std::vector<double> err(n), over(n);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
         double aij = aij_pj(i);
         aij *= h;
         const double s_i = -rho_adv[i];
         double residuum = std::min(s_i - aij, 0.0);
-        if (num_neighbors(*this, i) < 20) residuum = 0.0;
+        const bool few = num_neighbors(*this, i) < 20;
+        if (few) residuum = 0.0;
         pv[i] = std::max(pv[i] - 0.5 * (s_i - aij) * factor[i], 0.0);
         err[i] = -density0 * residuum;
+        over[i] = (pv[i] > 0.0 && s_i - aij > 0.0 && !few) ? density0 * (s_i - aij) : 0.0;
     }
-    double e = 0.0;
-    for (size_t i = 0; i < n; i++) e += err[i];
+    double e = 0.0, eo = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        e += err[i];
+        eo += over[i];
+    }
+    lastOverV = n ? eo / (double)n : 0.0;
     return n ? e / (double)n : 0.0;
 }
 
@@ -1226,7 +1232,7 @@ double Sim::pressure_iteration() {
     const double h = dt, density0 = P.density0;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) pressure_accel(p, i, &ap[3 * i]);
-    std::vector<double> err(n);
+    std::vector<double> err(n), over(n);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
         double aij = aij_pj(i);
@@ -1235,9 +1241,14 @@ double Sim::pressure_iteration() {
         const double residuum = std::min(s_i - aij, 0.0);
         p[i] = std::max(p[i] - 0.5 * (s_i - aij) * factor[i], 0.0);
         err[i] = -density0 * residuum;
+        over[i] = (p[i] > 0.0 && s_i - aij > 0.0) ? density0 * (s_i - aij) : 0.0;
     }
-    double e = 0.0;
-    for (size_t i = 0; i < n; i++) e += err[i];
+    double e = 0.0, eo = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        e += err[i];
+        eo += over[i];
+    }
+    lastOver = n ? eo / (double)n : 0.0;
     return n ? e / (double)n : 0.0;
 }
 
@@ -1484,6 +1495,8 @@ int orc_get_counters(void *h, sph_dfsph_counters_t *c) {
     c->time_step_size = s->dt;
     c->avg_density_err = s->lastErr;
     c->avg_divergence_err = s->lastErrV;
+    c->avg_overpressure_err = s->lastOver;
+    c->avg_overpressure_err_v = s->lastOverV;
     c->iterations = s->iterations;
     c->iterations_v = s->iterationsV;
     c->visco_iterations = s->viscoIterations;

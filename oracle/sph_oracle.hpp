@@ -126,6 +126,7 @@ public:
     int iterations = 0, iterationsV = 0, viscoIterations = 0, steps = 0;
     long long totIt = 0, totItV = 0, totVisco = 0;
     double lastErr = 0, lastErrV = 0;
+    double lastOver = 0, lastOverV = 0;  // excess: positive residual where p > 0 (not a reference quantity)
     bool initialized = false;
 
     void set_fluid(const double *x, const double *v, size_t n);

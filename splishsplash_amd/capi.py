@@ -82,6 +82,7 @@ _SIGS = {
     "sph_dfsph_get_training": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_size_t, _szp]),
     "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_time_only": (C.c_int, [_vp, C.c_char_p]),
     "sph_dfsph_kernel_stats": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_longlong),
                                          _dp, C.POINTER(C.c_int)]),
     "sph_dfsph_reset_kernel_stats": (C.c_int, [_vp]),
@@ -436,6 +437,10 @@ class TimeStepDFSPH:
 
     def enable_timing(self, on=True):
         _chk(lib().sph_dfsph_enable_timing(self.h, int(on)))
+
+    def time_only(self, names):
+        """Restrict timing to these kernel names (None: all)."""
+        _chk(lib().sph_dfsph_time_only(self.h, ",".join(names).encode() if names else None))
 
     def reset_kernel_stats(self):
         _chk(lib().sph_dfsph_reset_kernel_stats(self.h))

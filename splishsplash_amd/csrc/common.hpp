@@ -159,6 +159,14 @@ public:
         double ms = 0.0;
     };
     bool enabled = false;
+    std::vector<std::string> only;  // if non-empty, time only these names
+    bool wants(const char *name) const {
+        if (!enabled) return false;
+        if (only.empty()) return true;
+        for (const auto &n : only)
+            if (n == name) return true;
+        return false;
+    }
     int begin(hipStream_t s, const char *name);
     void end(hipStream_t s, int slot);
     void collect();  // synchronises pending events into the stats
@@ -181,7 +189,8 @@ struct TimedScope {
     KernelTimer &t;
     hipStream_t s;
     int slot;
-    TimedScope(KernelTimer &tt, hipStream_t ss, const char *name) : t(tt), s(ss), slot(tt.enabled ? tt.begin(ss, name) : -1) {}
+    TimedScope(KernelTimer &tt, hipStream_t ss, const char *name)
+        : t(tt), s(ss), slot(tt.wants(name) ? tt.begin(ss, name) : -1) {}
     ~TimedScope() {
         if (slot >= 0) t.end(s, slot);
     }

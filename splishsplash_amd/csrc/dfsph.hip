@@ -232,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
     if (check_done && (PRESSURE ? C->done : C->done_v)) return;
     const unsigned i = gtid();
     const double h = C->h;
-    double err = 0.0;
+    double err = 0.0, over = 0.0;
     if (i < D.n) {
         const float4 xi = D.x[i];
         const float4 ai = D.ap[i];
@@ -279,19 +279,27 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
             const float aij = s * (float)(h * h);
             const float si = 1.0f - D.radv[i];
             const float res = fminf(si - aij, 0.0f);
-            D.p[i] = fmaxf(D.p[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            const float pn = fmaxf(D.p[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            D.p[i] = pn;
             err = -(double)D.rho0 * (double)res;
+            if (pn > 0.0f && si - aij > 0.0f) over = (double)D.rho0 * (double)(si - aij);
         } else {
             const float aij = s * (float)h;
             const float si = -D.radv[i];
             float res = fminf(si - aij, 0.0f);
             if (nnb < 20u) res = 0.0f;
-            D.pv[i] = fmaxf(D.pv[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            const float pn = fmaxf(D.pv[i] - 0.5f * (si - aij) * alpha, 0.0f);
+            D.pv[i] = pn;
             err = -(double)D.rho0 * (double)res;
+            if (pn > 0.0f && si - aij > 0.0f && nnb >= 20u) over = (double)D.rho0 * (double)(si - aij);
         }
     }
     const double bs = block_sum(err, lds);
-    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+    const double bo = block_sum(over, lds);
+    if (threadIdx.x == 0) {
+        D.partial[2 * blockIdx.x] = bs;
+        D.partial[2 * blockIdx.x + 1] = bo;
+    }
 }
 
 // Sums the per-block error partials of k_jacobi in a fixed order (one block)
@@ -302,25 +310,36 @@ __global__ __launch_bounds__(RBLOCK) void k_jacobi_reduce(Dev D, unsigned nparts
     __shared__ double lds[RBLOCK / 64];
     Ctl *C = D.ctl;
     if (check_done && (PRESSURE ? C->done : C->done_v)) return;
-    double t = 0.0;
-    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t += D.partial[b];
+    double t = 0.0, u = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
+        t += D.partial[2 * b];
+        u += D.partial[2 * b + 1];
+    }
     t = block_sum(t, lds);
+    u = block_sum(u, lds);
     if (threadIdx.x == 0) {
         const double h = C->h;
         const double avg = D.n ? t / (double)D.n : 0.0;
-        const bool finite = isfinite(avg);
+        const double avo = D.n ? u / (double)D.n : 0.0;
+        const bool finite = isfinite(avg) && isfinite(avo);
         if (!finite) C->flags |= 1;
+        // The reference's criterion counts remaining compression only. A
+        // learned start can also overshoot, which that criterion never
+        // sees, so a solve started from it must also bring the excess
+        // (positive residual at particles that keep pressure) below eta.
         if (PRESSURE) {
             C->avg_err = avg;
+            C->avg_over = avo;
             C->it += 1;
             const double eta = D.maxError * 0.01 * (double)D.rho0;
-            const bool chk = avg <= eta;
+            const bool chk = avg <= eta && (!D.strictP || avo <= eta);
             C->done = ((chk && C->it >= D.minIt) || C->it >= D.maxIt || !finite) ? 1 : 0;
         } else {
             C->avg_err_v = avg;
+            C->avg_over_v = avo;
             C->it_v += 1;
             const double eta = (1.0 / h) * D.maxErrorV * 0.01 * (double)D.rho0;
-            const bool chk = avg <= eta;
+            const bool chk = avg <= eta && (!D.strictV || avo <= eta);
             C->done_v = ((chk && C->it_v >= 1) || C->it_v >= D.maxItV || !finite) ? 1 : 0;
         }
     }
@@ -1036,7 +1055,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
     Ctl *C = D.ctl;
     const unsigned i = gtid();
     const double h = C->h;
-    double err = 0.0;
+    double err = 0.0, over = 0.0;
     bool took = false;
     if (i < D.n) {
         const float4 xi = D.x[i], ai = D.ap[i], bi = D.ap2[i];
@@ -1086,11 +1105,16 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
         else
             D.pv[i] = pn;
         err = -(double)D.rho0 * (double)res;
+        if (pn > 0.0f && rr > 0.0f && (PRESSURE || nnb >= 20u)) over = (double)D.rho0 * (double)rr;
     }
     const unsigned long long m = __ballot(took);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(PRESSURE ? &C->nlearn : &C->nlearn_v, (unsigned)__popcll(m));
     const double bs = block_sum(err, lds);
-    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+    const double bo = block_sum(over, lds);
+    if (threadIdx.x == 0) {
+        D.partial[2 * blockIdx.x] = bs;
+        D.partial[2 * blockIdx.x + 1] = bo;
+    }
 }
 
 // Training samples: features of every stride-th particle and its converged
@@ -1241,6 +1265,8 @@ Dev DFSPH::dev() const {
     D.hs2 = (float)(hs * hs);
     D.viscoMaxIt = P.visco_max_iter;
     D.viscoTol = P.visco_max_error;
+    D.strictP = learned_[0] ? 1 : 0;
+    D.strictV = learned_[1] ? 1 : 0;
     D.feats = feats_[0].get();
     D.feats_v = feats_[1].get();
     D.pred = pred_.get();
@@ -1666,7 +1692,17 @@ int DFSPH::run_solver(bool pressure, int already) {
         if (done || launched >= maxIt) break;
         chunk = 1;
     }
-    return pressure ? hctl_->it : hctl_->it_v;
+    const int it = pressure ? hctl_->it : hctl_->it_v;
+    if (it >= maxIt) {
+        // stopped by the cap: count it unless the tolerance was met anyway
+        const double eta = pressure ? P.max_error * 0.01 * P.density0
+                                    : (1.0 / hctl_->h) * P.max_error_v * 0.01 * P.density0;
+        const bool strict = learned_[pressure ? 0 : 1];
+        const double err = pressure ? hctl_->avg_err : hctl_->avg_err_v;
+        const double over = pressure ? hctl_->avg_over : hctl_->avg_over_v;
+        if (!(err <= eta) || (strict && !(over <= eta))) (pressure ? capped_ : cappedV_)++;
+    }
+    return it;
 }
 
 int DFSPH::run_cg() {
@@ -1977,6 +2013,10 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->flags = hctl_->flags;
     c->learned_particles = learned_[0] ? (int)hctl_->nlearn : 0;
     c->learned_particles_v = learned_[1] ? (int)hctl_->nlearn_v : 0;
+    c->capped_solves = capped_;
+    c->capped_solves_v = cappedV_;
+    c->avg_overpressure_err = hctl_->avg_over;
+    c->avg_overpressure_err_v = hctl_->avg_over_v;
 }
 
 void DFSPH::set_time(double t, double h) {

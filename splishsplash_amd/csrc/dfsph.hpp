@@ -23,6 +23,7 @@ struct Ctl {
     double time;
     double avg_err;    // last pressure iteration, average density error
     double avg_err_v;  // last divergence iteration
+    double avg_over, avg_over_v;  // excess (positive residual where p > 0), same units
     double cg_rhs2, cg_thresh, cg_absNew, cg_alpha, cg_beta, cg_r2;
     int it, done;      // pressure solver
     int it_v, done_v;  // divergence solver
@@ -63,6 +64,7 @@ struct Dev {
     int viscoMaxIt;
     double viscoTol;
     // learned warm start
+    int strictP, strictV;    // solves started from the network: also bound the excess
     float *feats, *feats_v;  // 8 floats per particle
     float *pred, *kprev;     // network output, previous-step stiffness
     float4 *ap2;             // pressure accelerations of the predicted start
@@ -128,6 +130,7 @@ private:
     long long totIt_ = 0, totItV_ = 0, totVisco_ = 0;
     int steps_ = 0, lastIt_ = 2, lastItV_ = 1, lastCg_ = 4;
     int itLast_ = 0, itVLast_ = 0, cgLast_ = 0;
+    int capped_ = 0, cappedV_ = 0;  // solves stopped at the iteration cap short of the tolerance
     // learned warm start
     bool learned_[2] = {false, false};
     unsigned stride_s_[2] = {0, 0};

@@ -196,6 +196,25 @@ int sph_dfsph_enable_timing(sph_dfsph_t s, int enable) {
     });
 }
 
+int sph_dfsph_time_only(sph_dfsph_t s, const char *names) {
+    return guarded([&] {
+        DCHK(s);
+        auto &only = s->s->timer.only;
+        only.clear();
+        if (!names) return;
+        std::string cur;
+        for (const char *c = names;; c++) {
+            if (*c == ',' || *c == 0) {
+                if (!cur.empty()) only.push_back(cur);
+                cur.clear();
+                if (*c == 0) break;
+            } else {
+                cur.push_back(*c);
+            }
+        }
+    });
+}
+
 int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name, long long *launches, double *total_ms,
                            int *count) {
     return guarded([&] {

@@ -69,7 +69,11 @@ class DFSPHCounters(C.Structure):
         ("flags", C.c_int),
         ("learned_particles", C.c_int),
         ("learned_particles_v", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("capped_solves", C.c_int),
+        ("capped_solves_v", C.c_int),
+        ("reserved", C.c_int),
+        ("avg_overpressure_err", C.c_double),
+        ("avg_overpressure_err_v", C.c_double),
     ]
 
     def as_dict(self):

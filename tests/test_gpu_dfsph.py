@@ -89,6 +89,9 @@ def run_stage_sequence(g, o, seq, extra_state=()):
             k = "avg_divergence_err" if st == "divergence_iter" else "avg_density_err"
             eg, eo = g.counters()[k], o.counters()[k]
             assert eg == pytest.approx(eo, rel=1e-4, abs=1e-9 * max(1.0, abs(eo)))
+            k = "avg_overpressure_err" + ("_v" if st == "divergence_iter" else "")
+            eg, eo = g.counters()[k], o.counters()[k]
+            assert eg == pytest.approx(eo, rel=1e-3, abs=1e-7 * max(1.0, abs(eo)))
     return report
 
 

@@ -1,0 +1,95 @@
+"""Learned warm start on the device: sample recording, the residual-picked
+first iteration, convergence to the configured tolerance, and the gain of a
+network trained on the solver's own states over the reference's start."""
+import numpy as np
+import pytest
+
+from splishsplash_amd import capi, learned, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_variant(seed, per_side=24, **kw):
+    return scenes.dam_break_variant(seed, per_side=per_side, n_spheres=kw.pop("n_spheres", 0), **kw)
+
+
+def test_sample_recording_counts_and_ranges():
+    sc = small_variant(1)
+    n = len(sc["x"])
+    stride = 5
+    sim = learned.make_sim(sc)
+    # past the start, where the lattice (rho = 0.8 rho0) is under-dense and
+    # no stiffness is stored yet
+    sim.step(60)
+    assert sim.get("p / rho^2").max() > 0
+    sim.collect_training("pressure", stride, 1 << 20)
+    sim.collect_training("divergence", stride, 1 << 20)
+    sim.step(6)
+    for s in learned.SOLVERS:
+        x, t = sim.get_training(s)
+        # one sample per stride-th particle and step (the offset rotates)
+        assert 6 * (n // stride) <= len(x) <= 6 * (n // stride + 1), (s, len(x))
+        assert np.isfinite(x).all() and np.isfinite(t).all()
+        assert (t >= 0).all()
+        assert ((x[:, 2] >= 0) & (x[:, 2] <= 1)).all()  # previous stiffness / step maximum
+        assert (x[:, 6:] == 0).all()
+        assert (x[:, 5] > 0).all()  # neighbour count / 40
+    # the record is cleared by reading it, and capacity bounds it
+    x, _ = sim.get_training("pressure")
+    assert len(x) == 0
+    sim.collect_training("pressure", 1, 100)
+    sim.step(1)
+    x, _ = sim.get_training("pressure")
+    assert len(x) == 100
+
+
+def test_untrained_network_still_converges():
+    # Any prediction only changes the start: the solve must reach the same
+    # tolerance on remaining compression and on overshoot, so even a random
+    # network leaves the motion as it was (particles whose predicted start
+    # is worse keep the reference's start).
+    sc = small_variant(2)
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    th = {s: learned.init_theta(7) for s in learned.SOLVERS}
+    sim = learned.make_sim(sc, th)
+    ref = learned.make_sim(sc)
+    for _ in range(40):
+        sim.step(1)
+        ref.step(1)
+        c = sim.counters()
+        assert (c["avg_density_err"] <= eta and c["avg_overpressure_err"] <= eta) or c["iterations"] == 100
+    assert c["flags"] == 0
+    assert np.isfinite(sim.get("position")).all()
+    # same physics within the solver tolerance: mean height of the fluid
+    yl, yr = sim.get("position")[:, 1].mean(), ref.get("position")[:, 1].mean()
+    assert abs(yl - yr) < 0.01 * yr
+
+
+def test_zero_network_equals_plain_start():
+    # At the first step every stored stiffness is 0, so the reference's start
+    # and the prediction are both 0 and the first iteration keeps the
+    # reference's start: the learned path must reproduce the plain one bit
+    # for bit (same kernels' arithmetic, same iteration count).
+    sc = small_variant(3)
+    th0 = {s: np.zeros(capi.MLP_NPARAM, np.float32) for s in learned.SOLVERS}
+    a = learned.make_sim(sc, th0)
+    b = learned.make_sim(sc)
+    a.step(1)
+    b.step(1)
+    for nm in ("position", "velocity", "p / rho^2"):
+        np.testing.assert_array_equal(a.get(nm), b.get(nm))
+    assert a.counters()["iterations"] == b.counters()["iterations"]
+
+
+def test_trained_network_cuts_pressure_iterations():
+    # Train on the solver states of three seeded variants, evaluate on a
+    # held-out one: fewer pressure iterations at the same tolerance.
+    data = [learned.collect(small_variant(s, per_side=28), steps=80, skip=30, stride=3) for s in (11, 12, 13)]
+    th, loss = learned.train(learned.merge(data), steps=1500, batch=8192, lr=2e-3)
+    assert all(np.isfinite(v) for v in loss.values())
+    sc = small_variant(21, per_side=32)
+    plain = learned.evaluate(sc, None, steps=60, warmup=30)
+    lrn = learned.evaluate(sc, th, steps=60, warmup=30)
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    assert lrn["avg_density_err"] <= eta and lrn["flags"] == 0
+    assert lrn["pressure_iterations"] < 0.9 * plain["pressure_iterations"], (plain, lrn)
