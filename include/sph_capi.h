@@ -187,6 +187,8 @@ typedef struct sph_dfsph_counters {
     int    capped_solves;    /* pressure solves that stopped at max_iterations
                                 without reaching the tolerance (cumulative)  */
     int    capped_solves_v;  /* same, divergence solves                     */
+    int    learned_solves;   /* pressure solves started from the network     */
+    int    learned_solves_v; /* divergence solves started from the network   */
     int    reserved0;
     /* Excess of the last iteration: rho0 * mean over particles that keep
      * pressure of the positive residual (s - A p)_i, i.e. expansion the
@@ -316,10 +318,21 @@ int sph_mlp_train(float *theta, const float *x, const float *t, size_t n,
  * :449-453). solver: 0 pressure solve, 1 divergence solve. With a model set,
  * the solve starts from the network's prediction wherever its initial
  * residual is smaller than that of the plain previous-step warm start, and
- * then iterates to the same tolerance. theta == NULL restores the plain
- * start. */
+ * then iterates to the same tolerance on the reference's error (remaining
+ * compression) and on the overshoot (counters avg_overpressure_err*), which
+ * the reference's conservative start keeps near zero by construction.
+ * theta == NULL restores the plain start. */
 int sph_dfsph_set_warm_start_model(sph_dfsph_t s, int solver,
                                    const float *theta);
+/* When to use the network (no reference counterpart): after a solve from
+ * the reference's start, the next solve of the same kind starts from the
+ * network if that solve took at least `min_prev_iterations` iterations
+ * (defaults: pressure 4, divergence 3); below that the reference's start is
+ * already near the solution and the network's extra sweeps cost more than
+ * they save. After a solve from the network, the network stays in use
+ * unless that solve ended at the minimum iteration count. 0 = always. */
+int sph_dfsph_set_warm_start_policy(sph_dfsph_t s, int solver,
+                                    int min_prev_iterations);
 /* Training-sample collection: after each solve, every `stride`-th particle
  * (rotating offset) contributes its 6 features (rows of 8 floats) and its
  * converged stiffness relative to the step's scale; up to `capacity`

@@ -37,6 +37,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         sigs = {
             "orc_last_error": (C.c_char_p, []),
+            "orc_sizeof_params": (C.c_size_t, []),
+            "orc_sizeof_counters": (C.c_size_t, []),
             "orc_create": (C.c_int, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
             "orc_destroy": (C.c_int, [C.c_void_p]),
             "orc_set_fluid": (C.c_int, [C.c_void_p, _dp, _dp, C.c_size_t]),

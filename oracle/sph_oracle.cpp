@@ -1399,6 +1399,10 @@ int vm_field(Sim *s, const char *name, size_t count, double *&out) {
 extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
+// Sizes of the shared parameter and counter structs as compiled here, so a
+// caller can check that its declarations match the header.
+size_t orc_sizeof_params(void) { return sizeof(sph_dfsph_params_t); }
+size_t orc_sizeof_counters(void) { return sizeof(sph_dfsph_counters_t); }
 
 int orc_create(const sph_dfsph_params_t *p, void **out) {
     if (!p || !out) return fail("null argument");

@@ -79,6 +79,7 @@ _SIGS = {
     "sph_dfsph_get_counters": (C.c_int, [_vp, C.POINTER(DFSPHCounters)]),
     "sph_dfsph_set_warm_start_model": (C.c_int, [_vp, C.c_int, _fp]),
     "sph_dfsph_collect_training": (C.c_int, [_vp, C.c_int, C.c_uint, C.c_size_t]),
+    "sph_dfsph_set_warm_start_policy": (C.c_int, [_vp, C.c_int, C.c_int]),
     "sph_dfsph_get_training": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_size_t, _szp]),
     "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
@@ -415,6 +416,11 @@ class TimeStepDFSPH:
         self._theta = getattr(self, "_theta", {})
         self._theta[solver] = th
         _chk(lib().sph_dfsph_set_warm_start_model(self.h, s, th.ctypes.data_as(_fp)))
+
+    def set_warm_start_policy(self, solver, min_prev_iterations):
+        """Use the network for `solver` only when its previous solve took at
+        least this many iterations (0: always)."""
+        _chk(lib().sph_dfsph_set_warm_start_policy(self.h, self._SOLVERS[solver], int(min_prev_iterations)))
 
     def collect_training(self, solver, stride, capacity):
         """Records (features, converged stiffness / scale) of every

@@ -305,19 +305,22 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
 // Sums the per-block error partials of k_jacobi in a fixed order (one block)
 // and decides convergence (TimeStepDFSPH.cpp:324-341, 477-495). The kernel
 // boundary makes every partial visible without device-scope fences.
-template <bool PRESSURE>
+template <bool PRESSURE, bool LEARNED = false>
 __global__ __launch_bounds__(RBLOCK) void k_jacobi_reduce(Dev D, unsigned nparts, int check_done) {
     __shared__ double lds[RBLOCK / 64];
     Ctl *C = D.ctl;
     if (check_done && (PRESSURE ? C->done : C->done_v)) return;
-    double t = 0.0, u = 0.0;
+    double t = 0.0, u = 0.0, l = 0.0;
     for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
         t += D.partial[2 * b];
         u += D.partial[2 * b + 1];
+        if (LEARNED) l += D.partial[2 * nparts + b];
     }
     t = block_sum(t, lds);
     u = block_sum(u, lds);
+    if (LEARNED) l = block_sum(l, lds);
     if (threadIdx.x == 0) {
+        if (LEARNED) (PRESSURE ? C->nlearn : C->nlearn_v) = (unsigned)l;
         const double h = C->h;
         const double avg = D.n ? t / (double)D.n : 0.0;
         const double avo = D.n ? u / (double)D.n : 0.0;
@@ -941,12 +944,28 @@ __global__ __launch_bounds__(BLOCK) void k_features(Dev D) {
     const float w0 = D.V * D.K.W0;
     float wsum = w0, ssrc = w0 * src_i, sk = w0 * D.kprev[i];
     const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float w = D.V * kernel_W(xi, D.x[j], D.K);
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    auto add = [&](const float4 &xj, float rj, float kj) {
+        const float w = D.V * kernel_W(xi, xj, D.K);
         wsum += w;
-        ssrc += w * ((D.radv[j] - off) * sh);
-        sk += w * D.kprev[j];
+        ssrc += w * ((rj - off) * sh);
+        sk += w * kj;
+    };
+    unsigned k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+        const float r0 = D.radv[j0], r1 = D.radv[j1], r2 = D.radv[j2], r3 = D.radv[j3];
+        const float k0 = D.kprev[j0], k1 = D.kprev[j1], k2 = D.kprev[j2], k3 = D.kprev[j3];
+        add(x0, r0, k0);
+        add(x1, r1, k1);
+        add(x2, r2, k2);
+        add(x3, r3, k3);
+    }
+    for (; k < cnt; k++) {
+        const unsigned j = L[k * S];
+        add(D.x[j], D.radv[j], D.kprev[j]);
     }
     float bvol = 0.f;
     unsigned nnb = cnt;
@@ -995,24 +1014,41 @@ __global__ __launch_bounds__(BLOCK) void k_accel2(Dev D) {
     const float pi = P[i], qi = Q[i];
     float ax = 0.f, ay = 0.f, az = 0.f, bxx = 0.f, byy = 0.f, bzz = 0.f;
     const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float ps = pi + P[j], qs = qi + Q[j];
-        if (fabsf(ps) > D.eps || fabsf(qs) > D.eps) {
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    auto pair = [&](const float4 &xj, float pj, float qj) {
+        const float ps = pi + pj, qs = qi + qj;
+        const bool a = fabsf(ps) > D.eps, b = fabsf(qs) > D.eps;
+        if (a || b) {
             float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float g = kernel_G(xi, xj, D.K, r);
             const float gx = -D.V * (g * r.x), gy = -D.V * (g * r.y), gz = -D.V * (g * r.z);
-            if (fabsf(ps) > D.eps) {
+            if (a) {
                 ax += ps * gx;
                 ay += ps * gy;
                 az += ps * gz;
             }
-            if (fabsf(qs) > D.eps) {
+            if (b) {
                 bxx += qs * gx;
                 byy += qs * gy;
                 bzz += qs * gz;
             }
         }
+    };
+    unsigned k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+        const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
+        const float q0 = Q[j0], q1 = Q[j1], q2 = Q[j2], q3 = Q[j3];
+        pair(x0, p0, q0);
+        pair(x1, p1, q1);
+        pair(x2, p2, q2);
+        pair(x3, p3, q3);
+    }
+    for (; k < cnt; k++) {
+        const unsigned j = L[k * S];
+        pair(D.x[j], P[j], Q[j]);
     }
     const bool bp = fabsf(pi) > D.eps, bq = fabsf(qi) > D.eps;
     if (bp || bq) {
@@ -1061,14 +1097,29 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
         const float4 xi = D.x[i], ai = D.ap[i], bi = D.ap2[i];
         float s1 = 0.f, s2 = 0.f;
         const unsigned cnt = D.cf[i];
-        for (unsigned k = 0; k < cnt; k++) {
-            const unsigned j = D.nf[(size_t)k * D.stride + i];
-            const float4 aj = D.ap[j], bj = D.ap2[j];
+        const unsigned *__restrict__ L = D.nf + i;
+        const size_t S = D.stride;
+        auto pair = [&](const float4 &xj, const float4 &aj, const float4 &bj) {
             float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float g = kernel_G(xi, xj, D.K, r);
             const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
             s1 += ((ai.x - aj.x) * gx + (ai.y - aj.y) * gy) + (ai.z - aj.z) * gz;
             s2 += ((bi.x - bj.x) * gx + (bi.y - bj.y) * gy) + (bi.z - bj.z) * gz;
+        };
+        unsigned k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+            const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+            const float4 a0 = D.ap[j0], a1 = D.ap[j1], a2 = D.ap[j2], a3 = D.ap[j3];
+            const float4 b0 = D.ap2[j0], b1 = D.ap2[j1], b2 = D.ap2[j2], b3 = D.ap2[j3];
+            pair(x0, a0, b0);
+            pair(x1, a1, b1);
+            pair(x2, a2, b2);
+            pair(x3, a3, b3);
+        }
+        for (; k < cnt; k++) {
+            const unsigned j = L[k * S];
+            pair(D.x[j], D.ap[j], D.ap2[j]);
         }
         s1 *= D.V;
         s2 *= D.V;
@@ -1107,13 +1158,13 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
         err = -(double)D.rho0 * (double)res;
         if (pn > 0.0f && rr > 0.0f && (PRESSURE || nnb >= 20u)) over = (double)D.rho0 * (double)rr;
     }
-    const unsigned long long m = __ballot(took);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(PRESSURE ? &C->nlearn : &C->nlearn_v, (unsigned)__popcll(m));
     const double bs = block_sum(err, lds);
     const double bo = block_sum(over, lds);
+    const double bl = block_sum(took ? 1.0 : 0.0, lds);
     if (threadIdx.x == 0) {
         D.partial[2 * blockIdx.x] = bs;
         D.partial[2 * blockIdx.x + 1] = bo;
+        D.partial[2 * gridDim.x + blockIdx.x] = bl;  // particles that kept the network's start
     }
 }
 
@@ -1265,8 +1316,8 @@ Dev DFSPH::dev() const {
     D.hs2 = (float)(hs * hs);
     D.viscoMaxIt = P.visco_max_iter;
     D.viscoTol = P.visco_max_error;
-    D.strictP = learned_[0] ? 1 : 0;
-    D.strictV = learned_[1] ? 1 : 0;
+    D.strictP = engaged_[0] ? 1 : 0;
+    D.strictV = engaged_[1] ? 1 : 0;
     D.feats = feats_[0].get();
     D.feats_v = feats_[1].get();
     D.pred = pred_.get();
@@ -1529,9 +1580,21 @@ void DFSPH::pressure_setup() {
 
 // Scale, features and (with a model) the network's start plus the first
 // iteration from the per-particle better start. Returns iterations done.
-void DFSPH::learned_start(bool pressure) {
+bool DFSPH::learned_start(bool pressure) {
     const int solver = pressure ? 0 : 1;
-    if (!learn_or_collect(solver)) return;
+    const bool was = engaged_[solver];
+    engaged_[solver] = false;
+    if (!learn_or_collect(solver)) return false;
+    // The network pays only when the reference's start is far from the
+    // solution. After a solve from the reference's start, engage it if that
+    // solve took at least engage_min_ iterations; after a solve from the
+    // network, keep it unless that solve ended at the minimum count (then
+    // the next step tries the reference's start again).
+    const int prev = pressure ? itLast_ : itVLast_;
+    const int floor = pressure ? std::max(P.min_iterations, 1) : 1;
+    const bool use = learned_[solver] &&
+                     (engage_min_[solver] == 0 || (was ? prev > floor : prev >= engage_min_[solver]));
+    if (!use && stride_s_[solver] == 0) return false;
     Dev D = dev();
     D.kprev = kprev_.get();
     LAUNCH("learn_scale", k_max_partial, D, kprev_.get());
@@ -1543,7 +1606,11 @@ void DFSPH::learned_start(bool pressure) {
         LAUNCH("learn_features", k_features<true>, D);
     else
         LAUNCH("learn_features", k_features<false>, D);
-    if (!learned_[solver]) return;
+    if (!use) return false;
+    engaged_[solver] = true;
+    (pressure ? learnedSolves_ : learnedSolvesV_)++;
+    D.strictP = engaged_[0];
+    D.strictV = engaged_[1];
     {
         TimedScope ts(timer, stream, "learn_mlp");
         mlp_forward(stream, theta_[solver].get(), feats_[solver].get(), (unsigned)n_, pred_.get());
@@ -1552,13 +1619,14 @@ void DFSPH::learned_start(bool pressure) {
         LAUNCH("learn_start", k_pred_start<true>, D);
         LAUNCH("learn_accel2", k_accel2<true>, D);
         LAUNCH("learn_jacobi2", k_jacobi2<true>, D);
-        REDUCE("jacobi_reduce", k_jacobi_reduce<true>, D, nblocks(n_), 0);
+        REDUCE("jacobi_reduce", (k_jacobi_reduce<true, true>), D, nblocks(n_), 0);
     } else {
         LAUNCH("learn_start", k_pred_start<false>, D);
         LAUNCH("learn_accel2", k_accel2<false>, D);
         LAUNCH("learn_jacobi2", k_jacobi2<false>, D);
-        REDUCE("jacobi_reduce", k_jacobi_reduce<false>, D, nblocks(n_), 0);
+        REDUCE("jacobi_reduce", (k_jacobi_reduce<false, true>), D, nblocks(n_), 0);
     }
+    return true;
 }
 
 void DFSPH::collect(bool pressure) {
@@ -1581,6 +1649,12 @@ void DFSPH::set_warm_start_model(int solver, const float *theta) {
     SPH_HIP(hipMemcpy(theta_[solver].get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
     learned_[solver] = true;
     ensure_learn_buffers();
+}
+
+void DFSPH::set_warm_start_policy(int solver, int min_prev_iterations) {
+    SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
+    SPH_REQUIRE(min_prev_iterations >= 0, SPH_ERR_INVALID, "min_prev_iterations must be >= 0");
+    engage_min_[solver] = min_prev_iterations;
 }
 
 void DFSPH::collect_training(int solver, unsigned stride, size_t capacity) {
@@ -1697,7 +1771,7 @@ int DFSPH::run_solver(bool pressure, int already) {
         // stopped by the cap: count it unless the tolerance was met anyway
         const double eta = pressure ? P.max_error * 0.01 * P.density0
                                     : (1.0 / hctl_->h) * P.max_error_v * 0.01 * P.density0;
-        const bool strict = learned_[pressure ? 0 : 1];
+        const bool strict = engaged_[pressure ? 0 : 1];
         const double err = pressure ? hctl_->avg_err : hctl_->avg_err_v;
         const double over = pressure ? hctl_->avg_over : hctl_->avg_over_v;
         if (!(err <= eta) || (strict && !(over <= eta))) (pressure ? capped_ : cappedV_)++;
@@ -1743,8 +1817,8 @@ void DFSPH::step() {
     if (div) k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
     sweep_init(true, true, div);
     if (div) {
-        learned_start(false);
-        itVLast_ = run_solver(false, learned_[1] ? 1 : 0);
+        const bool lv = learned_start(false);
+        itVLast_ = run_solver(false, lv ? 1 : 0);
         lastItV_ = std::max(1, itVLast_);
         totItV_ += itVLast_;
         div_finish(false);
@@ -1758,8 +1832,8 @@ void DFSPH::step() {
     if (div) collect(false);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
     pressure_setup();
-    learned_start(true);
-    itLast_ = run_solver(true, learned_[0] ? 1 : 0);
+    const bool lp = learned_start(true);
+    itLast_ = run_solver(true, lp ? 1 : 0);
     lastIt_ = std::max(P.min_iterations, itLast_);
     totIt_ += itLast_;
     pressure_finish(true);
@@ -1785,8 +1859,7 @@ void DFSPH::stage(int s) {
         case SPH_STAGE_DIVERGENCE_SOLVE:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
             sweep_init(false, false, true);
-            learned_start(false);
-            itVLast_ = run_solver(false, learned_[1] ? 1 : 0);
+            itVLast_ = run_solver(false, learned_start(false) ? 1 : 0);
             totItV_ += itVLast_;
             div_finish(true);
             break;
@@ -1811,8 +1884,7 @@ void DFSPH::stage(int s) {
         case SPH_STAGE_PRESSURE_SOLVE:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
             pressure_setup();
-            learned_start(true);
-            itLast_ = run_solver(true, learned_[0] ? 1 : 0);
+            itLast_ = run_solver(true, learned_start(true) ? 1 : 0);
             totIt_ += itLast_;
             pressure_finish(false);
             finalize(true, false);
@@ -2011,8 +2083,10 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->max_fluid_neighbors = (int)hctl_->maxcnt_f;
     c->max_boundary_neighbors = (int)hctl_->maxcnt_b;
     c->flags = hctl_->flags;
-    c->learned_particles = learned_[0] ? (int)hctl_->nlearn : 0;
-    c->learned_particles_v = learned_[1] ? (int)hctl_->nlearn_v : 0;
+    c->learned_particles = engaged_[0] ? (int)hctl_->nlearn : 0;
+    c->learned_particles_v = engaged_[1] ? (int)hctl_->nlearn_v : 0;
+    c->learned_solves = learnedSolves_;
+    c->learned_solves_v = learnedSolvesV_;
     c->capped_solves = capped_;
     c->capped_solves_v = cappedV_;
     c->avg_overpressure_err = hctl_->avg_over;

@@ -96,6 +96,7 @@ public:
     void set_field(const std::string &name, const void *host, size_t bytes);
     void get_neighbors(int boundary, unsigned *offsets, unsigned *indices, size_t capacity, size_t *nnz);
     void set_warm_start_model(int solver, const float *theta);
+    void set_warm_start_policy(int solver, int min_prev_iterations);
     void collect_training(int solver, unsigned stride, size_t capacity);
     void get_training(int solver, float *x, float *t, size_t capacity, size_t *n);
     void counters(sph_dfsph_counters_t *c);
@@ -139,7 +140,10 @@ private:
     DBuf<float4> ap2_;
     unsigned soff_ = 0;
     bool learn_or_collect(int solver) const { return learned_[solver] || stride_s_[solver] > 0; }
-    void learned_start(bool pressure);
+    bool learned_start(bool pressure);  // true if this solve starts from the network
+    bool engaged_[2] = {false, false};   // the current/last solve of each kind started from the network
+    int engage_min_[2] = {4, 3};         // engage when the previous solve took at least this many iterations
+    int learnedSolves_ = 0, learnedSolvesV_ = 0;
     void collect(bool pressure);
     void ensure_learn_buffers();
 

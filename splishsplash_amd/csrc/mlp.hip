@@ -30,9 +30,13 @@ __device__ __forceinline__ short f2bf(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
     return (short)(u >> 16);
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU z * sigmoid(z) with the hardware reciprocal (1 ulp): the activations
+// are rounded to BF16 (8 bits) right after, so a correctly rounded division
+// would buy nothing.
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {

@@ -146,6 +146,13 @@ def evaluate(scene, thetas, steps, warmup=0):
            "cg_iterations": (c1["total_visco_iterations"] - c0["total_visco_iterations"]) / steps,
            "ms_per_step": 1e3 * dt / steps, "sim_time": c1["time"] - c0["time"],
            "avg_density_err": c1["avg_density_err"], "flags": c1["flags"],
-           "learned_particles": c1["learned_particles"], "learned_particles_v": c1["learned_particles_v"]}
+           "learned_particles": c1["learned_particles"], "learned_particles_v": c1["learned_particles_v"],
+           "learned_solves": c1["learned_solves"] - c0["learned_solves"],
+           "avg_overpressure_err": c1["avg_overpressure_err"]}
+    # state at the end, to compare the motion of runs
+    v = sim.get("velocity")
+    speed2 = (v.astype(np.float64) ** 2).sum(1)
+    res.update({"mean_height": float(sim.get("position")[:, 1].mean()), "kinetic_energy": float(0.5 * speed2.mean()),
+                "max_speed": float(np.sqrt(speed2.max()))})
     del sim
     return res

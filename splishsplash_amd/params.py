@@ -71,6 +71,8 @@ class DFSPHCounters(C.Structure):
         ("learned_particles_v", C.c_int),
         ("capped_solves", C.c_int),
         ("capped_solves_v", C.c_int),
+        ("learned_solves", C.c_int),
+        ("learned_solves_v", C.c_int),
         ("reserved", C.c_int),
         ("avg_overpressure_err", C.c_double),
         ("avg_overpressure_err_v", C.c_double),

@@ -51,3 +51,30 @@ def test_params_struct_matches_c_defaults():
             assert list(a) == list(b)
         else:
             assert a == b, name
+
+
+def test_struct_layouts_agree_with_the_header():
+    # The Python declarations and the oracle (compiled against the header)
+    # must agree on the shared structs; a stale build or a field added on one
+    # side only shows up here.
+    from oracle import oracle
+    from splishsplash_amd.params import DFSPHCounters, DFSPHParams
+    L = oracle.lib()
+    assert L.orc_sizeof_params() == ctypes.sizeof(DFSPHParams)
+    assert L.orc_sizeof_counters() == ctypes.sizeof(DFSPHCounters)
+    # a C compile of the header gives the same sizes and offsets
+    src = os.path.join(ROOT, "include", "sph_capi.h")
+    fields = [n for n, _ in DFSPHCounters._fields_ if n != "reserved"]
+    code = "#include <stdio.h>\n#include <stddef.h>\n#include \"%s\"\nint main(void){printf(\"%%zu %%zu\", " \
+           "sizeof(sph_dfsph_params_t), sizeof(sph_dfsph_counters_t));" % src
+    code += "".join('printf(" %%zu", offsetof(sph_dfsph_counters_t, %s));' % f for f in fields) + "return 0;}\n"
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(code)
+        subprocess.run(["gcc", "-o", exe, c], check=True)
+        out = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(DFSPHParams) and out[1] == ctypes.sizeof(DFSPHCounters)
+    assert out[2:] == [getattr(DFSPHCounters, f).offset for f in fields]

@@ -52,6 +52,8 @@ def test_untrained_network_still_converges():
     eta = sc["params"].max_error * 0.01 * sc["params"].density0
     th = {s: learned.init_theta(7) for s in learned.SOLVERS}
     sim = learned.make_sim(sc, th)
+    for s in learned.SOLVERS:
+        sim.set_warm_start_policy(s, 0)  # use the network in every solve
     ref = learned.make_sim(sc)
     for _ in range(40):
         sim.step(1)
@@ -73,9 +75,12 @@ def test_zero_network_equals_plain_start():
     sc = small_variant(3)
     th0 = {s: np.zeros(capi.MLP_NPARAM, np.float32) for s in learned.SOLVERS}
     a = learned.make_sim(sc, th0)
+    for s in learned.SOLVERS:
+        a.set_warm_start_policy(s, 0)
     b = learned.make_sim(sc)
     a.step(1)
     b.step(1)
+    assert a.counters()["learned_solves"] == 1
     for nm in ("position", "velocity", "p / rho^2"):
         np.testing.assert_array_equal(a.get(nm), b.get(nm))
     assert a.counters()["iterations"] == b.counters()["iterations"]
@@ -93,3 +98,35 @@ def test_trained_network_cuts_pressure_iterations():
     eta = sc["params"].max_error * 0.01 * sc["params"].density0
     assert lrn["avg_density_err"] <= eta and lrn["flags"] == 0
     assert lrn["pressure_iterations"] < 0.9 * plain["pressure_iterations"], (plain, lrn)
+
+
+def test_engagement_policy():
+    # With the default policy the network is used only after a solve that
+    # needed at least 4 (pressure) / 3 (divergence) iterations; a threshold
+    # above the iteration cap never engages it.
+    sc = small_variant(4)
+    th = learned.load()
+    a = learned.make_sim(sc, th)
+    a.set_warm_start_policy("pressure", 1000)
+    a.set_warm_start_policy("divergence", 1000)
+    a.step(30)
+    c = a.counters()
+    assert c["learned_solves"] == 0 and c["learned_solves_v"] == 0
+    b = learned.make_sim(sc, th)
+    b.set_warm_start_policy("pressure", 0)
+    b.step(30)
+    assert b.counters()["learned_solves"] == 30
+
+
+def test_packaged_networks_gain_on_heldout_variant():
+    # The packaged networks (trained on 32 seeded variants) on a small
+    # held-out variant: fewer pressure iterations, the tolerance met on both
+    # sides, and the same motion as the reference's start.
+    sc = scenes.dam_break_variant(4242, per_side=40, n_spheres=3)
+    th = learned.load()
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    plain = learned.evaluate(sc, None, steps=80, warmup=80)
+    lrn = learned.evaluate(sc, th, steps=80, warmup=80)
+    assert lrn["flags"] == 0 and lrn["avg_density_err"] <= eta
+    assert lrn["pressure_iterations"] < 0.8 * plain["pressure_iterations"], (plain, lrn)
+    assert abs(lrn["sim_time"] - plain["sim_time"]) < 0.05 * plain["sim_time"]
