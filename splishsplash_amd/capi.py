@@ -84,6 +84,7 @@ _SIGS = {
     "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_time_only": (C.c_int, [_vp, C.c_char_p]),
+    "sph_dfsph_set_lanes_per_particle": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_kernel_stats": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_longlong),
                                          _dp, C.POINTER(C.c_int)]),
     "sph_dfsph_reset_kernel_stats": (C.c_int, [_vp]),
@@ -443,6 +444,10 @@ class TimeStepDFSPH:
 
     def enable_timing(self, on=True):
         _chk(lib().sph_dfsph_enable_timing(self.h, int(on)))
+
+    def set_lanes_per_particle(self, lanes):
+        """1, 2 or 4 lanes per particle in the solver sweeps; 0 = autotune."""
+        _chk(lib().sph_dfsph_set_lanes_per_particle(self.h, int(lanes)))
 
     def time_only(self, names):
         """Restrict timing to these kernel names (None: all)."""

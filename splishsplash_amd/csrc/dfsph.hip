@@ -206,13 +206,79 @@ __device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float 
     return make_float3(ax, ay, az);
 }
 
-template <bool PRESSURE>
+// Neighbour sums of the solver iterations may be split over LPP adjacent
+// lanes: lane h of a particle takes neighbours h, h + LPP, ..., and the
+// partial sums are combined by butterfly shuffles in a fixed order (results
+// are deterministic; LPP = 1 is the plain sequential sum). Shorter per-lane
+// chains of dependent gathers keep more loads in flight per wave.
+template <int LPP>
+__device__ __forceinline__ float lane_sum(float v) {
+    if (LPP >= 2) v += __shfl_xor(v, 1, 64);
+    if (LPP >= 4) v += __shfl_xor(v, 2, 64);
+    if (LPP >= 8) v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+template <bool PRESSURE, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
     if (check_done && (PRESSURE ? D.ctl->done : D.ctl->done_v)) return;
-    const unsigned i = gtid();
-    if (i >= D.n) return;
-    const float3 a = accel_i(D, i, PRESSURE ? D.p : D.pv);
-    D.ap[i] = make_float4(a.x, a.y, a.z, 0.f);
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, h = t % LPP;
+    const bool act = i < D.n;
+    const float *__restrict__ P = PRESSURE ? D.p : D.pv;
+    float ax = 0.f, ay = 0.f, az = 0.f, pi = 0.f;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+        xi = D.x[i];
+        pi = P[i];
+        const unsigned cnt = D.cf[i];
+        const unsigned *__restrict__ L = D.nf + i;
+        const size_t S = D.stride;
+        unsigned k = h;
+        for (; k + 3 * LPP < cnt; k += 4 * LPP) {
+            const unsigned j0 = L[k * S], j1 = L[(k + LPP) * S], j2 = L[(k + 2 * LPP) * S],
+                           j3 = L[(k + 3 * LPP) * S];
+            const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
+            accel_pair(D, xi, pi, j0, p0, ax, ay, az);
+            accel_pair(D, xi, pi, j1, p1, ax, ay, az);
+            accel_pair(D, xi, pi, j2, p2, ax, ay, az);
+            accel_pair(D, xi, pi, j3, p3, ax, ay, az);
+        }
+        for (; k < cnt; k += LPP) {
+            const unsigned j = L[k * S];
+            accel_pair(D, xi, pi, j, P[j], ax, ay, az);
+        }
+    }
+    if (LPP > 1) {
+        ax = lane_sum<LPP>(ax);
+        ay = lane_sum<LPP>(ay);
+        az = lane_sum<LPP>(az);
+    }
+    if (!act || h != 0) return;
+    if (fabsf(pi) > D.eps) {
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned k = 0; k < cb; k++) {
+                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                ax += pi * (-xb.w * (g * r.x));
+                ay += pi * (-xb.w * (g * r.y));
+                az += pi * (-xb.w * (g * r.z));
+            }
+        } else if (D.bmode == 2) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
+                float3 r;
+                const float g = kernel_G(xi, xb, D.K, r);
+                ax += pi * (-xb.w * (g * r.x));
+                ay += pi * (-xb.w * (g * r.y));
+                az += pi * (-xb.w * (g * r.z));
+            }
+        }
+    }
+    D.ap[i] = make_float4(ax, ay, az, 0.f);
 }
 
 __device__ __forceinline__ float aij_term(const float4 &xi, const float4 &ai, const float4 &xj, const float4 &aj,
@@ -225,24 +291,29 @@ __device__ __forceinline__ float aij_term(const float4 &xi, const float4 &ai, co
 // compute_aij_pj (TimeStepDFSPH.cpp:1370-1420) and the Jacobi update of
 // pressureSolveIteration / divergenceSolveIteration (:544-706), FP64 error
 // sum finished in the last block.
-template <bool PRESSURE>
+template <bool PRESSURE, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
     __shared__ double lds[BLOCK / 64];
     Ctl *C = D.ctl;
     if (check_done && (PRESSURE ? C->done : C->done_v)) return;
-    const unsigned i = gtid();
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
     const double h = C->h;
     double err = 0.0, over = 0.0;
-    if (i < D.n) {
-        const float4 xi = D.x[i];
-        const float4 ai = D.ap[i];
-        float s = 0.f;
-        const unsigned cnt = D.cf[i];
+    float s = 0.f;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), ai = xi;
+    unsigned cnt = 0;
+    if (act) {
+        xi = D.x[i];
+        ai = D.ap[i];
+        cnt = D.cf[i];
         const unsigned *__restrict__ L = D.nf + i;
         const size_t S = D.stride;
-        unsigned k = 0;
-        for (; k + 4 <= cnt; k += 4) {
-            const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        unsigned k = hl;
+        for (; k + 3 * LPP < cnt; k += 4 * LPP) {
+            const unsigned j0 = L[k * S], j1 = L[(k + LPP) * S], j2 = L[(k + 2 * LPP) * S],
+                           j3 = L[(k + 3 * LPP) * S];
             const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
             const float4 a0 = D.ap[j0], a1 = D.ap[j1], a2 = D.ap[j2], a3 = D.ap[j3];
             s += aij_term(xi, ai, x0, a0, D.K);
@@ -250,10 +321,13 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
             s += aij_term(xi, ai, x2, a2, D.K);
             s += aij_term(xi, ai, x3, a3, D.K);
         }
-        for (; k < cnt; k++) {
+        for (; k < cnt; k += LPP) {
             const unsigned j = L[k * S];
             s += aij_term(xi, ai, D.x[j], D.ap[j], D.K);
         }
+    }
+    if (LPP > 1) s = lane_sum<LPP>(s);
+    if (act && hl == 0) {
         s *= D.V;
         unsigned nnb = cnt;
         if (D.bmode == 1) {
@@ -1363,7 +1437,7 @@ void DFSPH::init() {
     id2_.reserve(S);
     cf_.reserve(S);
     cb_.reserve(S);
-    partial_.reserve(3 * (size_t)nblocks(S) + 64);
+    partial_.reserve(8 * (size_t)nblocks(S) + 64);  // two values per block of up to 4 lanes per particle
     {
         std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
         std::vector<unsigned> ids(n);
@@ -1552,18 +1626,85 @@ void DFSPH::sweep_init(bool dens, bool fact, bool div) {
         LAUNCH("divergence_setup", (k_sweep_init<false, false, true>), D);
 }
 
-void DFSPH::div_iteration(bool force) {
+// One Jacobi iteration: accelerations, then the update, with LPP lanes per
+// particle in both neighbour sums.
+template <bool PRESSURE, int LPP>
+void DFSPH::iteration(bool force) {
     const Dev D = dev();
-    LAUNCH("pressure_accel_v", k_accel<false>, D, force ? 0 : 1);
-    LAUNCH("jacobi_v", k_jacobi<false>, D, force ? 0 : 1);
-    REDUCE("jacobi_reduce", k_jacobi_reduce<false>, D, nblocks(n_), force ? 0 : 1);
+    const unsigned nb = nblocks((size_t)n_ * LPP);
+    {
+        TimedScope ts(timer, stream, PRESSURE ? "pressure_accel" : "pressure_accel_v");
+        k_accel<PRESSURE, LPP><<<nb, BLOCK, 0, stream>>>(D, force ? 0 : 1);
+        SPH_HIP(hipGetLastError());
+    }
+    {
+        TimedScope ts(timer, stream, PRESSURE ? "jacobi" : "jacobi_v");
+        k_jacobi<PRESSURE, LPP><<<nb, BLOCK, 0, stream>>>(D, force ? 0 : 1);
+        SPH_HIP(hipGetLastError());
+    }
+    REDUCE("jacobi_reduce", k_jacobi_reduce<PRESSURE>, D, nb, force ? 0 : 1);
 }
 
-void DFSPH::pressure_iteration(bool force) {
+template <bool PRESSURE>
+void DFSPH::iteration_any(bool force) {
+    if (lpp_ == 4)
+        iteration<PRESSURE, 4>(force);
+    else if (lpp_ == 2)
+        iteration<PRESSURE, 2>(force);
+    else
+        iteration<PRESSURE, 1>(force);
+}
+
+void DFSPH::div_iteration(bool force) { iteration_any<false>(force); }
+
+void DFSPH::pressure_iteration(bool force) { iteration_any<true>(force); }
+
+// Picks the lanes per particle of the solver sweeps by timing the pressure
+// acceleration sweep (which only writes a_p, recomputed by every iteration)
+// for 1, 2 and 4 lanes on the current state; best of three launches each.
+void DFSPH::tune_lanes() {
+    tuned_ = true;
+    if (n_ < 65536 || lpp_fixed_) return;
     const Dev D = dev();
-    LAUNCH("pressure_accel", k_accel<true>, D, force ? 0 : 1);
-    LAUNCH("jacobi", k_jacobi<true>, D, force ? 0 : 1);
-    REDUCE("jacobi_reduce", k_jacobi_reduce<true>, D, nblocks(n_), force ? 0 : 1);
+    hipEvent_t a, b;
+    SPH_HIP(hipEventCreate(&a));
+    SPH_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    int pick = 1;
+    for (int L : {1, 2, 4}) {
+        const unsigned nb = nblocks((size_t)n_ * L);
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            SPH_HIP(hipEventRecord(a, stream));
+            if (L == 1)
+                k_accel<true, 1><<<nb, BLOCK, 0, stream>>>(D, 0);
+            else if (L == 2)
+                k_accel<true, 2><<<nb, BLOCK, 0, stream>>>(D, 0);
+            else
+                k_accel<true, 4><<<nb, BLOCK, 0, stream>>>(D, 0);
+            SPH_HIP(hipGetLastError());
+            SPH_HIP(hipEventRecord(b, stream));
+            SPH_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            SPH_HIP(hipEventElapsedTime(&ms, a, b));
+            tmin = std::min(tmin, ms);
+        }
+        tune_ms_[L == 1 ? 0 : (L == 2 ? 1 : 2)] = tmin;
+        if (tmin < best) {
+            best = tmin;
+            pick = L;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    lpp_ = pick;
+}
+
+void DFSPH::set_lanes_per_particle(int lpp) {
+    SPH_REQUIRE(lpp == 0 || lpp == 1 || lpp == 2 || lpp == 4, SPH_ERR_INVALID, "lanes per particle must be 0, 1, 2 or 4");
+    lpp_fixed_ = lpp != 0;
+    lpp_ = lpp ? lpp : 1;
+    tuned_ = lpp_fixed_;
 }
 
 void DFSPH::div_finish(bool scale_pv) {
@@ -1751,6 +1892,7 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
 }
 
 int DFSPH::run_solver(bool pressure, int already) {
+    if (!tuned_) tune_lanes();
     const int maxIt = pressure ? P.max_iterations : P.max_iterations_v;
     int launched = already;
     int chunk = std::max(1, pressure ? std::max(lastIt_, P.min_iterations) : lastItV_);
@@ -2086,6 +2228,7 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->learned_particles = engaged_[0] ? (int)hctl_->nlearn : 0;
     c->learned_particles_v = engaged_[1] ? (int)hctl_->nlearn_v : 0;
     c->learned_solves = learnedSolves_;
+    c->lanes_per_particle = lpp_;
     c->learned_solves_v = learnedSolvesV_;
     c->capped_solves = capped_;
     c->capped_solves_v = cappedV_;

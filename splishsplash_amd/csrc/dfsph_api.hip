@@ -222,6 +222,13 @@ int sph_dfsph_time_only(sph_dfsph_t s, const char *names) {
     });
 }
 
+int sph_dfsph_set_lanes_per_particle(sph_dfsph_t s, int lanes) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_lanes_per_particle(lanes);
+    });
+}
+
 int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name, long long *launches, double *total_ms,
                            int *count) {
     return guarded([&] {

@@ -73,7 +73,7 @@ class DFSPHCounters(C.Structure):
         ("capped_solves_v", C.c_int),
         ("learned_solves", C.c_int),
         ("learned_solves_v", C.c_int),
-        ("reserved", C.c_int),
+        ("lanes_per_particle", C.c_int),
         ("avg_overpressure_err", C.c_double),
         ("avg_overpressure_err_v", C.c_double),
     ]
