@@ -165,6 +165,65 @@ def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None):
     return res, kernels, live, K
 
 
+def ns_microbench(kind, h, n, reps, check):
+    """Neighbour search + FP32 density on `n` points in the unit cube
+    (`kind` uniform or clustered, seed 3) at support radius h: the time
+    step's own cell sort, neighbour lists and density sweep, timed per
+    kernel with HIP events over `reps` repetitions."""
+    import numpy as np
+
+    from splishsplash_amd import capi, scenes
+    from splishsplash_amd.params import default_params
+
+    x = scenes.uniform_points(n, seed=3) if kind == "uniform" else scenes.clustered_points(n, seed=3)
+    prm = default_params(particle_radius=h / 4.0, boundary_handling_method=-1, viscosity_method=0)
+    sim = capi.TimeStepDFSPH(prm)
+    sim.set_fluid(x)
+    sim.stage("neighborhood_search")  # first build sizes the lists
+    sim.stage("densities")
+    sim.enable_timing(True)
+    sim.reset_kernel_stats()
+    capi.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.stage("neighborhood_search")
+        sim.stage("densities")
+    capi.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    stats = sim.kernel_stats()
+    cnt = sim.get("fluid neighbor count")
+    K = float(cnt.mean())
+    ker = {}
+    for name, (launches, ms) in stats.items():
+        if launches:
+            ker[name] = {"avg_us": round(1e3 * ms / launches, 2), "per_rep": launches / reps}
+    # algorithmic bytes: sort = keys + histogram + scatter + permute of x
+    # (20 + 12 + 32), lists = 4K + 16, density = 4K + 12 + 16 + 4
+    bpp = {"fluid_lists": 4 * K + 16, "density": 4 * K + 32, "density_factor": 4 * K + 32}
+    for name, b in bpp.items():
+        if name in ker:
+            ker[name]["gbs"] = round(n * b / (ker[name]["avg_us"] * 1e-6) / 1e9, 1)
+    res = {"kind": kind, "h": h, "n": n, "mean_neighbors": K, "max_neighbors": int(cnt.max()),
+           "ms_per_rep": 1e3 * wall, "kernels": ker}
+    if check:
+        from oracle import oracle
+        x32 = np.ascontiguousarray(x, np.float32)
+        if K < 100:
+            off_g, idx_g = sim.neighbors(boundary=False)
+            off_o, idx_o = oracle.neighbors(x32, radius=np.float32(h))
+            same = np.array_equal(off_g, off_o) and all(
+                np.array_equal(np.sort(idx_g[off_g[i]:off_g[i + 1]]), np.sort(idx_o[off_o[i]:off_o[i + 1]]))
+                for i in np.random.default_rng(0).integers(0, n, 20000))
+            res["check"] = {"sets": "all counts, 20000 sampled sets", "equal": bool(same)}
+        else:
+            q = np.random.default_rng(0).integers(0, n, 2000)
+            off_o, _ = oracle.neighbors(x32[q], x32, radius=np.float32(h), brute=True)
+            res["check"] = {"sets": "counts of 2000 sampled particles (brute force)",
+                            "equal": bool(np.array_equal(np.diff(off_o) - 1, cnt[q]))}
+    del sim
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +235,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--plain-only", action="store_true", help="skip the learned warm-start run")
     ap.add_argument("--weights", default=None, help="warm-start networks (default: the packaged ones)")
+    ap.add_argument("--workload", default="dam1m", choices=["dam1m", "ns"],
+                    help="ns: the 4M-point neighbour-search + density microbenchmark")
+    ap.add_argument("--ns-points", type=int, default=4_000_000)
+    ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -183,6 +246,19 @@ def main():
         sys.exit(2)
 
     from splishsplash_amd import learned, scenes
+
+    if args.workload == "ns":
+        cases = []
+        for kind, h in (("uniform", 0.0128), ("clustered", 0.0128), ("uniform", 0.0375)):
+            t0 = time.perf_counter()
+            r = ns_microbench(kind, h, args.ns_points, reps=5, check=not args.no_check)
+            log(f"bench ns: {kind} h={h}: K={r['mean_neighbors']:.1f} (max {r['max_neighbors']}), "
+                f"{r['ms_per_rep']:.3f} ms per search + density, check {r.get('check')} "
+                f"({time.perf_counter() - t0:.1f} s)")
+            cases.append(r)
+        print(json.dumps({"metric": "neighbour search + FP32 density, 4M points", "unit": "ms per search",
+                          "cases": cases}))
+        return
 
     sc = scenes.dam_break_1m(per_side=args.per_side)
     n = len(sc["x"])

@@ -1660,12 +1660,17 @@ void DFSPH::div_iteration(bool force) { iteration_any<false>(force); }
 void DFSPH::pressure_iteration(bool force) { iteration_any<true>(force); }
 
 // Picks the lanes per particle of the solver sweeps by timing the pressure
-// acceleration sweep (which only writes a_p, recomputed by every iteration)
-// for 1, 2 and 4 lanes on the current state; best of three launches each.
+// acceleration sweep for 1, 2 and 4 lanes on the current neighbour lists,
+// best of three launches each. The sweep reads a uniform pressure (every
+// pair active) from scratch and writes to scratch, so the solver state is
+// untouched.
 void DFSPH::tune_lanes() {
     tuned_ = true;
     if (n_ < 65536 || lpp_fixed_) return;
-    const Dev D = dev();
+    Dev D = dev();
+    fill_u32(stream, reinterpret_cast<unsigned *>(p2_.get()), n_, 0x3f800000u);  // 1.0f
+    D.p = p2_.get();
+    D.ap = x2_.get();
     hipEvent_t a, b;
     SPH_HIP(hipEventCreate(&a));
     SPH_HIP(hipEventCreate(&b));

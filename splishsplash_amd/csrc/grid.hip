@@ -40,16 +40,6 @@ __global__ void k_fill_u32(unsigned *p, size_t n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cell_hist(const float4 *__restrict__ x, unsigned n, GridDesc g,
-                                                     unsigned *__restrict__ pcell, unsigned *__restrict__ prank,
-                                                     unsigned *__restrict__ count) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const unsigned c = cell_id(cell_of(x[i], g), g);
-    pcell[i] = c;
-    prank[i] = atomicAdd(&count[c], 1u);
-}
-
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
 
@@ -134,31 +124,107 @@ __global__ __launch_bounds__(BLOCK) void k_scan_add(unsigned *__restrict__ out, 
         if (base + k < n) out[base + k] += add;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scatter(const unsigned *__restrict__ pcell,
-                                                   const unsigned *__restrict__ prank,
-                                                   const unsigned *__restrict__ start, unsigned n,
-                                                   unsigned *__restrict__ perm) {
+// ---------------------------------------------------------- radix sort
+// Stable LSD radix sort of the cell keys, 8 bits per pass, 1024 keys per
+// block. Points start in index order and every pass is stable, so each cell
+// ends up ordered by point index whatever the cell sizes (a dense cluster
+// costs no more than a uniform cloud) and no atomic order leaks into it.
+constexpr int RADIX = 256;
+constexpr int RROUNDS = 4;
+constexpr unsigned RTILE = BLOCK * RROUNDS;
+static_assert(BLOCK == RADIX, "one thread per digit");
+
+__global__ __launch_bounds__(BLOCK) void k_cell_keys(const float4 *__restrict__ x, unsigned n, GridDesc g,
+                                                     unsigned *__restrict__ key) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    perm[start[pcell[i]] + prank[i]] = i;
+    if (i < n) key[i] = cell_id(cell_of(x[i], g), g);
 }
 
-// Orders the slots of each cell by point index (insertion sort; cells hold a
-// handful of points).
-__global__ __launch_bounds__(BLOCK) void k_sort_cells(const unsigned *__restrict__ start, unsigned ncells,
-                                                      unsigned *__restrict__ perm) {
-    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const unsigned s = start[c], e = start[c + 1];
-    for (unsigned a = s + 1; a < e; a++) {
-        const unsigned v = perm[a];
-        unsigned b = a;
-        while (b > s && perm[b - 1] > v) {
-            perm[b] = perm[b - 1];
-            b--;
-        }
-        perm[b] = v;
+// hist[d * nb + b] = keys of block b with digit d.
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const unsigned *__restrict__ key, unsigned n, int shift,
+                                                      unsigned nb, unsigned *__restrict__ hist) {
+    __shared__ unsigned h[RADIX];
+    const unsigned t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * RTILE;
+#pragma unroll
+    for (int r = 0; r < RROUNDS; r++) {
+        const size_t idx = base + (size_t)r * BLOCK + t;
+        if (idx < n) atomicAdd(&h[(key[idx] >> shift) & (RADIX - 1)], 1u);
     }
+    __syncthreads();
+    hist[(size_t)t * nb + blockIdx.x] = h[t];
+}
+
+// Stable scatter of one pass: an element's slot is its digit's offset for
+// the block plus the number of same-digit elements before it in the block
+// (earlier rounds, earlier waves, lower lanes). Lanes with the same digit
+// are found with eight ballots.
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(const unsigned *__restrict__ kin,
+                                                         const unsigned *__restrict__ vin, unsigned n, int shift,
+                                                         unsigned nb, const unsigned *__restrict__ hoff,
+                                                         unsigned *__restrict__ kout, unsigned *__restrict__ vout) {
+    __shared__ unsigned run[RADIX];
+    __shared__ unsigned wc[BLOCK / 64][RADIX];
+    const unsigned t = threadIdx.x, w = t >> 6, lane = t & 63u;
+    run[t] = hoff[(size_t)t * nb + blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * RTILE;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    for (int r = 0; r < RROUNDS; r++) {
+#pragma unroll
+        for (int q = 0; q < BLOCK / 64; q++) wc[q][t] = 0u;
+        __syncthreads();
+        const size_t idx = base + (size_t)r * BLOCK + t;
+        const bool valid = idx < n;
+        const unsigned k = valid ? kin[idx] : 0u;
+        const unsigned d = (k >> shift) & (RADIX - 1);
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const unsigned long long lower = m & below;
+        const unsigned rank = (unsigned)__popcll(lower);
+        if (valid && lower == 0ull) wc[w][d] = (unsigned)__popcll(m);
+        __syncthreads();
+        {
+            unsigned acc = run[t];
+#pragma unroll
+            for (int q = 0; q < BLOCK / 64; q++) {
+                const unsigned c = wc[q][t];
+                wc[q][t] = acc;
+                acc += c;
+            }
+            run[t] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const unsigned pos = wc[w][d] + rank;
+            kout[pos] = k;
+            vout[pos] = vin ? vin[idx] : (unsigned)idx;
+        }
+        __syncthreads();
+    }
+}
+
+// start[c] = first slot whose key is >= c (binary search in the sorted
+// keys), c = 0 .. ncells; start[ncells] = n.
+__global__ __launch_bounds__(BLOCK) void k_cell_start(const unsigned *__restrict__ key, unsigned n, unsigned ncells,
+                                                      unsigned *__restrict__ start) {
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    unsigned lo = 0, hi = n;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (key[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[c] = lo;
 }
 
 template <class T>
@@ -215,6 +281,47 @@ __global__ void k_lists_cm(const float4 *__restrict__ q, unsigned nq, const floa
         if (r < kk) nbr[(size_t)r * stride + i] = wt[r * 64u + lane];
     const unsigned mk = wave_max_u(k);
     if (lane == 0 && mk > 0) atomicMax(maxcnt, mk);
+}
+
+// Same lists for capacities beyond the LDS tile (hundreds of neighbours):
+// entries are stored as they are found; each 4-byte store of a wave then
+// touches up to 64 rows, so this is used only where the tile cannot be.
+template <bool SAME>
+__global__ __launch_bounds__(BLOCK) void k_lists_cm_wide(const float4 *__restrict__ q, unsigned nq,
+                                                         const float4 *__restrict__ src,
+                                                         const unsigned *__restrict__ start, GridDesc g, float r2,
+                                                         unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
+                                                         unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    unsigned k = 0;
+    if (i < nq) {
+        const float4 xi = q[i];
+        const int3 c = cell_of(xi, g);
+        const int x0 = c.x > 0 ? c.x - 1 : 0, x1 = c.x < g.nx - 1 ? c.x + 1 : g.nx - 1;
+        for (int dz = -1; dz <= 1; dz++) {
+            const int z = c.z + dz;
+            if (z < 0 || z >= g.nz) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                const int y = c.y + dy;
+                if (y < 0 || y >= g.ny) continue;
+                const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
+                const unsigned s = start[row + x0], e = start[row + x1 + 1];
+                for (unsigned j = s; j < e; j++) {
+                    if (SAME && j == i) continue;
+                    const float4 xj = src[j];
+                    const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
+                    const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
+                    if (d2 < r2) {
+                        if (k < cap) nbr[(size_t)k * stride + i] = j;
+                        k++;
+                    }
+                }
+            }
+        }
+        cnt[i] = k;
+    }
+    const unsigned mk = wave_max_u(k);
+    if ((threadIdx.x & 63u) == 0 && mk > 0) atomicMax(maxcnt, mk);
 }
 
 template <bool SAME, bool FILL>
@@ -280,22 +387,44 @@ void exclusive_scan(hipStream_t s, const unsigned *in, unsigned *out, unsigned n
 void CellSorter::sort(hipStream_t s, const float4 *x, unsigned nn, const GridDesc &g, KernelTimer &T) {
     n = nn;
     grid = g;
-    count.reserve(g.ncells);
     start.reserve((size_t)g.ncells + 1);
-    pcell.reserve(std::max(n, 1u));
-    prank.reserve(std::max(n, 1u));
-    perm.reserve(std::max(n, 1u));
-    fill_u32(s, count.get(), g.ncells, 0u);
-    if (n) {
-        TimedScope ts(T, s, "cell_hist");
-        k_cell_hist<<<nblocks(n), BLOCK, 0, s>>>(x, n, g, pcell.get(), prank.get(), count.get());
+    const size_t cap = std::max(n, 1u);
+    pcell.reserve(cap);
+    keys2.reserve(cap);
+    vals2.reserve(cap);
+    perm.reserve(cap);
+    if (!n) {
+        fill_u32(s, start.get(), (size_t)g.ncells + 1, 0u);
+        return;
+    }
+    int bits = 1;
+    while (bits < 32 && ((g.ncells - 1u) >> bits) != 0u) bits++;
+    const int passes = (bits + 7) / 8;
+    const unsigned nb = (n + RTILE - 1) / RTILE;
+    hist.reserve((size_t)RADIX * nb);
+    hoff.reserve((size_t)RADIX * nb + 1);
+    // Pass p writes buffer set (passes - 1 - p) % 2 and reads the other one,
+    // so the last pass lands in set 0 = (pcell, perm).
+    unsigned *K[2] = {pcell.get(), keys2.get()};
+    unsigned *V[2] = {perm.get(), vals2.get()};
+    {
+        TimedScope ts(T, s, "cell_keys");
+        k_cell_keys<<<nblocks(n), BLOCK, 0, s>>>(x, n, g, K[passes % 2]);
         SPH_HIP(hipGetLastError());
     }
-    exclusive_scan(s, count.get(), start.get(), g.ncells, bsum, T);
-    if (n) {
-        TimedScope ts(T, s, "cell_scatter");
-        k_scatter<<<nblocks(n), BLOCK, 0, s>>>(pcell.get(), prank.get(), start.get(), n, perm.get());
-        k_sort_cells<<<nblocks(g.ncells), BLOCK, 0, s>>>(start.get(), g.ncells, perm.get());
+    for (int p = 0; p < passes; p++) {
+        TimedScope ts(T, s, "cell_radix");
+        const int out = (passes - 1 - p) % 2, in = 1 - out;
+        k_radix_hist<<<nb, BLOCK, 0, s>>>(K[in], n, 8 * p, nb, hist.get());
+        SPH_HIP(hipGetLastError());
+        exclusive_scan(s, hist.get(), hoff.get(), RADIX * nb, bsum, T);
+        k_radix_scatter<<<nb, BLOCK, 0, s>>>(K[in], p == 0 ? nullptr : V[in], n, 8 * p, nb, hoff.get(), K[out],
+                                             V[out]);
+        SPH_HIP(hipGetLastError());
+    }
+    {
+        TimedScope ts(T, s, "cell_start");
+        k_cell_start<<<nblocks((size_t)g.ncells + 1), BLOCK, 0, s>>>(pcell.get(), n, g.ncells, start.get());
         SPH_HIP(hipGetLastError());
     }
 }
@@ -326,7 +455,14 @@ void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const flo
     const unsigned waves = cap <= 64 ? 4u : (cap <= 128 ? 2u : 1u);
     const unsigned threads = 64u * waves;
     const size_t lds = (size_t)waves * cap * 64u * sizeof(unsigned);
-    SPH_REQUIRE(lds <= 65536, SPH_ERR_INVALID, "neighbour capacity above 256 does not fit the LDS tile");
+    if (lds > 65536) {
+        if (same_set)
+            k_lists_cm_wide<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        else
+            k_lists_cm_wide<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        SPH_HIP(hipGetLastError());
+        return;
+    }
     const unsigned grid = (nq + threads - 1) / threads;
     if (same_set)
         k_lists_cm<true><<<grid, threads, lds, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);

@@ -41,7 +41,7 @@ __device__ __forceinline__ unsigned cell_id(const int3 &c, const GridDesc &g) {
 // only on the input order (no atomics order leaks into it).
 class CellSorter {
 public:
-    DBuf<unsigned> count, start, pcell, prank, perm, bsum;
+    DBuf<unsigned> start, pcell, perm, keys2, vals2, hist, hoff, bsum;  // pcell: sorted cell keys after sort()
     unsigned n = 0;
     GridDesc grid{};
     void sort(hipStream_t s, const float4 *x, unsigned n, const GridDesc &g, KernelTimer &T);

@@ -132,6 +132,31 @@ def test_microbench_uniform_literal_radius_small():
     assert_same_csr(off_g, idx_g, off_o, idx_o)
 
 
+def test_dfsph_lists_beyond_the_lds_tile():
+    # Clustered points with several hundred neighbours each: the time step's
+    # lists take the path for capacities beyond the LDS tile; sets and the
+    # FP32 densities still match the oracle.
+    from splishsplash_amd.params import default_params
+    x = np.ascontiguousarray(scenes.clustered_points(60000, seed=5, clusters=4, sigma=0.02), np.float32)
+    h = 0.02
+    prm = default_params(particle_radius=h / 4, boundary_handling_method=-1, viscosity_method=0)
+    g = capi.TimeStepDFSPH(prm)
+    g.set_fluid(x.astype(np.float64))
+    g.stage("neighborhood_search")
+    assert g.counters()["max_fluid_neighbors"] > 300
+    off_g, idx_g = g.neighbors(boundary=False)
+    off_o, idx_o = oracle.neighbors(x, radius=np.float32(h))
+    assert_same_csr(off_g, idx_g, off_o, idx_o)
+    g.stage("densities")
+    o = oracle.OracleDFSPH(prm)
+    o.set_fluid(x.astype(np.float64), np.zeros_like(x, np.float64))
+    o.init()
+    o.stage("neighborhood_search")
+    o.stage("densities")
+    ro = o.get("density")
+    assert np.abs(g.get("density") - ro).max() / np.abs(ro).max() <= 1e-5
+
+
 def test_empty_and_tiny_sets():
     ns = capi.NeighborhoodSearch(0.1)
     e = np.zeros((0, 3), np.float32)
