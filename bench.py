@@ -112,7 +112,7 @@ def cpu_baseline(sim, sc, steps):
                       f"{oc['iterations']}/{oc['iterations_v']}/{oc['visco_iterations']})"}
 
 
-def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None):
+def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None, mode=None):
     """Steps one fresh simulation of `sc` (thetas: learned warm-start
     networks or None for the reference's start): `warmup` untimed steps,
     `breakdown_steps` steps with every kernel timed by HIP events (the
@@ -120,7 +120,7 @@ def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None):
     `timed_kernel` (default: the largest in the table) carries events."""
     from splishsplash_amd import capi, learned
 
-    sim = learned.make_sim(sc, thetas)
+    sim = learned.make_sim(sc, thetas, mode)
     sim.step(warmup)
     kernels = {}
     if breakdown_steps:
@@ -159,7 +159,8 @@ def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None):
            "divergence_solves_at_cap": c1["capped_solves_v"] - c0["capped_solves_v"],
            "avg_density_err": c1["avg_density_err"], "avg_overpressure_err": c1["avg_overpressure_err"],
            "time_step_size": c1["time_step_size"], "sim_time": c1["time"], "flags": c1["flags"],
-           "learned_particles": c1["learned_particles"]}
+           "learned_particles": c1["learned_particles"], "lanes_per_particle": c1["lanes_per_particle"],
+           "host_syncs_per_step": (c1["host_syncs"] - c0["host_syncs"]) / steps}
     K = float(sim.get("fluid neighbor count").mean())
     del sim
     return res, kernels, live, K
@@ -276,6 +277,13 @@ def main():
         lrn, k_lrn, live_lrn, _ = run_path(sc, thetas, args.warmup, args.steps, args.breakdown_steps)
         log(f"bench: learned {lrn['ms_per_step']:.3f} ms/step, {lrn['pressure_iterations_per_step']:.2f} pressure "
             f"iterations/step ({time.perf_counter() - t0:.1f} s)")
+    prev = None
+    if not args.plain_only:
+        # the previous step's full stiffness as the start (warm-start mode 1,
+        # no network): what the network adds beyond that
+        prev, _, _, _ = run_path(sc, None, args.warmup, args.steps, 0, mode="previous")
+        log(f"bench: previous-stiffness start {prev['ms_per_step']:.3f} ms/step, "
+            f"{prev['pressure_iterations_per_step']:.2f} pressure iterations/step")
     head, kernels, live = (lrn, k_lrn, live_lrn) if lrn else (plain, k_plain, live_plain)
 
     roofline = None
@@ -323,6 +331,7 @@ def main():
         "cpu_baseline": cpu,
         "plain": plain,
         "learned": lrn,
+        "previous_stiffness_start": prev,
         "learned_vs_plain": ({"speedup": lrn["value"] / plain["value"],
                               "pressure_iteration_cut": 1.0 - lrn["pressure_iterations_per_step"]
                               / max(plain["pressure_iterations_per_step"], 1e-30),

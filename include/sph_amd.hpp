@@ -168,6 +168,21 @@ public:
                             double thickness = 0.0) {
         check(sph_dfsph_set_volume_map_box(h_, lo, hi, invert, thickness, res), "set_volume_map_box");
     }
+    void add_volume_map_sphere(const double centre[3], double radius, const unsigned res[3], bool invert = false,
+                               double thickness = 0.0) {
+        check(sph_dfsph_add_volume_map_sphere(h_, centre, radius, invert, thickness, res), "add_volume_map_sphere");
+    }
+    // Learned warm start (no reference counterpart): solver 0 pressure, 1
+    // divergence; theta = 4673 floats, or nullptr for the reference's start.
+    void set_warm_start_model(int solver, const float *theta) {
+        check(sph_dfsph_set_warm_start_model(h_, solver, theta), "set_warm_start_model");
+    }
+    void set_warm_start_mode(int solver, int mode) {
+        check(sph_dfsph_set_warm_start_mode(h_, solver, mode), "set_warm_start_mode");
+    }
+    void set_warm_start_policy(int solver, int min_prev_iterations) {
+        check(sph_dfsph_set_warm_start_policy(h_, solver, min_prev_iterations), "set_warm_start_policy");
+    }
     void init() { check(sph_dfsph_init(h_), "init"); }
     void step() { check(sph_dfsph_step(h_, 1), "step"); }
     void step(int n) { check(sph_dfsph_step(h_, n), "step"); }

@@ -198,6 +198,8 @@ typedef struct sph_dfsph_counters {
      * bring both below the tolerance. Same units as the errors above. */
     double avg_overpressure_err;
     double avg_overpressure_err_v;
+    long long host_syncs;    /* host waits on the device so far (solver
+                                chunk ends, list capacity checks)           */
 } sph_dfsph_counters_t;
 
 /* Sub-steps of TimeStepDFSPH::step() (TimeStepDFSPH.cpp:117-249), callable
@@ -319,16 +321,24 @@ int sph_mlp_train(float *theta, const float *x, const float *t, size_t n,
 
 /* Learned warm start inside the time step (no counterpart in the reference;
  * the hook replaces the warm starts of TimeStepDFSPH.cpp:292-296 and
- * :449-453). solver: 0 pressure solve, 1 divergence solve. With a model set,
- * the solve starts from the network's prediction wherever its initial
- * residual is smaller than that of the plain previous-step warm start, and
- * then iterates to the same tolerance on the reference's error (remaining
- * compression) and on the overshoot (counters avg_overpressure_err*), which
- * the reference's conservative start keeps near zero by construction.
- * theta == NULL restores the plain start. */
+ * :449-453). solver: 0 pressure solve, 1 divergence solve. With a model set
+ * (warm-start mode 2), the solve starts from the network's prediction
+ * wherever its initial residual is smaller than that of the previous-step
+ * start, and then iterates to the same tolerance on the reference's error
+ * (remaining compression) and on the overshoot (counters
+ * avg_overpressure_err*), which the reference's conservative start keeps
+ * near zero by construction. theta == NULL restores the reference's start
+ * (mode 0). */
 int sph_dfsph_set_warm_start_model(sph_dfsph_t s, int solver,
                                    const float *theta);
-/* When to use the network (no reference counterpart): after a solve from
+/* Start of the solves of `solver` (no reference counterpart): 0 the
+ * reference's warm start; 1 the previous step's full stiffness (the
+ * reference's start without its factor 1/2 and clamp), kept per particle
+ * where it beats the same start masked as the reference masks it; 2 the
+ * network's prediction (needs sph_dfsph_set_warm_start_model, which selects
+ * it), with the masked previous stiffness as the per-particle fallback. */
+int sph_dfsph_set_warm_start_mode(sph_dfsph_t s, int solver, int mode);
+/* When to use the better start (no reference counterpart): after a solve from
  * the reference's start, the next solve of the same kind starts from the
  * network if that solve took at least `min_prev_iterations` iterations
  * (defaults: pressure 4, divergence 3); below that the reference's start is

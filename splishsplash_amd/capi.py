@@ -80,6 +80,7 @@ _SIGS = {
     "sph_dfsph_set_warm_start_model": (C.c_int, [_vp, C.c_int, _fp]),
     "sph_dfsph_collect_training": (C.c_int, [_vp, C.c_int, C.c_uint, C.c_size_t]),
     "sph_dfsph_set_warm_start_policy": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "sph_dfsph_set_warm_start_mode": (C.c_int, [_vp, C.c_int, C.c_int]),
     "sph_dfsph_get_training": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_size_t, _szp]),
     "sph_dfsph_set_time": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_enable_timing": (C.c_int, [_vp, C.c_int]),
@@ -417,6 +418,14 @@ class TimeStepDFSPH:
         self._theta = getattr(self, "_theta", {})
         self._theta[solver] = th
         _chk(lib().sph_dfsph_set_warm_start_model(self.h, s, th.ctypes.data_as(_fp)))
+
+    WARM_START_MODES = {"reference": 0, "previous": 1, "network": 2}
+
+    def set_warm_start_mode(self, solver, mode):
+        """Start of `solver`'s solves: "reference" (0), "previous" (1, the
+        previous step's full stiffness) or "network" (2)."""
+        m = self.WARM_START_MODES[mode] if isinstance(mode, str) else int(mode)
+        _chk(lib().sph_dfsph_set_warm_start_mode(self.h, self._SOLVERS[solver], m))
 
     def set_warm_start_policy(self, solver, min_prev_iterations):
         """Use the network for `solver` only when its previous solve took at

@@ -1060,19 +1060,36 @@ __global__ __launch_bounds__(BLOCK) void k_features(Dev D) {
     reinterpret_cast<float4 *>(F)[1] = make_float4(bvol / D.V, (float)nnb * (1.0f / 40.0f), 0.f, 0.f);
 }
 
-// Network output -> start value of the solve: p = max(y, 0) kref / h^2, or
-// p_v = max(y, 0) kref_v / h where the reference's divergence warm start
-// applies (rho_dot > 0 and at least 20 neighbours, TimeStepDFSPH.cpp:418-453).
-template <bool PRESSURE>
+// The two candidate starts of a learned solve. Fallback (into p / p_v): the
+// previous step's full stiffness wherever the reference starts from it
+// (rho_adv > 1; rho_dot > 0 with at least 20 neighbours,
+// TimeStepDFSPH.cpp:292-296, 418-453), i.e. the reference's start without
+// its factor 1/2 and clamp. Candidate (into pred): the previous stiffness
+// everywhere (NET = false), or plus the network's predicted change,
+// p = max(kappa_prev / kref + y, 0) kref / h^2 (NET = true; p_v: / h).
+template <bool PRESSURE, bool NET>
 __global__ __launch_bounds__(BLOCK) void k_pred_start(Dev D) {
     const unsigned i = gtid();
     if (i >= D.n) return;
     const double h = D.ctl->h;
-    const float sc = PRESSURE ? (float)(D.ctl->kref / (h * h)) : (float)(D.ctl->kref_v / h);
-    float q = fmaxf(D.pred[i], 0.0f) * sc;
-    if (!PRESSURE) {
+    const float c = PRESSURE ? (float)(1.0 / (h * h)) : (float)(1.0 / h);
+    const float kp = fmaxf(D.kprev[i], 0.0f);
+    float q;
+    if (NET) {
+        const float kref = (float)(PRESSURE ? D.ctl->kref : D.ctl->kref_v);
+        const float *F = (PRESSURE ? D.feats : D.feats_v) + (size_t)i * 8;
+        q = fmaxf(F[2] + D.pred[i], 0.0f) * (kref * c);
+    } else {
+        q = kp * c;
+    }
+    float b = kp * c;
+    if (PRESSURE) {
+        if (!(D.radv[i] > 1.0f)) b = 0.0f;
+        D.p[i] = b;
+    } else {
         const unsigned nnb = D.cf[i] + (D.bmode == 1 ? D.cb[i] : 0u);
-        if (!(D.radv[i] > 0.0f) || nnb < 20u) q = 0.0f;
+        if (!(D.radv[i] > 0.0f) || nnb < 20u) b = q = 0.0f;
+        D.pv[i] = b;
     }
     D.pred[i] = q;
 }
@@ -1410,6 +1427,7 @@ Dev DFSPH::dev() const {
 }
 
 void DFSPH::read_ctl() {
+    syncs_++;
     SPH_HIP(hipMemcpyAsync(hctl_, ctl_.get(), sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     SPH_HIP(hipStreamSynchronize(stream));
 }
@@ -1521,7 +1539,7 @@ void DFSPH::init() {
     }
     nvm_ = (unsigned)nobj;
     initialized_ = true;
-    if (learned_[0] || learned_[1] || stride_s_[0] || stride_s_[1]) ensure_learn_buffers();
+    if (mode_[0] || mode_[1] || stride_s_[0] || stride_s_[1]) ensure_learn_buffers();
     if (P.viscosity_method == SPH_VISCOSITY_WEILER2018) {
         minv_.reserve(2 * S);
         for (DBuf<float4> *b : {&cgx_, &cgr_, &cgp_, &cgz_, &cgt_}) b->reserve(S);
@@ -1731,43 +1749,52 @@ bool DFSPH::learned_start(bool pressure) {
     const bool was = engaged_[solver];
     engaged_[solver] = false;
     if (!learn_or_collect(solver)) return false;
-    // The network pays only when the reference's start is far from the
+    // A better start pays only when the reference's start is far from the
     // solution. After a solve from the reference's start, engage it if that
     // solve took at least engage_min_ iterations; after a solve from the
-    // network, keep it unless that solve ended at the minimum count (then
-    // the next step tries the reference's start again).
+    // better start, keep it unless that solve ended at the minimum count
+    // (then the next step tries the reference's start again).
+    const int mode = mode_[solver];
     const int prev = pressure ? itLast_ : itVLast_;
     const int floor = pressure ? std::max(P.min_iterations, 1) : 1;
-    const bool use = learned_[solver] &&
-                     (engage_min_[solver] == 0 || (was ? prev > floor : prev >= engage_min_[solver]));
-    if (!use && stride_s_[solver] == 0) return false;
+    const bool use = mode != 0 && (engage_min_[solver] == 0 || (was ? prev > floor : prev >= engage_min_[solver]));
+    const bool collecting = stride_s_[solver] > 0;
+    if (!use && !collecting) return false;
     Dev D = dev();
     D.kprev = kprev_.get();
-    LAUNCH("learn_scale", k_max_partial, D, kprev_.get());
-    if (pressure)
-        REDUCE("learn_scale", k_kref_reduce<true>, D, nblocks(n_));
-    else
-        REDUCE("learn_scale", k_kref_reduce<false>, D, nblocks(n_));
-    if (pressure)
-        LAUNCH("learn_features", k_features<true>, D);
-    else
-        LAUNCH("learn_features", k_features<false>, D);
+    if (collecting || mode == 2) {
+        LAUNCH("learn_scale", k_max_partial, D, kprev_.get());
+        if (pressure)
+            REDUCE("learn_scale", k_kref_reduce<true>, D, nblocks(n_));
+        else
+            REDUCE("learn_scale", k_kref_reduce<false>, D, nblocks(n_));
+        if (pressure)
+            LAUNCH("learn_features", k_features<true>, D);
+        else
+            LAUNCH("learn_features", k_features<false>, D);
+    }
     if (!use) return false;
     engaged_[solver] = true;
     (pressure ? learnedSolves_ : learnedSolvesV_)++;
     D.strictP = engaged_[0];
     D.strictV = engaged_[1];
-    {
+    if (mode == 2) {
         TimedScope ts(timer, stream, "learn_mlp");
         mlp_forward(stream, theta_[solver].get(), feats_[solver].get(), (unsigned)n_, pred_.get());
     }
     if (pressure) {
-        LAUNCH("learn_start", k_pred_start<true>, D);
+        if (mode == 2)
+            LAUNCH("learn_start", (k_pred_start<true, true>), D);
+        else
+            LAUNCH("learn_start", (k_pred_start<true, false>), D);
         LAUNCH("learn_accel2", k_accel2<true>, D);
         LAUNCH("learn_jacobi2", k_jacobi2<true>, D);
         REDUCE("jacobi_reduce", (k_jacobi_reduce<true, true>), D, nblocks(n_), 0);
     } else {
-        LAUNCH("learn_start", k_pred_start<false>, D);
+        if (mode == 2)
+            LAUNCH("learn_start", (k_pred_start<false, true>), D);
+        else
+            LAUNCH("learn_start", (k_pred_start<false, false>), D);
         LAUNCH("learn_accel2", k_accel2<false>, D);
         LAUNCH("learn_jacobi2", k_jacobi2<false>, D);
         REDUCE("jacobi_reduce", (k_jacobi_reduce<false, true>), D, nblocks(n_), 0);
@@ -1789,12 +1816,22 @@ void DFSPH::set_warm_start_model(int solver, const float *theta) {
     SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
     if (!theta) {
         learned_[solver] = false;
+        mode_[solver] = 0;
         return;
     }
     theta_[solver].reserve(MLP_NPARAM);
     SPH_HIP(hipMemcpy(theta_[solver].get(), theta, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
     learned_[solver] = true;
+    mode_[solver] = 2;
     ensure_learn_buffers();
+}
+
+void DFSPH::set_warm_start_mode(int solver, int mode) {
+    SPH_REQUIRE(solver == 0 || solver == 1, SPH_ERR_INVALID, "solver must be 0 (pressure) or 1 (divergence)");
+    SPH_REQUIRE(mode >= 0 && mode <= 2, SPH_ERR_INVALID, "warm-start mode must be 0, 1 or 2");
+    SPH_REQUIRE(mode != 2 || learned_[solver], SPH_ERR_STATE, "mode 2 needs a network (set_warm_start_model)");
+    mode_[solver] = mode;
+    if (mode) ensure_learn_buffers();
 }
 
 void DFSPH::set_warm_start_policy(int solver, int min_prev_iterations) {
@@ -1911,7 +1948,10 @@ int DFSPH::run_solver(bool pressure, int already) {
         read_ctl();
         const int done = pressure ? hctl_->done : hctl_->done_v;
         if (done || launched >= maxIt) break;
-        chunk = 1;
+        // past the predicted count: two iterations per round trip (an
+        // iteration of a converged solve costs a few microseconds of early
+        // exits, a host round trip tens)
+        chunk = 2;
     }
     const int it = pressure ? hctl_->it : hctl_->it_v;
     if (it >= maxIt) {
@@ -2234,6 +2274,7 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->learned_particles_v = engaged_[1] ? (int)hctl_->nlearn_v : 0;
     c->learned_solves = learnedSolves_;
     c->lanes_per_particle = lpp_;
+    c->host_syncs = syncs_;
     c->learned_solves_v = learnedSolvesV_;
     c->capped_solves = capped_;
     c->capped_solves_v = cappedV_;

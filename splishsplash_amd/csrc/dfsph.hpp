@@ -97,6 +97,7 @@ public:
     void get_neighbors(int boundary, unsigned *offsets, unsigned *indices, size_t capacity, size_t *nnz);
     void set_warm_start_model(int solver, const float *theta);
     void set_warm_start_policy(int solver, int min_prev_iterations);
+    void set_warm_start_mode(int solver, int mode);
     void collect_training(int solver, unsigned stride, size_t capacity);
     void get_training(int solver, float *x, float *t, size_t capacity, size_t *n);
     void counters(sph_dfsph_counters_t *c);
@@ -131,7 +132,8 @@ private:
     long long totIt_ = 0, totItV_ = 0, totVisco_ = 0;
     int steps_ = 0, lastIt_ = 2, lastItV_ = 1, lastCg_ = 4;
     int itLast_ = 0, itVLast_ = 0, cgLast_ = 0;
-    int capped_ = 0, cappedV_ = 0;  // solves stopped at the iteration cap short of the tolerance
+    int capped_ = 0, cappedV_ = 0;
+    long long syncs_ = 0;  // host round trips (control-block reads)  // solves stopped at the iteration cap short of the tolerance
     // learned warm start
     bool learned_[2] = {false, false};
     unsigned stride_s_[2] = {0, 0};
@@ -139,7 +141,8 @@ private:
     DBuf<float> theta_[2], feats_[2], pred_, kprev_, sx_[2], st_[2];
     DBuf<float4> ap2_;
     unsigned soff_ = 0;
-    bool learn_or_collect(int solver) const { return learned_[solver] || stride_s_[solver] > 0; }
+    int mode_[2] = {0, 0};  // start of each solve: 0 reference, 1 previous stiffness, 2 network
+    bool learn_or_collect(int solver) const { return mode_[solver] != 0 || stride_s_[solver] > 0; }
     bool learned_start(bool pressure);  // true if this solve starts from the network
     bool engaged_[2] = {false, false};   // the current/last solve of each kind started from the network
     int engage_min_[2] = {4, 3};         // engage when the previous solve took at least this many iterations

@@ -160,6 +160,13 @@ int sph_dfsph_set_warm_start_model(sph_dfsph_t s, int solver, const float *theta
     });
 }
 
+int sph_dfsph_set_warm_start_mode(sph_dfsph_t s, int solver, int mode) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_warm_start_mode(solver, mode);
+    });
+}
+
 int sph_dfsph_set_warm_start_policy(sph_dfsph_t s, int solver, int min_prev_iterations) {
     return guarded([&] {
         DCHK(s);

@@ -238,47 +238,73 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const unsigned *__restrict__ 
 // appends its lanes' neighbours into its own LDS tile [cap][64] and then
 // writes the tile row by row: row k of a wave is 64 consecutive words, one
 // coalesced store, instead of scattered 4-byte appends that leave partial
-// cache lines behind. The candidates of the 9 row ranges are read straight
-// from the cell-sorted array (neighbouring queries share them in L1/L2).
+// cache lines behind. Tile entries are 16 bits -- the (dy, dz) row r of the
+// neighbour (4 bits) and its offset from that row's first candidate slot
+// (12 bits), decoded at write-out from the row bases kept in LDS -- which
+// halves the tile and doubles the waves a CU can hold. A lane whose row
+// range exceeds 4096 slots writes its entries straight to the lists.
+// Candidates are read from the cell-sorted array (neighbouring queries share
+// them in L1/L2).
+constexpr unsigned LIST_ROWS = 9;
 template <bool SAME>
 __global__ void k_lists_cm(const float4 *__restrict__ q, unsigned nq, const float4 *__restrict__ src,
                            const unsigned *__restrict__ start, GridDesc g, float r2, unsigned *__restrict__ nbr,
                            unsigned stride, unsigned cap, unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
-    extern __shared__ unsigned tile[];
-    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    unsigned *__restrict__ wt = tile + (size_t)wid * cap * 64u;
+    extern __shared__ unsigned lds_u32[];
+    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    unsigned *__restrict__ rowbase = lds_u32 + (size_t)wid * LIST_ROWS * 64u;  // [9][64] per wave
+    unsigned short *__restrict__ wt =
+        reinterpret_cast<unsigned short *>(lds_u32 + (size_t)waves * LIST_ROWS * 64u) + (size_t)wid * cap * 64u;
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned k = 0;
+    bool wide = false;
     if (i < nq) {
         const float4 xi = q[i];
         const int3 c = cell_of(xi, g);
         const int x0 = c.x > 0 ? c.x - 1 : 0, x1 = c.x < g.nx - 1 ? c.x + 1 : g.nx - 1;
-        for (int dz = -1; dz <= 1; dz++) {
-            const int z = c.z + dz;
-            if (z < 0 || z >= g.nz) continue;
-            for (int dy = -1; dy <= 1; dy++) {
-                const int y = c.y + dy;
-                if (y < 0 || y >= g.ny) continue;
+        unsigned s_r[LIST_ROWS], e_r[LIST_ROWS];
+        unsigned rr = 0;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++, rr++) {
+                const int z = c.z + dz, y = c.y + dy;
+                if (z < 0 || z >= g.nz || y < 0 || y >= g.ny) {
+                    s_r[rr] = e_r[rr] = 0u;
+                    continue;
+                }
                 const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
-                const unsigned s = start[row + x0], e = start[row + x1 + 1];
-                for (unsigned j = s; j < e; j++) {
-                    if (SAME && j == i) continue;
-                    const float4 xj = src[j];
-                    const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
-                    const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
-                    if (d2 < r2) {
-                        if (k < cap) wt[k * 64u + lane] = j;
-                        k++;
+                s_r[rr] = start[row + x0];
+                e_r[rr] = start[row + x1 + 1];
+                if (e_r[rr] - s_r[rr] > 4096u) wide = true;
+            }
+#pragma unroll
+        for (unsigned r = 0; r < LIST_ROWS; r++) {
+            rowbase[r * 64u + lane] = s_r[r];
+            const unsigned s0 = s_r[r], e0 = e_r[r];
+            for (unsigned j = s0; j < e0; j++) {
+                if (SAME && j == i) continue;
+                const float4 xj = src[j];
+                const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
+                const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
+                if (d2 < r2) {
+                    if (k < cap) {
+                        if (wide)
+                            nbr[(size_t)k * stride + i] = j;
+                        else
+                            wt[k * 64u + lane] = (unsigned short)((r << 12) | (j - s0));
                     }
+                    k++;
                 }
             }
         }
         cnt[i] = k;
     }
-    const unsigned kk = k < cap ? k : cap;
+    const unsigned kk = (k < cap && !wide) ? k : (wide ? 0u : cap);
     const unsigned m = wave_max_u(kk);
     for (unsigned r = 0; r < m; r++)
-        if (r < kk) nbr[(size_t)r * stride + i] = wt[r * 64u + lane];
+        if (r < kk) {
+            const unsigned e = wt[r * 64u + lane];
+            nbr[(size_t)r * stride + i] = rowbase[(e >> 12) * 64u + lane] + (e & 4095u);
+        }
     const unsigned mk = wave_max_u(k);
     if (lane == 0 && mk > 0) atomicMax(maxcnt, mk);
 }
@@ -451,11 +477,13 @@ void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const flo
     if (!nq) return;
     SPH_REQUIRE(cap >= 1, SPH_ERR_INVALID, "neighbour capacity out of range");
     TimedScope ts(T, s, name);
-    // waves per block so that the LDS tiles stay within 64 KiB per block
-    const unsigned waves = cap <= 64 ? 4u : (cap <= 128 ? 2u : 1u);
-    const unsigned threads = 64u * waves;
-    const size_t lds = (size_t)waves * cap * 64u * sizeof(unsigned);
-    if (lds > 65536) {
+    // waves per block so that the LDS (16-bit tile + row bases) stays within
+    // 64 KiB per block
+    const size_t per_wave = (size_t)cap * 64u * sizeof(unsigned short) + LIST_ROWS * 64u * sizeof(unsigned);
+    const unsigned waves = (unsigned)std::min<size_t>(4, 65536 / per_wave);
+    const unsigned threads = 64u * std::max(waves, 1u);
+    const size_t lds = (size_t)std::max(waves, 1u) * per_wave;
+    if (waves == 0) {
         if (same_set)
             k_lists_cm_wide<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
         else

@@ -51,7 +51,7 @@ __device__ __forceinline__ f32x16 zero16() {
 }
 
 struct Wregs {
-    bf16x4 B1[2];     // layer 1: K = features (6 + 2 zero), two 32-column halves
+    float w1[2][MLP_NIN];  // layer 1 (FP32): W1[k][32 h + r] of this lane's two columns
     bf16x4 B2[8][2];  // layer 2: 8 K-chunks, two halves
     bf16x4 B3[8];     // layer 3: only column 0 non-zero
     float b1[2], b2[2], w3[2], b3;
@@ -61,10 +61,7 @@ __device__ __forceinline__ void load_w(const float *__restrict__ th, Wregs &R, i
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = 4 * g + j;
-            R.B1[h][j] = (k < MLP_NIN) ? f2bf(th[MLP_OFF_W1 + k * 64 + 32 * h + r]) : (short)0;
-        }
+        for (int k = 0; k < MLP_NIN; k++) R.w1[h][k] = th[MLP_OFF_W1 + k * 64 + 32 * h + r];
 #pragma unroll
     for (int kc = 0; kc < 8; kc++)
 #pragma unroll
@@ -85,20 +82,30 @@ __device__ __forceinline__ void load_w(const float *__restrict__ th, Wregs &R, i
 
 // Forward pass of one 32-row tile. Returns the pre-activations z1, z2 (with
 // bias) in C layout and the output (valid in lanes r == 0). Leaves SiLU(z2)
-// in S. Block-uniform: contains __syncthreads.
+// in S. Layer 1 (K = 6) runs in FP32 on the vector units from the inputs
+// staged in XF, so the features keep their full precision (a normalisation
+// folded into W1 and b1 costs nothing); layers 2 and 3 run on MFMA.
+// Block-uniform: contains __syncthreads.
 __device__ __forceinline__ void fwd_tile(const Wregs &R, const float *__restrict__ X, unsigned n, unsigned row0,
-                                         short *S, int r, int g, f32x16 &z1a, f32x16 &z1b, f32x16 &z2a, f32x16 &z2b,
-                                         f32x16 &yv) {
+                                         short *S, float *XF, int r, int g, f32x16 &z1a, f32x16 &z1b, f32x16 &z2a,
+                                         f32x16 &z2b, f32x16 &yv) {
     const unsigned row = row0 + (unsigned)r;
     float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
     if (row < n) xv = *reinterpret_cast<const float4 *>(X + (size_t)row * MLP_XSTRIDE + 4 * g);
-    bf16x4 a;
-    a[0] = f2bf(xv.x);
-    a[1] = f2bf(xv.y);
-    a[2] = f2bf(xv.z);
-    a[3] = f2bf(xv.w);
-    z1a = mfma(a, R.B1[0], zero16());
-    z1b = mfma(a, R.B1[1], zero16());
+    *reinterpret_cast<float4 *>(XF + r * MLP_XSTRIDE + 4 * g) = xv;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const float *xr = XF + crow(e, g) * MLP_XSTRIDE;
+        float a = 0.f, b = 0.f;
+#pragma unroll
+        for (int k = 0; k < MLP_NIN; k++) {
+            a += xr[k] * R.w1[0][k];
+            b += xr[k] * R.w1[1][k];
+        }
+        z1a[e] = a;
+        z1b[e] = b;
+    }
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         z1a[e] += R.b1[0];
@@ -138,6 +145,7 @@ __device__ __forceinline__ void fwd_tile(const Wregs &R, const float *__restrict
 __global__ __launch_bounds__(64 * MWAVES) void k_mlp_fwd(const float *__restrict__ th, const float *__restrict__ X,
                                                         unsigned n, float *__restrict__ y) {
     __shared__ short sm[MWAVES][32 * SP];
+    __shared__ float xf[MWAVES][32 * MLP_XSTRIDE];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, g = lane >> 5;
     Wregs R;
     load_w(th, R, r, g);
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(64 * MWAVES) void k_mlp_fwd(const float *__restrict
     for (unsigned t0 = blockIdx.x * MWAVES; t0 < ntiles; t0 += gridDim.x * MWAVES) {
         const unsigned row0 = (t0 + wid) * 32u;
         f32x16 z1a, z1b, z2a, z2b, yv;
-        fwd_tile(R, X, n, row0, sm[wid], r, g, z1a, z1b, z2a, z2b, yv);
+        fwd_tile(R, X, n, row0, sm[wid], xf[wid], r, g, z1a, z1b, z2a, z2b, yv);
         if (r == 0)
 #pragma unroll
             for (int e = 0; e < 16; e++) {
@@ -162,6 +170,7 @@ __global__ __launch_bounds__(64 * MWAVES) void k_mlp_grad(const float *__restric
     __shared__ short sm[MWAVES][32 * SP];
     __shared__ short sd[MWAVES][32 * SP];
     __shared__ short sx[MWAVES][32 * MLP_XSTRIDE];
+    __shared__ float xf[MWAVES][32 * MLP_XSTRIDE];
     __shared__ float sy[MWAVES][32];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, g = lane >> 5;
     Wregs R;
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(64 * MWAVES) void k_mlp_grad(const float *__restric
     for (unsigned t0 = blockIdx.x * MWAVES; t0 < ntiles; t0 += gridDim.x * MWAVES) {
         const unsigned row0 = (t0 + wid) * 32u;
         f32x16 z1a, z1b, z2a, z2b, yv;
-        fwd_tile(R, X, n, row0, S, r, g, z1a, z1b, z2a, z2b, yv);
+        fwd_tile(R, X, n, row0, S, xf[wid], r, g, z1a, z1b, z2a, z2b, yv);
         {
             const unsigned row = row0 + (unsigned)r;
             float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -1,10 +1,12 @@
-// mlp.hpp -- the warm-start network: 6 -> 64 -> 64 -> 1 with SiLU, BF16
-// MFMA (v_mfma_f32_32x32x8bf16_1k) with FP32 accumulation.
+// mlp.hpp -- the warm-start network: 6 -> 64 -> 64 -> 1 with SiLU. Layer 1
+// (K = 6) in FP32 on the vector units; layers 2 and 3 on BF16 MFMA
+// (v_mfma_f32_32x32x8bf16_1k) with FP32 accumulation.
 //
 // Parameters are one flat float vector: W1 (6x64, row-major [in][out]), b1
 // (64), W2 (64x64, [in][out]), b2 (64), W3 (64), b3 (1). Inputs are rows of
-// 8 floats (6 features, 2 zero pads). Weights are rounded to BF16 when they
-// are loaded into the MFMA operands; biases and accumulation stay FP32.
+// 8 floats (6 features, 2 zero pads). W2 and W3 are rounded to BF16 when
+// they are loaded into the MFMA operands, as are the layer-1 activations;
+// W1, the biases and all accumulation stay FP32.
 #pragma once
 #include "common.hpp"
 

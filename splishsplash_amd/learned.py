@@ -4,7 +4,8 @@ The reference starts each solve from half its previous-step stiffness
 (TimeStepDFSPH.cpp:292-296 pressure, :449-453 divergence). Here a per-particle
 network (6 -> 64 -> 64 -> 1, SiLU, BF16 MFMA with FP32 accumulation, the
 device kernels in csrc/mlp.hip) predicts the converged stiffness of the
-current step from neighbour-aggregated features, and the first Jacobi
+current step from neighbour-aggregated features (as a change from the
+previous step's stiffness, which the start adds back), and the first Jacobi
 iteration keeps, per particle, whichever start has the smaller initial
 residual. The solve still iterates to the configured tolerance, so the
 prediction only changes how many iterations that takes.
@@ -47,9 +48,11 @@ def init_theta(seed=0):
     return th.astype(np.float32)
 
 
-def make_sim(scene, thetas=None):
+def make_sim(scene, thetas=None, mode=None):
     """A TimeStepDFSPH set up from a scene dict (scenes.py) and initialised;
-    `thetas` maps solver name -> network parameters."""
+    `thetas` maps solver name -> network parameters (warm-start mode
+    "network"); `mode` ("previous", "reference") sets both solvers'
+    warm-start mode instead."""
     sim = capi.TimeStepDFSPH(scene["params"])
     sim.set_fluid(scene["x"], scene["v"])
     if "boundary_x" in scene:
@@ -61,6 +64,9 @@ def make_sim(scene, thetas=None):
     sim.init()
     for name, th in (thetas or {}).items():
         sim.set_warm_start_model(name, th)
+    if mode is not None:
+        for name in SOLVERS:
+            sim.set_warm_start_mode(name, mode)
     return sim
 
 
@@ -100,17 +106,36 @@ def merge(samples):
             for s in SOLVERS}
 
 
-def train(samples, steps=4000, batch=8192, lr=2e-3, seed=0, thetas=None):
-    """Fits one network per solver on the device (mean squared error, Adam).
-    Returns ({solver: theta}, {solver: final loss})."""
+def fold_normalisation(theta, mu, sd):
+    """Parameters for raw inputs x of a network trained on (x - mu) / sd:
+    W1[k][o] / sd[k], b1[o] - sum_k mu[k] W1[k][o] / sd[k] (exact in FP32
+    because layer 1 runs in FP32 on the device)."""
+    th = np.asarray(theta, np.float64).copy()
+    W1 = th[0:384].reshape(6, 64) / np.asarray(sd, np.float64)[:, None]
+    th[0:384] = W1.ravel()
+    th[384:448] -= np.asarray(mu, np.float64) @ W1
+    return th.astype(np.float32)
+
+
+def train(samples, steps=4000, batch=8192, lr=2e-3, seed=0):
+    """Fits one network per solver on the device (mean squared error, Adam)
+    to the change of the stiffness, t - input 2 (the start adds the previous
+    stiffness back), on standardised inputs; the standardisation is folded
+    into the first layer. Returns ({solver: theta}, {solver: final loss})."""
     out, losses = {}, {}
     for k, s in enumerate(SOLVERS):
         x, t = samples[s]
         if len(x) == 0:
             continue
-        th0 = thetas[s] if thetas and s in thetas else init_theta(seed + k)
-        th, loss = capi.mlp_train(th0, x[:, :6], t, steps=steps, batch=min(batch, len(x)), lr=lr, seed=seed + k)
-        out[s], losses[s] = th, loss
+        x6 = np.asarray(x[:, :6], np.float64)
+        mu = x6.mean(0)
+        sd = x6.std(0)
+        sd = np.where(sd > 1e-6, sd, 1.0)
+        xs = ((x6 - mu) / sd).astype(np.float32)
+        ts = (np.asarray(t, np.float64) - x6[:, 2]).astype(np.float32)
+        th, loss = capi.mlp_train(init_theta(seed + k), xs, ts, steps=steps, batch=min(batch, len(x)), lr=lr,
+                                  seed=seed + k)
+        out[s], losses[s] = fold_normalisation(th, mu, sd), loss
     return out, losses
 
 
@@ -127,11 +152,11 @@ def load(path=WEIGHTS):
         return {s: f[f"theta_{s}"].astype(np.float32) for s in SOLVERS if f"theta_{s}" in f}
 
 
-def evaluate(scene, thetas, steps, warmup=0):
+def evaluate(scene, thetas, steps, warmup=0, mode=None):
     """Runs `scene` for warmup + steps steps with the given networks (None =
-    the reference's warm start) and returns iteration and timing figures of
-    the last `steps` steps."""
-    sim = make_sim(scene, thetas)
+    the reference's warm start, or `mode`) and returns iteration and timing
+    figures of the last `steps` steps."""
+    sim = make_sim(scene, thetas, mode)
     if warmup:
         sim.step(warmup)
     c0 = sim.counters()
@@ -156,3 +181,42 @@ def evaluate(scene, thetas, steps, warmup=0):
                 "max_speed": float(np.sqrt(speed2.max()))})
     del sim
     return res
+
+
+FIELDS = ("position", "velocity", "p / rho^2", "p_v / rho^2", "velocity difference")
+
+
+def evaluate_snapshots(scene, arms, warmup=100, snapshots=20, every=5):
+    """Solve-level comparison from identical states: steps `scene` with the
+    reference's start, and at `snapshots` states (every `every` steps after
+    `warmup`) runs one step of each arm ({name: thetas or None}) from that
+    exact state (an arm may also be a warm-start mode name such as
+    "previous"). Returns {name: {"pressure_iterations": mean, ...}}. The
+    arms other than the reference use their start in every solve."""
+    ref = make_sim(scene)
+    sims = {}
+    for name, th in arms.items():
+        if isinstance(th, str):
+            sims[name] = make_sim(scene, None, th)
+        else:
+            sims[name] = make_sim(scene, th)
+        if th:
+            for sv in SOLVERS:
+                sims[name].set_warm_start_policy(sv, 0)
+    ref.step(warmup)
+    acc = {name: {"pressure_iterations": [], "divergence_iterations": [], "avg_overpressure_err": []}
+           for name in arms}
+    for k in range(snapshots):
+        state = {f: ref.get(f) for f in FIELDS}
+        c = ref.counters()
+        for name, sim in sims.items():
+            for f, v in state.items():
+                sim.set(f, v)
+            sim.set_time(c["time"], c["time_step_size"])
+            sim.step(1)
+            cc = sim.counters()
+            acc[name]["pressure_iterations"].append(cc["iterations"])
+            acc[name]["divergence_iterations"].append(cc["iterations_v"])
+            acc[name]["avg_overpressure_err"].append(cc["avg_overpressure_err"])
+        ref.step(every)
+    return {name: {key: float(np.mean(v)) for key, v in d.items()} for name, d in acc.items()}

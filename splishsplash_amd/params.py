@@ -76,6 +76,7 @@ class DFSPHCounters(C.Structure):
         ("lanes_per_particle", C.c_int),
         ("avg_overpressure_err", C.c_double),
         ("avg_overpressure_err_v", C.c_double),
+        ("host_syncs", C.c_longlong),
     ]
 
     def as_dict(self):

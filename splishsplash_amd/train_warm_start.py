@@ -42,7 +42,10 @@ def collect_variant(seed, skip, steps, samples_per_step_target, per_side=None):
 def evaluate_pair(sc, thetas, warmup, steps):
     plain = learned.evaluate(sc, None, steps=steps, warmup=warmup)
     lrn = learned.evaluate(sc, thetas, steps=steps, warmup=warmup)
-    return plain, lrn
+    # the previous step's full stiffness as the start (no network): what the
+    # network adds beyond that
+    prev = learned.evaluate(sc, None, steps=steps, warmup=warmup, mode="previous")
+    return plain, lrn, prev
 
 
 def main(argv=None):
@@ -76,9 +79,10 @@ def main(argv=None):
     fit = {}
     for s in learned.SOLVERS:
         x, t = samples[s]
-        y = capi.mlp_forward(thetas[s], x[:, :6])
-        fit[s] = {"mse": float(np.mean((y - t) ** 2)), "target_var": float(np.var(t)),
-                  "reference_start_mse": float(np.mean((t - 0.5 * x[:, 2]) ** 2)), "samples": int(len(t))}
+        y = capi.mlp_forward(thetas[s], x[:, :6]) + x[:, 2]
+        fit[s] = {"mse": float(np.mean((np.maximum(y, 0) - t) ** 2)), "target_var": float(np.var(t)),
+                  "reference_start_mse": float(np.mean((t - 0.5 * x[:, 2]) ** 2)),
+                  "previous_stiffness_mse": float(np.mean((t - x[:, 2]) ** 2)), "samples": int(len(t))}
     learned.save(args.out, thetas, train_seeds=np.arange(args.train), epochs=args.epochs, lr=args.lr)
     log(f"saved {args.out}")
 
@@ -86,20 +90,27 @@ def main(argv=None):
     for k in range(args.heldout):
         seed = 1000 + k
         sc = scenes.dam_break_variant(seed, per_side=args.eval_per_side)
-        plain, lrn = evaluate_pair(sc, thetas, args.eval_warmup, args.eval_steps)
+        plain, lrn, prev = evaluate_pair(sc, thetas, args.eval_warmup, args.eval_steps)
+        snap = learned.evaluate_snapshots(sc, {"plain": None, "learned": thetas, "previous": "previous"},
+                                          warmup=args.eval_warmup, snapshots=15, every=10)
         v = sc["variant"]
         rec = {"seed": seed, "particles": len(sc["x"]), "aspect": v["aspect"], "spheres": len(v["spheres"]),
-               "plain": plain, "learned": lrn}
+               "plain": plain, "learned": lrn, "previous_stiffness_start": prev, "from_identical_states": snap}
         evals.append(rec)
         log(f"held-out seed {seed}: pressure it {plain['pressure_iterations']:.2f} -> "
-            f"{lrn['pressure_iterations']:.2f}, divergence it {plain['divergence_iterations']:.2f} -> "
-            f"{lrn['divergence_iterations']:.2f}, ms/step {plain['ms_per_step']:.3f} -> {lrn['ms_per_step']:.3f}")
+            f"{lrn['pressure_iterations']:.2f} (previous-stiffness start {prev['pressure_iterations']:.2f}), "
+            f"divergence it {plain['divergence_iterations']:.2f} -> {lrn['divergence_iterations']:.2f}, "
+            f"ms/step {plain['ms_per_step']:.3f} -> {lrn['ms_per_step']:.3f} ({prev['ms_per_step']:.3f})")
     summary = {}
     if evals:
         for key in ("pressure_iterations", "divergence_iterations", "ms_per_step"):
             p = np.mean([e["plain"][key] for e in evals])
             q = np.mean([e["learned"][key] for e in evals])
-            summary[key] = {"plain": p, "learned": q, "reduction": 1.0 - q / p}
+            z = np.mean([e["previous_stiffness_start"][key] for e in evals])
+            summary[key] = {"plain": p, "learned": q, "reduction": 1.0 - q / p, "previous_stiffness_start": z}
+        summary["pressure_iterations_from_identical_states"] = {
+            arm: float(np.mean([e["from_identical_states"][arm]["pressure_iterations"] for e in evals]))
+            for arm in ("plain", "learned", "previous")}
     report = {"fit": fit, "heldout": evals, "summary": summary,
               "settings": {k: v for k, v in vars(args).items() if k not in ("out", "report")}}
     if args.report:

@@ -130,3 +130,14 @@ def test_packaged_networks_gain_on_heldout_variant():
     assert lrn["flags"] == 0 and lrn["avg_density_err"] <= eta
     assert lrn["pressure_iterations"] < 0.8 * plain["pressure_iterations"], (plain, lrn)
     assert abs(lrn["sim_time"] - plain["sim_time"]) < 0.05 * plain["sim_time"]
+
+
+def test_previous_stiffness_mode_converges_with_fewer_iterations():
+    # Warm-start mode 1 (the previous step's full stiffness, no network):
+    # same tolerance on both sides, fewer pressure iterations than the
+    # reference's halved and clamped start, from identical states.
+    sc = scenes.dam_break_variant(4243, per_side=40, n_spheres=2)
+    r = learned.evaluate_snapshots(sc, {"plain": None, "previous": "previous"}, warmup=80, snapshots=8, every=5)
+    assert r["previous"]["pressure_iterations"] < r["plain"]["pressure_iterations"], r
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    assert r["previous"]["avg_overpressure_err"] <= eta

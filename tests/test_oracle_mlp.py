@@ -27,3 +27,16 @@ def test_forward_shapes_and_silu():
     y, cache = M.forward(theta, np.zeros((5, 8)))
     assert y.shape == (5,)
     assert abs(M.silu(0.0)) == 0.0 and abs(M.dsilu(0.0) - 0.5) < 1e-15
+
+
+def test_folded_normalisation_is_exact():
+    # Training runs on standardised inputs; folding (mu, sd) into layer 1
+    # must give the same network on raw inputs.
+    from splishsplash_amd.learned import fold_normalisation
+    rng = np.random.default_rng(11)
+    th = M.init(3)
+    x = rng.normal(0.0, 1.0, (64, 6)) * [9.0, 8.0, 0.1, 0.1, 4.0, 0.16] + [-10.0, -9.0, 0.05, 0.05, 1.5, 0.8]
+    mu, sd = x.mean(0), x.std(0)
+    y_std, _ = M.forward(th, (x - mu) / sd)
+    y_raw, _ = M.forward(fold_normalisation(th, mu, sd).astype(np.float64), x)
+    np.testing.assert_allclose(y_raw, y_std, rtol=2e-5, atol=2e-5)
