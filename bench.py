@@ -59,13 +59,14 @@ BYTES = {
 
 # bench kernel names -> device kernel symbols in the rocprofv3 traffic record
 SYMBOL = {"jacobi": "k_jacobi", "jacobi_v": "k_jacobi", "pressure_accel": "k_accel", "pressure_accel_v": "k_accel",
-          "fluid_lists": "k_lists_cm"}
+          "fluid_lists": "k_lists_cm", "learn_jacobi2": "k_jacobi2", "learn_accel2": "k_accel2",
+          "density_factor_divsetup": "k_sweep_init"}
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     record (FETCH_SIZE/WRITE_SIZE scaled by a copy calibration; the commands
-    are in profiles/dam1m_traffic_k_jacobi.md), or None."""
+    are in profiles/dam1m_traffic_sweeps.md), or None."""
     path = os.path.join(ROOT, "profiles", "dam1m_traffic.json")
     try:
         rec = json.load(open(path))
@@ -235,6 +236,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--plain-only", action="store_true", help="skip the learned warm-start run")
+    ap.add_argument("--paths", default="plain,learned,previous",
+                    help="comma list of the runs: plain (reference start), learned (network), previous "
+                         "(previous-stiffness start); the first of learned/plain present gives `value`")
     ap.add_argument("--weights", default=None, help="warm-start networks (default: the packaged ones)")
     ap.add_argument("--workload", default="dam1m", choices=["dam1m", "ns"],
                     help="ns: the 4M-point neighbour-search + density microbenchmark")
@@ -266,25 +270,35 @@ def main():
     eta = sc["params"].max_error * 0.01 * sc["params"].density0
     log(f"bench: {n} particles, volume map {list(sc['vm_res'])} cells, {args.warmup} warm-up + "
         f"{args.steps} timed steps per path")
-    t0 = time.perf_counter()
-    plain, k_plain, live_plain, K = run_path(sc, None, args.warmup, args.steps, args.breakdown_steps)
-    log(f"bench: plain {plain['ms_per_step']:.3f} ms/step, {plain['pressure_iterations_per_step']:.2f} pressure "
-        f"iterations/step ({time.perf_counter() - t0:.1f} s)")
-    lrn = k_lrn = live_lrn = None
-    if not args.plain_only:
+    paths = set(args.paths.split(","))
+    if args.plain_only:
+        paths = {"plain"}
+    plain = k_plain = live_plain = lrn = k_lrn = live_lrn = None
+    K = 0.0
+    if "plain" in paths:
+        t0 = time.perf_counter()
+        plain, k_plain, live_plain, K = run_path(sc, None, args.warmup, args.steps, args.breakdown_steps)
+        log(f"bench: plain {plain['ms_per_step']:.3f} ms/step, {plain['pressure_iterations_per_step']:.2f} "
+            f"pressure iterations/step ({time.perf_counter() - t0:.1f} s)")
+    if "learned" in paths:
         thetas = learned.load(args.weights or learned.WEIGHTS)
         t0 = time.perf_counter()
-        lrn, k_lrn, live_lrn, _ = run_path(sc, thetas, args.warmup, args.steps, args.breakdown_steps)
+        lrn, k_lrn, live_lrn, K = run_path(sc, thetas, args.warmup, args.steps, args.breakdown_steps)
         log(f"bench: learned {lrn['ms_per_step']:.3f} ms/step, {lrn['pressure_iterations_per_step']:.2f} pressure "
             f"iterations/step ({time.perf_counter() - t0:.1f} s)")
     prev = None
-    if not args.plain_only:
+    if "previous" in paths:
         # the previous step's full stiffness as the start (warm-start mode 1,
         # no network): what the network adds beyond that
         prev, _, _, _ = run_path(sc, None, args.warmup, args.steps, 0, mode="previous")
         log(f"bench: previous-stiffness start {prev['ms_per_step']:.3f} ms/step, "
             f"{prev['pressure_iterations_per_step']:.2f} pressure iterations/step")
-    head, kernels, live = (lrn, k_lrn, live_lrn) if lrn else (plain, k_plain, live_plain)
+    if lrn:
+        head, kernels, live = lrn, k_lrn, live_lrn
+    elif plain:
+        head, kernels, live = plain, k_plain, live_plain
+    else:
+        head, kernels, live = prev, {}, None
 
     roofline = None
     if live:
@@ -335,7 +349,8 @@ def main():
         "learned_vs_plain": ({"speedup": lrn["value"] / plain["value"],
                               "pressure_iteration_cut": 1.0 - lrn["pressure_iterations_per_step"]
                               / max(plain["pressure_iterations_per_step"], 1e-30),
-                              "step_time_cut": 1.0 - lrn["ms_per_step"] / plain["ms_per_step"]} if lrn else None),
+                              "step_time_cut": 1.0 - lrn["ms_per_step"] / plain["ms_per_step"]}
+                             if lrn and plain else None),
         "mean_fluid_neighbors": K,
         "kernels": kernels,
         "kernels_plain": k_plain if lrn else None,

@@ -1,0 +1,80 @@
+"""HBM traffic per kernel launch from rocprofv3 PMC databases.
+
+    python -m splishsplash_amd.profiling FETCH.db WRITE.db COPY_FETCH.db COPY_WRITE.db \
+        --copy-bytes 1073741824 --workload "..." -o profiles/dam1m_traffic.json
+
+FETCH.db / WRITE.db are `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
+runs of the workload; COPY_*.db the same counters for a plain device copy of
+--copy-bytes bytes, which calibrates KiB-per-dispatch into bytes (the copy
+reads and writes exactly that many bytes). Per kernel name the record holds
+the mean counter values per dispatch, the mean duration, and the HBM bytes
+per launch = FETCH x fetch_scale + WRITE x write_scale, over the dispatches
+longer than --min-us (a converged solve's remaining launches return at
+entry and would dilute the means).
+"""
+import argparse
+import json
+import sqlite3
+from collections import defaultdict
+
+
+def per_kernel(db, counter, min_us=0.0):
+    """Mean counter value and duration per kernel name over the dispatches
+    that ran longer than `min_us` (drops the launches of a converged solve,
+    which return at entry)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, value, duration from counters_collection where counter_name = ?",
+                       (counter,)).fetchall()
+    con.close()
+    acc = defaultdict(lambda: [0.0, 0.0, 0])
+    for name, value, dur in rows:
+        if dur < min_us * 1e3:
+            continue
+        a = acc[name]
+        a[0] += value
+        a[1] += dur
+        a[2] += 1
+    return {k: {"value": v[0] / v[2], "duration_ns": v[1] / v[2], "dispatches": v[2]} for k, v in acc.items()}
+
+
+def copy_value(db, counter):
+    rec = per_kernel(db, counter)
+    # the copy is the dispatch with the largest counter value
+    return max(v["value"] for v in rec.values())
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("fetch")
+    ap.add_argument("write")
+    ap.add_argument("copy_fetch")
+    ap.add_argument("copy_write")
+    ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    ap.add_argument("--workload", default="")
+    ap.add_argument("--min-us", type=float, default=10.0,
+                    help="ignore dispatches shorter than this (early exits of converged solves)")
+    ap.add_argument("-o", "--out", required=True)
+    a = ap.parse_args(argv)
+    cf, cw = copy_value(a.copy_fetch, "FETCH_SIZE"), copy_value(a.copy_write, "WRITE_SIZE")
+    fs, ws = a.copy_bytes / (cf * 1024.0), a.copy_bytes / (cw * 1024.0)
+    fe, wr = per_kernel(a.fetch, "FETCH_SIZE", a.min_us), per_kernel(a.write, "WRITE_SIZE", a.min_us)
+    kernels = {}
+    for name in sorted(set(fe) & set(wr)):
+        b = fe[name]["value"] * 1024.0 * fs + wr[name]["value"] * 1024.0 * ws
+        dur = fe[name]["duration_ns"]
+        kernels[name] = {"FETCH_SIZE_KiB": fe[name]["value"], "WRITE_SIZE_KiB": wr[name]["value"],
+                         "dispatches": fe[name]["dispatches"], "avg_duration_ns": dur,
+                         "hbm_bytes_per_launch": b, "hbm_GBs": b / dur}
+    rec = {"workload": a.workload, "min_dispatch_us": a.min_us,
+           "copy_calibration": {"bytes": a.copy_bytes, "FETCH_SIZE_KiB": cf, "WRITE_SIZE_KiB": cw,
+                                "fetch_scale": fs, "write_scale": ws},
+           "kernels": kernels}
+    with open(a.out, "w") as f:
+        json.dump(rec, f, indent=1)
+    for k, v in kernels.items():
+        print(f"{k}: {v['hbm_bytes_per_launch'] / 1e6:.1f} MB/launch, {v['avg_duration_ns'] / 1e3:.1f} us, "
+              f"{v['hbm_GBs']:.0f} GB/s")
+
+
+if __name__ == "__main__":
+    main()
