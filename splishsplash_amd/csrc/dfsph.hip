@@ -55,35 +55,68 @@ __global__ void k_reset(Ctl *C, int which) {
 // TimeStep::computeDensities (TimeStep.cpp:116-169), computeDFSPHFactor
 // (TimeStepDFSPH.cpp:1106-1186), computeDensityChange + divergence warm start
 // (TimeStepDFSPH.cpp:398-463, 1247-1295).
-template <bool DENS, bool FACT, bool DIV>
+template <int LPP>
+__device__ __forceinline__ float lane_sum(float v);
+
+template <bool DENS, bool FACT, bool DIV, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
-    const unsigned i = gtid();
-    if (i >= D.n) return;
-    const float4 xi = D.x[i];
-    const float4 vi = DIV ? D.v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float dens = D.V * D.K.W0;
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+    float dens = (LPP == 1) ? D.V * D.K.W0 : 0.f;
     float sgk = 0.f, gpx = 0.f, gpy = 0.f, gpz = 0.f, div = 0.f, divb = 0.f;
-    const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float4 xj = D.x[j];
-        float3 r;
-        float w, g;
-        kernel_WG(xi, xj, D.K, r, w, g);
-        if (DENS) dens += D.V * w;
-        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
-        if (FACT) {
-            const float qx = -D.V * gx, qy = -D.V * gy, qz = -D.V * gz;
-            sgk += sqn(qx, qy, qz);
-            gpx -= qx;
-            gpy -= qy;
-            gpz -= qz;
+    unsigned cnt = 0;
+    if (act) {
+        xi = D.x[i];
+        if (DIV) vi = D.v[i];
+        cnt = D.cf[i];
+        auto pair = [&](const float4 &xj, const float4 &vj) {
+            float3 r;
+            float w, g;
+            kernel_WG(xi, xj, D.K, r, w, g);
+            if (DENS) dens += D.V * w;
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            if (FACT) {
+                const float qx = -D.V * gx, qy = -D.V * gy, qz = -D.V * gz;
+                sgk += sqn(qx, qy, qz);
+                gpx -= qx;
+                gpy -= qy;
+                gpz -= qz;
+            }
+            if (DIV) div += ((vi.x - vj.x) * gx + (vi.y - vj.y) * gy) + (vi.z - vj.z) * gz;
+        };
+        const unsigned *__restrict__ L = D.nf + i;
+        const size_t S = D.stride;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        unsigned k = hl;
+        for (; k + 3 * LPP < cnt; k += 4 * LPP) {
+            const unsigned j0 = L[k * S], j1 = L[(k + LPP) * S], j2 = L[(k + 2 * LPP) * S],
+                           j3 = L[(k + 3 * LPP) * S];
+            const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+            const float4 v0 = DIV ? D.v[j0] : z4, v1 = DIV ? D.v[j1] : z4, v2 = DIV ? D.v[j2] : z4,
+                         v3 = DIV ? D.v[j3] : z4;
+            pair(x0, v0);
+            pair(x1, v1);
+            pair(x2, v2);
+            pair(x3, v3);
         }
-        if (DIV) {
-            const float4 vj = D.v[j];
-            div += ((vi.x - vj.x) * gx + (vi.y - vj.y) * gy) + (vi.z - vj.z) * gz;
+        for (; k < cnt; k += LPP) {
+            const unsigned j = L[k * S];
+            pair(D.x[j], DIV ? D.v[j] : z4);
         }
     }
+    if (LPP > 1) {
+        if (DENS) dens = D.V * D.K.W0 + lane_sum<LPP>(dens);
+        if (FACT) {
+            sgk = lane_sum<LPP>(sgk);
+            gpx = lane_sum<LPP>(gpx);
+            gpy = lane_sum<LPP>(gpy);
+            gpz = lane_sum<LPP>(gpz);
+        }
+        if (DIV) div = lane_sum<LPP>(div);
+    }
+    if (!act || hl != 0) return;
     unsigned nnb = cnt;
     if (D.bmode == 1) {
         const unsigned cb = D.cb[i];
@@ -1629,19 +1662,38 @@ void DFSPH::volume_maps() {
         SPH_HIP(hipGetLastError());                               \
     } while (0)
 
-void DFSPH::sweep_init(bool dens, bool fact, bool div) {
+template <int LPP>
+void DFSPH::sweep_init_l(bool dens, bool fact, bool div) {
     Dev D = dev();
     if (div && learn_or_collect(1)) D.kprev = kprev_.get();
+    const unsigned nb = nblocks((size_t)n_ * LPP);
+    const char *name = (dens && fact && div) ? "density_factor_divsetup"
+                       : (dens && fact)      ? "density_factor"
+                       : dens                ? "density"
+                       : fact                ? "factor"
+                                             : "divergence_setup";
+    TimedScope ts(timer, stream, name);
     if (dens && fact && div)
-        LAUNCH("density_factor_divsetup", (k_sweep_init<true, true, true>), D);
+        k_sweep_init<true, true, true, LPP><<<nb, BLOCK, 0, stream>>>(D);
     else if (dens && fact)
-        LAUNCH("density_factor", (k_sweep_init<true, true, false>), D);
+        k_sweep_init<true, true, false, LPP><<<nb, BLOCK, 0, stream>>>(D);
     else if (dens)
-        LAUNCH("density", (k_sweep_init<true, false, false>), D);
+        k_sweep_init<true, false, false, LPP><<<nb, BLOCK, 0, stream>>>(D);
     else if (fact)
-        LAUNCH("factor", (k_sweep_init<false, true, false>), D);
+        k_sweep_init<false, true, false, LPP><<<nb, BLOCK, 0, stream>>>(D);
     else if (div)
-        LAUNCH("divergence_setup", (k_sweep_init<false, false, true>), D);
+        k_sweep_init<false, false, true, LPP><<<nb, BLOCK, 0, stream>>>(D);
+    SPH_HIP(hipGetLastError());
+}
+
+void DFSPH::sweep_init(bool dens, bool fact, bool div) {
+    if (!tuned_ && have_lists_) tune_lanes();
+    if (lpp_ == 4)
+        sweep_init_l<4>(dens, fact, div);
+    else if (lpp_ == 2)
+        sweep_init_l<2>(dens, fact, div);
+    else
+        sweep_init_l<1>(dens, fact, div);
 }
 
 // One Jacobi iteration: accelerations, then the update, with LPP lanes per

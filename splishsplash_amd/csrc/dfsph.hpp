@@ -157,6 +157,8 @@ private:
     void build_lists();
     void volume_maps();
     void sweep_init(bool density, bool factor, bool div_setup);
+    template <int LPP>
+    void sweep_init_l(bool density, bool factor, bool div_setup);
     void div_iteration(bool force);
     void div_finish(bool scale_pv);
     void scale(float *p, int which);

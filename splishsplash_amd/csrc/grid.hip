@@ -234,27 +234,27 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const unsigned *__restrict__ 
     if (k < n) dst[k] = src[perm[k]];
 }
 
-// 27-cell sweep, one thread per query, column-major output. Each wave
-// appends its lanes' neighbours into its own LDS tile [cap][64] and then
-// writes the tile row by row: row k of a wave is 64 consecutive words, one
-// coalesced store, instead of scattered 4-byte appends that leave partial
-// cache lines behind. Tile entries are 16 bits -- the (dy, dz) row r of the
-// neighbour (4 bits) and its offset from that row's first candidate slot
-// (12 bits), decoded at write-out from the row bases kept in LDS -- which
-// halves the tile and doubles the waves a CU can hold. A lane whose row
-// range exceeds 4096 slots writes its entries straight to the lists.
-// Candidates are read from the cell-sorted array (neighbouring queries share
-// them in L1/L2).
+// 27-cell sweep, one thread per query, column-major output (neighbour k of
+// query i at nbr[k * stride + i]). A lane records its hits as one 64-bit
+// mask per (dy, dz) row of candidates -- bit b set = the row's b-th
+// candidate slot is a neighbour -- and the wave then emits the lists row by
+// row: at output row k every lane writes its k-th neighbour, so each store
+// is 64 consecutive words (one coalesced 256-B write) rather than scattered
+// 4-byte appends that leave partial cache lines behind. Masks and row bases
+// sit in LDS (6.75 KiB per wave, independent of the list capacity), so the
+// LDS never limits the resident waves. A lane whose row holds more than 64
+// candidates stores its entries directly as it finds them. The order of a
+// list is row by row, ascending slot within a row.
 constexpr unsigned LIST_ROWS = 9;
 template <bool SAME>
-__global__ void k_lists_cm(const float4 *__restrict__ q, unsigned nq, const float4 *__restrict__ src,
-                           const unsigned *__restrict__ start, GridDesc g, float r2, unsigned *__restrict__ nbr,
-                           unsigned stride, unsigned cap, unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
-    extern __shared__ unsigned lds_u32[];
-    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6, waves = blockDim.x >> 6;
-    unsigned *__restrict__ rowbase = lds_u32 + (size_t)wid * LIST_ROWS * 64u;  // [9][64] per wave
-    unsigned short *__restrict__ wt =
-        reinterpret_cast<unsigned short *>(lds_u32 + (size_t)waves * LIST_ROWS * 64u) + (size_t)wid * cap * 64u;
+__global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q, unsigned nq,
+                                                    const float4 *__restrict__ src,
+                                                    const unsigned *__restrict__ start, GridDesc g, float r2,
+                                                    unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
+                                                    unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+    __shared__ unsigned rowbase[BLOCK / 64][LIST_ROWS][64];
+    __shared__ unsigned long long masks[BLOCK / 64][LIST_ROWS][64];
+    const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     unsigned k = 0;
     bool wide = false;
@@ -274,80 +274,56 @@ __global__ void k_lists_cm(const float4 *__restrict__ q, unsigned nq, const floa
                 const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
                 s_r[rr] = start[row + x0];
                 e_r[rr] = start[row + x1 + 1];
-                if (e_r[rr] - s_r[rr] > 4096u) wide = true;
+                if (e_r[rr] - s_r[rr] > 64u) wide = true;
             }
 #pragma unroll
         for (unsigned r = 0; r < LIST_ROWS; r++) {
-            rowbase[r * 64u + lane] = s_r[r];
             const unsigned s0 = s_r[r], e0 = e_r[r];
-            for (unsigned j = s0; j < e0; j++) {
-                if (SAME && j == i) continue;
-                const float4 xj = src[j];
+            unsigned long long m = 0ull;
+            auto test = [&](unsigned j, const float4 &xj) {
                 const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
                 const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
-                if (d2 < r2) {
-                    if (k < cap) {
-                        if (wide)
-                            nbr[(size_t)k * stride + i] = j;
-                        else
-                            wt[k * 64u + lane] = (unsigned short)((r << 12) | (j - s0));
+                if (d2 < r2 && !(SAME && j == i)) {
+                    if (wide) {
+                        if (k < cap) nbr[(size_t)k * stride + i] = j;
+                    } else {
+                        m |= 1ull << (j - s0);
                     }
                     k++;
                 }
+            };
+            unsigned j = s0;
+            // four candidates per trip: their loads are in flight together
+            for (; j + 4 <= e0; j += 4) {
+                const float4 c0 = src[j], c1 = src[j + 1], c2 = src[j + 2], c3 = src[j + 3];
+                test(j, c0);
+                test(j + 1, c1);
+                test(j + 2, c2);
+                test(j + 3, c3);
             }
+            for (; j < e0; j++) test(j, src[j]);
+            rowbase[wid][r][lane] = s0;
+            masks[wid][r][lane] = m;
         }
         cnt[i] = k;
     }
-    const unsigned kk = (k < cap && !wide) ? k : (wide ? 0u : cap);
-    const unsigned m = wave_max_u(kk);
-    for (unsigned r = 0; r < m; r++)
-        if (r < kk) {
-            const unsigned e = wt[r * 64u + lane];
-            nbr[(size_t)r * stride + i] = rowbase[(e >> 12) * 64u + lane] + (e & 4095u);
+    const unsigned kk = (i < nq && !wide) ? (k < cap ? k : cap) : 0u;
+    const unsigned mx = wave_max_u(kk);
+    unsigned r = 0;
+    unsigned long long cur = kk ? masks[wid][0][lane] : 0ull;
+    for (unsigned t = 0; t < mx; t++) {
+        if (t < kk) {
+            while (cur == 0ull) {
+                r++;
+                cur = masks[wid][r][lane];
+            }
+            const unsigned b = (unsigned)__builtin_ctzll(cur);
+            cur &= cur - 1ull;
+            nbr[(size_t)t * stride + i] = rowbase[wid][r][lane] + b;
         }
+    }
     const unsigned mk = wave_max_u(k);
     if (lane == 0 && mk > 0) atomicMax(maxcnt, mk);
-}
-
-// Same lists for capacities beyond the LDS tile (hundreds of neighbours):
-// entries are stored as they are found; each 4-byte store of a wave then
-// touches up to 64 rows, so this is used only where the tile cannot be.
-template <bool SAME>
-__global__ __launch_bounds__(BLOCK) void k_lists_cm_wide(const float4 *__restrict__ q, unsigned nq,
-                                                         const float4 *__restrict__ src,
-                                                         const unsigned *__restrict__ start, GridDesc g, float r2,
-                                                         unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
-                                                         unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
-    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
-    unsigned k = 0;
-    if (i < nq) {
-        const float4 xi = q[i];
-        const int3 c = cell_of(xi, g);
-        const int x0 = c.x > 0 ? c.x - 1 : 0, x1 = c.x < g.nx - 1 ? c.x + 1 : g.nx - 1;
-        for (int dz = -1; dz <= 1; dz++) {
-            const int z = c.z + dz;
-            if (z < 0 || z >= g.nz) continue;
-            for (int dy = -1; dy <= 1; dy++) {
-                const int y = c.y + dy;
-                if (y < 0 || y >= g.ny) continue;
-                const unsigned row = ((unsigned)z * (unsigned)g.ny + (unsigned)y) * (unsigned)g.nx;
-                const unsigned s = start[row + x0], e = start[row + x1 + 1];
-                for (unsigned j = s; j < e; j++) {
-                    if (SAME && j == i) continue;
-                    const float4 xj = src[j];
-                    const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
-                    const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
-                    if (d2 < r2) {
-                        if (k < cap) nbr[(size_t)k * stride + i] = j;
-                        k++;
-                    }
-                }
-            }
-        }
-        cnt[i] = k;
-    }
-    const unsigned mk = wave_max_u(k);
-    if ((threadIdx.x & 63u) == 0 && mk > 0) atomicMax(maxcnt, mk);
 }
 
 template <bool SAME, bool FILL>
@@ -477,25 +453,10 @@ void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const flo
     if (!nq) return;
     SPH_REQUIRE(cap >= 1, SPH_ERR_INVALID, "neighbour capacity out of range");
     TimedScope ts(T, s, name);
-    // waves per block so that the LDS (16-bit tile + row bases) stays within
-    // 64 KiB per block
-    const size_t per_wave = (size_t)cap * 64u * sizeof(unsigned short) + LIST_ROWS * 64u * sizeof(unsigned);
-    const unsigned waves = (unsigned)std::min<size_t>(4, 65536 / per_wave);
-    const unsigned threads = 64u * std::max(waves, 1u);
-    const size_t lds = (size_t)std::max(waves, 1u) * per_wave;
-    if (waves == 0) {
-        if (same_set)
-            k_lists_cm_wide<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
-        else
-            k_lists_cm_wide<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
-        SPH_HIP(hipGetLastError());
-        return;
-    }
-    const unsigned grid = (nq + threads - 1) / threads;
     if (same_set)
-        k_lists_cm<true><<<grid, threads, lds, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
     else
-        k_lists_cm<false><<<grid, threads, lds, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
     SPH_HIP(hipGetLastError());
 }
 
