@@ -191,28 +191,9 @@ __device__ __forceinline__ void accel_pair(const Dev &D, const float4 &xi, float
     }
 }
 
-__device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float *__restrict__ P) {
-    const float4 xi = D.x[i];
-    const float pi = P[i];
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    const unsigned cnt = D.cf[i];
-    const unsigned *__restrict__ L = D.nf + i;
-    const size_t S = D.stride;
-    unsigned k = 0;
-    // four neighbours per trip: the index and pressure loads of all four are
-    // issued before the first use, so each wave keeps more gathers in flight
-    for (; k + 4 <= cnt; k += 4) {
-        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
-        const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
-        accel_pair(D, xi, pi, j0, p0, ax, ay, az);
-        accel_pair(D, xi, pi, j1, p1, ax, ay, az);
-        accel_pair(D, xi, pi, j2, p2, ax, ay, az);
-        accel_pair(D, xi, pi, j3, p3, ax, ay, az);
-    }
-    for (; k < cnt; k++) {
-        const unsigned j = L[k * S];
-        accel_pair(D, xi, pi, j, P[j], ax, ay, az);
-    }
+// Boundary part of computePressureAccel for particle i (TimeStepDFSPH.cpp:1330-1365).
+__device__ __forceinline__ void accel_boundary(const Dev &D, unsigned i, const float4 &xi, float pi, float3 &acc) {
+    float ax = acc.x, ay = acc.y, az = acc.z;
     if (fabsf(pi) > D.eps) {
         if (D.bmode == 1) {
             const unsigned cb = D.cb[i];
@@ -236,7 +217,34 @@ __device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float 
             }
         }
     }
-    return make_float3(ax, ay, az);
+    acc = make_float3(ax, ay, az);
+}
+
+__device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float *__restrict__ P) {
+    const float4 xi = D.x[i];
+    const float pi = P[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const unsigned cnt = D.cf[i];
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    unsigned k = 0;
+    // four neighbours per trip: the index and pressure loads of all four are
+    // issued before the first use, so each wave keeps more gathers in flight
+    for (; k + 4 <= cnt; k += 4) {
+        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
+        accel_pair(D, xi, pi, j0, p0, ax, ay, az);
+        accel_pair(D, xi, pi, j1, p1, ax, ay, az);
+        accel_pair(D, xi, pi, j2, p2, ax, ay, az);
+        accel_pair(D, xi, pi, j3, p3, ax, ay, az);
+    }
+    for (; k < cnt; k++) {
+        const unsigned j = L[k * S];
+        accel_pair(D, xi, pi, j, P[j], ax, ay, az);
+    }
+    float3 a = make_float3(ax, ay, az);
+    accel_boundary(D, i, xi, pi, a);
+    return a;
 }
 
 // Neighbour sums of the solver iterations may be split over LPP adjacent
@@ -250,6 +258,24 @@ __device__ __forceinline__ float lane_sum(float v) {
     if (LPP >= 4) v += __shfl_xor(v, 2, 64);
     if (LPP >= 8) v += __shfl_xor(v, 4, 64);
     return v;
+}
+
+// Calls f(j) for the fluid neighbours of particle i taken by lane hl of the
+// particle's LPP lanes (neighbours hl, hl + LPP, ...), four indices loaded
+// per trip.
+template <int LPP, class F>
+__device__ __forceinline__ void fluid_nbrs(const Dev &D, unsigned i, unsigned hl, unsigned cnt, F &&f) {
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    unsigned k = hl;
+    for (; k + 3 * LPP < cnt; k += 4 * LPP) {
+        const unsigned j0 = L[k * S], j1 = L[(k + LPP) * S], j2 = L[(k + 2 * LPP) * S], j3 = L[(k + 3 * LPP) * S];
+        f(j0);
+        f(j1);
+        f(j2);
+        f(j3);
+    }
+    for (; k < cnt; k += LPP) f(L[k * S]);
 }
 
 template <bool PRESSURE, int LPP>
@@ -457,12 +483,32 @@ __global__ __launch_bounds__(RBLOCK) void k_jacobi_reduce(Dev D, unsigned nparts
 
 // divergenceSolve / pressureSolve finalisation (TimeStepDFSPH.cpp:345-383,
 // 500-540): v += h a_p; the divergence solve also restores alpha *= h.
-template <bool PRESSURE>
+template <bool PRESSURE, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_finish(Dev D) {
-    const unsigned i = gtid();
-    if (i >= D.n) return;
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
+    const float *__restrict__ P = PRESSURE ? D.p : D.pv;
+    float3 a = make_float3(0.f, 0.f, 0.f);
+    float pi = 0.f;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+        xi = D.x[i];
+        pi = P[i];
+        if (LPP == 1) {
+            a = accel_i(D, i, P);
+        } else {
+            fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) { accel_pair(D, xi, pi, j, P[j], a.x, a.y, a.z); });
+        }
+    }
+    if (LPP > 1) {
+        a.x = lane_sum<LPP>(a.x);
+        a.y = lane_sum<LPP>(a.y);
+        a.z = lane_sum<LPP>(a.z);
+        if (act && hl == 0) accel_boundary(D, i, xi, pi, a);
+    }
+    if (!act || hl != 0) return;
     const float h = (float)D.ctl->h;
-    const float3 a = accel_i(D, i, PRESSURE ? D.p : D.pv);
     D.ap[i] = make_float4(a.x, a.y, a.z, 0.f);
     float4 v = D.v[i];
     v.x += h * a.x;
@@ -484,63 +530,71 @@ __global__ __launch_bounds__(BLOCK) void k_scale(Dev D, float *q, int which) {
 // clearAccelerations (TimeStep.cpp:35-50), Viscosity_Weiler2018::applyForces
 // (:928-940), XSPH::step (XSPH.cpp:138-216), updateTimeStepSizeCFL
 // (Simulation.cpp:415-493) and v += h a (TimeStepDFSPH.cpp:192-208).
-template <bool CLEAR, bool APPLY, bool XSPH, bool CFL, bool VUPD, bool SCALEPV>
+template <bool CLEAR, bool APPLY, bool XSPH, bool CFL, bool VUPD, bool SCALEPV, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
     __shared__ double lds[BLOCK / 64];
     Ctl *C = D.ctl;
-    const unsigned i = gtid();
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
     const double h = C->h;
     const float hf = (float)h;
+    const float invH = (float)(1.0 / h);
     double vm = 0.0;
-    if (i < D.n) {
-        const float4 vi = D.v[i];
+    float4 vi = make_float4(0.f, 0.f, 0.f, 0.f), xi = vi;
+    float sx = 0.f, sy = 0.f, sz = 0.f;  // XSPH fluid sum
+    if (act) {
+        vi = D.v[i];
+        if (XSPH) {
+            xi = D.x[i];
+            if (D.xsph != 0.f)
+                fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
+                    const float4 vj = D.v[j];
+                    const float w = kernel_W(xi, D.x[j], D.K);
+                    const float s = invH * D.xsph * (D.mass / D.rho[j]);
+                    sx -= s * (vi.x - vj.x) * w;
+                    sy -= s * (vi.y - vj.y) * w;
+                    sz -= s * (vi.z - vj.z) * w;
+                });
+        }
+    }
+    if (XSPH && LPP > 1) {
+        sx = lane_sum<LPP>(sx);
+        sy = lane_sum<LPP>(sy);
+        sz = lane_sum<LPP>(sz);
+    }
+    if (act && hl == 0) {
         float4 ai = CLEAR ? make_float4(D.grav.x, D.grav.y, D.grav.z, 0.f) : D.a[i];
         if (APPLY) {
             float4 xs = D.cgx[i];
             if (C->cg_done == 2) xs = make_float4(0.f, 0.f, 0.f, 0.f);
             const float dx = xs.x - vi.x, dy = xs.y - vi.y, dz = xs.z - vi.z;
-            const float ih = (float)(1.0 / h);
-            ai.x += ih * dx;
-            ai.y += ih * dy;
-            ai.z += ih * dz;
+            ai.x += invH * dx;
+            ai.y += invH * dy;
+            ai.z += invH * dz;
             D.vdiff[i] = make_float4(dx, dy, dz, 0.f);
         }
         if (XSPH) {
-            const float4 xi = D.x[i];
-            const float invH = (float)(1.0 / h);
-            if (D.xsph != 0.f) {
-                const unsigned cnt = D.cf[i];
-                for (unsigned k = 0; k < cnt; k++) {
-                    const unsigned j = D.nf[(size_t)k * D.stride + i];
-                    const float4 vj = D.v[j];
-                    const float w = kernel_W(xi, D.x[j], D.K);
-                    const float s = invH * D.xsph * (D.mass / D.rho[j]);
-                    ai.x -= s * (vi.x - vj.x) * w;
-                    ai.y -= s * (vi.y - vj.y) * w;
-                    ai.z -= s * (vi.z - vj.z) * w;
-                }
-            }
+            ai.x += sx;
+            ai.y += sy;
+            ai.z += sz;
             if (D.xsphB != 0.f) {
                 const float rhoi = D.rho[i];
+                auto bterm = [&](const float4 &xb) {
+                    const float w = kernel_W(xi, xb, D.K);
+                    const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
+                    ai.x += s * vi.x * w;
+                    ai.y += s * vi.y * w;
+                    ai.z += s * vi.z * w;
+                };
                 if (D.bmode == 1) {
                     const unsigned cb = D.cb[i];
-                    for (unsigned k = 0; k < cb; k++) {
-                        const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
-                        const float w = kernel_W(xi, xb, D.K);
-                        const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
-                        ai.x += s * vi.x * w;
-                        ai.y += s * vi.y * w;
-                        ai.z += s * vi.z * w;
-                    }
+                    for (unsigned k = 0; k < cb; k++) bterm(D.bx[D.nb[(size_t)k * D.stride + i]]);
                 } else if (D.bmode == 2) {
                     for (unsigned o = 0; o < D.nvm; o++) {
                         const float4 xb = D.vmb[(size_t)o * D.stride + i];
                         if (!(xb.w > 0.f)) break;
-                        const float w = kernel_W(xi, xb, D.K);
-                        const float s = -invH * D.xsphB * (D.rho0 * xb.w / rhoi);
-                        ai.x += s * vi.x * w;
-                        ai.y += s * vi.y * w;
-                        ai.z += s * vi.z * w;
+                        bterm(xb);
                     }
                 }
             }
@@ -594,20 +648,26 @@ __global__ __launch_bounds__(RBLOCK) void k_cfl_reduce(Dev D, unsigned nparts) {
 
 // ------------------------------------------------------- pressure set-up
 // computeDensityAdv + warm start (TimeStepDFSPH.cpp:264-309, 1191-1242).
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
-    const unsigned i = gtid();
-    if (i >= D.n) return;
-    const double h = D.ctl->h;
-    const float4 xi = D.x[i], vi = D.v[i];
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
     float delta = 0.f;
-    const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float4 vj = D.v[j];
-        float3 r;
-        const float g = kernel_G(xi, D.x[j], D.K, r);
-        delta += ((vi.x - vj.x) * (g * r.x) + (vi.y - vj.y) * (g * r.y)) + (vi.z - vj.z) * (g * r.z);
+    if (act) {
+        xi = D.x[i];
+        vi = D.v[i];
+        fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
+            const float4 vj = D.v[j];
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            delta += ((vi.x - vj.x) * (g * r.x) + (vi.y - vj.y) * (g * r.y)) + (vi.z - vj.z) * (g * r.z);
+        });
     }
+    if (LPP > 1) delta = lane_sum<LPP>(delta);
+    if (!act || hl != 0) return;
+    const double h = D.ctl->h;
     delta *= D.V;
     if (D.bmode == 1) {
         const unsigned cb = D.cb[i];
@@ -696,27 +756,34 @@ __device__ __forceinline__ void ortho(const float3 &n, float3 &t1, float3 &t2) {
     t2 = make_float3(t2.x / l2, t2.y / l2, t2.z / l2);
 }
 
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_weiler_diag(Dev D) {
-    const unsigned i = gtid();
-    if (i >= D.n) return;
-    const float4 xi = D.x[i];
-    const float rhoi = D.rho[i];
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
     const float dd = 10.0f;
     float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        float3 r;
-        const float g = kernel_G(xi, D.x[j], D.K, r);
-        const float s = dd * D.mu * (D.mass / D.rho[j]) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
-        m[0] += s * (gx * r.x);
-        m[1] += s * (gx * r.y);
-        m[2] += s * (gx * r.z);
-        m[3] += s * (gy * r.y);
-        m[4] += s * (gy * r.z);
-        m[5] += s * (gz * r.z);
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+        xi = D.x[i];
+        fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float s = dd * D.mu * (D.mass / D.rho[j]) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+            m[0] += s * (gx * r.x);
+            m[1] += s * (gx * r.y);
+            m[2] += s * (gx * r.z);
+            m[3] += s * (gy * r.y);
+            m[4] += s * (gy * r.z);
+            m[5] += s * (gz * r.z);
+        });
     }
+    if (LPP > 1)
+#pragma unroll
+        for (int q = 0; q < 6; q++) m[q] = lane_sum<LPP>(m[q]);
+    if (!act || hl != 0) return;
+    const float rhoi = D.rho[i];
     if (D.mub != 0.f) {
         if (D.bmode == 1) {
             const unsigned cb = D.cb[i];
@@ -781,24 +848,35 @@ __device__ __forceinline__ float3 minv_apply(const Dev &D, unsigned i, const flo
 }
 
 // matrixVecProd (Viscosity_Weiler2018.cpp:271-441): (A vec)_i.
-__device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, const float4 *__restrict__ vec) {
-    const float4 xi = D.x[i];
-    const float4 vi = vec[i];
-    const float rhoi = D.rho[i];
+// For LPP > 1 every lane of the particle calls this (the fluid sum is split
+// over them); the result is valid in lane hl == 0 only.
+template <int LPP>
+__device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, unsigned hl, bool act,
+                                            const float4 *__restrict__ vec) {
     const float dd = 10.0f;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    const unsigned cnt = D.cf[i];
-    for (unsigned k = 0; k < cnt; k++) {
-        const unsigned j = D.nf[(size_t)k * D.stride + i];
-        const float4 vj = vec[j];
-        float3 r;
-        const float g = kernel_G(xi, D.x[j], D.K, r);
-        const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
-        const float s = dd * D.mu * (D.mass / D.rho[j]) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-        ax += s * (g * r.x);
-        ay += s * (g * r.y);
-        az += s * (g * r.z);
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), vi = xi;
+    if (act) {
+        xi = D.x[i];
+        vi = vec[i];
+        fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
+            const float4 vj = vec[j];
+            float3 r;
+            const float g = kernel_G(xi, D.x[j], D.K, r);
+            const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
+            const float s = dd * D.mu * (D.mass / D.rho[j]) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+            ax += s * (g * r.x);
+            ay += s * (g * r.y);
+            az += s * (g * r.z);
+        });
     }
+    if (LPP > 1) {
+        ax = lane_sum<LPP>(ax);
+        ay = lane_sum<LPP>(ay);
+        az = lane_sum<LPP>(az);
+    }
+    if (!act || hl != 0) return make_float3(0.f, 0.f, 0.f);
+    const float rhoi = D.rho[i];
     if (D.mub != 0.f) {
         if (D.bmode == 1) {
             const unsigned cb = D.cb[i];
@@ -847,12 +925,15 @@ __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, const floa
 // Initial residual r = b - A x (b = v for static boundaries), z = M^-1 r,
 // p = z; k_cg_init_reduce then sets up Eigen's CG state
 // (ConjugateGradient.h:43-70).
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_cg_init(Dev D) {
     __shared__ double lds[BLOCK / 64];
-    const unsigned i = gtid();
+    const unsigned t = gtid();
+    const unsigned i = t / LPP, hl = t % LPP;
+    const bool act = i < D.n;
     double b2 = 0.0, r2 = 0.0, rz = 0.0;
-    if (i < D.n) {
-        const float3 Ax = weiler_Av(D, i, D.cgx);
+    const float3 Ax = weiler_Av<LPP>(D, i, hl, act, D.cgx);
+    if (act && hl == 0) {
         const float4 b = D.v[i];
         const float3 r = make_float3(b.x - Ax.x, b.y - Ax.y, b.z - Ax.z);
         const float3 z = minv_apply(D, i, r);
@@ -899,14 +980,17 @@ __global__ __launch_bounds__(RBLOCK) void k_cg_init_reduce(Dev D, unsigned npart
 }
 
 // tmp = A p and alpha = absNew / (p . tmp).
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_cg_matvec(Dev D) {
     __shared__ double lds[BLOCK / 64];
     Ctl *C = D.ctl;
     if (C->cg_done) return;
-    const unsigned i = gtid();
+    const unsigned tid = gtid();
+    const unsigned i = tid / LPP, hl = tid % LPP;
+    const bool act = i < D.n;
     double pt = 0.0;
-    if (i < D.n) {
-        const float3 t = weiler_Av(D, i, D.cgp);
+    const float3 t = weiler_Av<LPP>(D, i, hl, act, D.cgp);
+    if (act && hl == 0) {
         D.cgt[i] = make_float4(t.x, t.y, t.z, 0.f);
         const float4 p = D.cgp[i];
         pt = ((double)p.x * t.x + (double)p.y * t.y) + (double)p.z * t.z;
@@ -1488,7 +1572,7 @@ void DFSPH::init() {
     id2_.reserve(S);
     cf_.reserve(S);
     cb_.reserve(S);
-    partial_.reserve(8 * (size_t)nblocks(S) + 64);  // two values per block of up to 4 lanes per particle
+    partial_.reserve(12 * (size_t)nblocks(S) + 64);  // up to three values per block of up to 4 lanes per particle
     {
         std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
         std::vector<unsigned> ids(n);
@@ -1662,6 +1746,34 @@ void DFSPH::volume_maps() {
         SPH_HIP(hipGetLastError());                               \
     } while (0)
 
+// Launch of kernel template K<..., LPP> with the handle's lanes per particle.
+#define LAUNCH_LPP(name, K, ...)                                                   \
+    do {                                                                           \
+        TimedScope ts_(timer, stream, name);                                       \
+        const unsigned nb_ = nblocks((size_t)n_ * lpp_);                           \
+        if (lpp_ == 4)                                                             \
+            K<__VA_ARGS__, 4><<<nb_, BLOCK, 0, stream>>>(D);                       \
+        else if (lpp_ == 2)                                                        \
+            K<__VA_ARGS__, 2><<<nb_, BLOCK, 0, stream>>>(D);                       \
+        else                                                                       \
+            K<__VA_ARGS__, 1><<<nb_, BLOCK, 0, stream>>>(D);                       \
+        SPH_HIP(hipGetLastError());                                                \
+    } while (0)
+
+// Same for kernel templates whose only parameter is LPP.
+#define LAUNCH_LPP1(name, K)                                                       \
+    do {                                                                           \
+        TimedScope ts_(timer, stream, name);                                       \
+        const unsigned nb_ = nblocks((size_t)n_ * lpp_);                           \
+        if (lpp_ == 4)                                                             \
+            K<4><<<nb_, BLOCK, 0, stream>>>(D);                                    \
+        else if (lpp_ == 2)                                                        \
+            K<2><<<nb_, BLOCK, 0, stream>>>(D);                                    \
+        else                                                                       \
+            K<1><<<nb_, BLOCK, 0, stream>>>(D);                                    \
+        SPH_HIP(hipGetLastError());                                                \
+    } while (0)
+
 template <int LPP>
 void DFSPH::sweep_init_l(bool dens, bool fact, bool div) {
     Dev D = dev();
@@ -1784,14 +1896,14 @@ void DFSPH::set_lanes_per_particle(int lpp) {
 
 void DFSPH::div_finish(bool scale_pv) {
     const Dev D = dev();
-    LAUNCH("divergence_finish", k_finish<false>, D);
+    LAUNCH_LPP("divergence_finish", k_finish, false);
     if (scale_pv) LAUNCH("scale", k_scale, D, pv_.get(), 1);
 }
 
 void DFSPH::pressure_setup() {
     Dev D = dev();
     if (learn_or_collect(0)) D.kprev = kprev_.get();
-    LAUNCH("pressure_setup", k_psetup, D);
+    LAUNCH_LPP1("pressure_setup", k_psetup);
 }
 
 // Scale, features and (with a model) the network's start plus the first
@@ -1930,7 +2042,7 @@ void DFSPH::ensure_learn_buffers() {
 
 void DFSPH::pressure_finish(bool finalize_too) {
     const Dev D = dev();
-    LAUNCH("pressure_finish", k_finish<true>, D);
+    LAUNCH_LPP("pressure_finish", k_finish, true);
     if (finalize_too) finalize(true, true);
 }
 
@@ -1948,18 +2060,18 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
     const Dev D = dev();
     // the step uses the full fusion; single stages use the one-flag forms
     if (clear && !apply && !xsph && !cfl && !vupd && !scale_pv)
-        LAUNCH("clear_accel", (k_post<true, false, false, false, false, false>), D);
+        LAUNCH_LPP("clear_accel", k_post, true, false, false, false, false, false);
     else if (!clear && apply && !xsph && !cfl && !vupd)
-        LAUNCH("viscosity_apply", (k_post<false, true, false, false, false, false>), D);
+        LAUNCH_LPP("viscosity_apply", k_post, false, true, false, false, false, false);
     else if (!clear && !apply && xsph && !cfl && !vupd)
-        LAUNCH("xsph", (k_post<false, false, true, false, false, false>), D);
+        LAUNCH_LPP("xsph", k_post, false, false, true, false, false, false);
     else if (!clear && !apply && !xsph && cfl && !vupd)
-        LAUNCH("cfl", (k_post<false, false, false, true, false, false>), D);
+        LAUNCH_LPP("cfl", k_post, false, false, false, true, false, false);
     else if (!clear && !apply && !xsph && !cfl && vupd)
-        LAUNCH("update_velocity", (k_post<false, false, false, false, true, false>), D);
+        LAUNCH_LPP("update_velocity", k_post, false, false, false, false, true, false);
     else {
         // step: clear, [apply], [xsph], [cfl], v update, [pv scale]
-#define POSTV(A, X, C, S) LAUNCH("post_forces", (k_post<true, A, X, C, true, S>), D)
+#define POSTV(A, X, C, S) LAUNCH_LPP("post_forces", k_post, true, A, X, C, true, S)
         const int key = (apply ? 8 : 0) | (xsph ? 4 : 0) | (cfl ? 2 : 0) | (scale_pv ? 1 : 0);
         switch (key) {
             case 0: POSTV(false, false, false, false); break;
@@ -1981,7 +2093,7 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
         }
 #undef POSTV
     }
-    if (cfl) REDUCE("cfl_reduce", k_cfl_reduce, D, nblocks(n_));
+    if (cfl) REDUCE("cfl_reduce", k_cfl_reduce, D, nblocks((size_t)n_ * lpp_));
     if (vupd) v_.swap(v2_);
 }
 
@@ -2020,15 +2132,15 @@ int DFSPH::run_solver(bool pressure, int already) {
 
 int DFSPH::run_cg() {
     const Dev D = dev();
-    LAUNCH("weiler_diag", k_weiler_diag, D);
+    LAUNCH_LPP1("weiler_diag", k_weiler_diag);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 3);
-    LAUNCH("cg_init", k_cg_init, D);
-    REDUCE("cg_reduce", k_cg_init_reduce, D, nblocks(n_));
+    LAUNCH_LPP1("cg_init", k_cg_init);
+    REDUCE("cg_reduce", k_cg_init_reduce, D, nblocks((size_t)n_ * lpp_));
     int launched = 0, chunk = std::max(1, lastCg_ + 1);
     for (;;) {
         for (int c = 0; c < chunk && launched < P.visco_max_iter; c++, launched++) {
-            LAUNCH("cg_matvec", k_cg_matvec, D);
-            REDUCE("cg_reduce", k_cg_alpha, D, nblocks(n_));
+            LAUNCH_LPP1("cg_matvec", k_cg_matvec);
+            REDUCE("cg_reduce", k_cg_alpha, D, nblocks((size_t)n_ * lpp_));
             LAUNCH("cg_update", k_cg_update, D);
             REDUCE("cg_reduce", k_cg_update_reduce, D, nblocks(n_));
             LAUNCH("cg_pupdate", k_cg_pupdate, D);

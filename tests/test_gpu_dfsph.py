@@ -207,7 +207,8 @@ def test_volume_map_spheres_match_oracle():
     run_stage_sequence(g, o, STEP_SEQ, extra_state=tuple(names[2:]))
 
 
-def test_weiler_viscosity_matches_oracle():
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_weiler_viscosity_matches_oracle(lanes):
     sc = scenes.dam_break_small(per_side=20)
     prm = sc["params"]
     prm.viscosity_method = VISCOSITY_WEILER2018
@@ -217,6 +218,7 @@ def test_weiler_viscosity_matches_oracle():
     rng = np.random.default_rng(3)
     v = scenes.f32(rng.normal(0.0, 0.5, sc["x"].shape))
     g, o = make_pair(prm, sc["x"], v, boundary_x=sc["boundary_x"])
+    g.set_lanes_per_particle(lanes)
     g.set_time(0.0, 0.004)
     o.set_time(0.0, 0.004)
     for st in ("neighborhood_search", "densities"):
@@ -238,16 +240,21 @@ def test_weiler_viscosity_matches_oracle():
     assert io > 2
 
 
-def test_full_step_matches_oracle_and_converges():
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_full_step_matches_oracle_and_converges(lanes):
     sc = scenes.dam_break_small(per_side=20)
     g, o = make_pair(sc["params"], sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    g.set_lanes_per_particle(lanes)
     eta = sc["params"].max_error * 0.01 * 1000.0
     for step in range(12):
         sync_state(g, o)
         g.step(1)
         o.step(1)
         cg, co = g.counters(), o.counters()
-        assert cg["iterations"] == co["iterations"], (step, cg["iterations"], co["iterations"])
+        # split sums change the FP32 summation order: a solve may end one
+        # iteration earlier or later at the tolerance boundary
+        assert abs(cg["iterations"] - co["iterations"]) <= (0 if lanes == 1 else 1), \
+            (step, cg["iterations"], co["iterations"])
         assert cg["avg_density_err"] <= eta
         assert cg["time_step_size"] == pytest.approx(co["time_step_size"], rel=1e-6)
         for nm in ("position", "velocity"):
