@@ -226,6 +226,40 @@ def ns_microbench(kind, h, n, reps, check):
     return res
 
 
+def ensemble_bench(nscenes, warmup, steps, threads):
+    """The batched ensemble: `nscenes` seeded scenes (scenes.ensemble_scene:
+    20k-500k particles log-uniform, seeded boxes, block placements and 0-8
+    spheres) stepped one after the other (each handle's own step loop) and
+    together (sph_dfsph_step_batch over `threads` host threads)."""
+    from splishsplash_amd import capi, learned, scenes
+
+    t0 = time.perf_counter()
+    scs = [scenes.ensemble_scene(s) for s in range(nscenes)]
+    sims = [learned.make_sim(sc) for sc in scs]
+    total = sum(len(sc["x"]) for sc in scs)
+    log(f"bench ensemble: {nscenes} scenes, {total} particles, set-up {time.perf_counter() - t0:.1f} s")
+    capi.step_batch(sims, warmup, threads)
+    capi.synchronize()
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step(steps)
+    capi.synchronize()
+    t_seq = time.perf_counter() - t0
+    log(f"bench ensemble: scene by scene {t_seq:.2f} s for {steps} steps")
+    t0 = time.perf_counter()
+    capi.step_batch(sims, steps, threads)
+    capi.synchronize()
+    t_bat = time.perf_counter() - t0
+    log(f"bench ensemble: batched {t_bat:.2f} s for {steps} steps")
+    its = [sim.counters() for sim in sims]
+    return {"metric": "batched ensemble particle-steps/s", "unit": "particle-steps/s", "scenes": nscenes,
+            "particles": total, "steps": steps, "warmup": warmup, "threads": threads,
+            "value": total * steps / t_bat, "scene_by_scene": total * steps / t_seq,
+            "speedup": t_seq / t_bat,
+            "mean_pressure_iterations": float(sum(c["total_iterations"] for c in its) / sum(c["steps"] for c in its)),
+            "flags": int(max(c["flags"] for c in its))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,8 +274,11 @@ def main():
                     help="comma list of the runs: plain (reference start), learned (network), previous "
                          "(previous-stiffness start); the first of learned/plain present gives `value`")
     ap.add_argument("--weights", default=None, help="warm-start networks (default: the packaged ones)")
-    ap.add_argument("--workload", default="dam1m", choices=["dam1m", "ns"],
-                    help="ns: the 4M-point neighbour-search + density microbenchmark")
+    ap.add_argument("--workload", default="dam1m", choices=["dam1m", "ns", "ensemble"],
+                    help="ns: the 4M-point neighbour-search + density microbenchmark; ensemble: 64 seeded "
+                         "scenes stepped together vs one after the other")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--ns-points", type=int, default=4_000_000)
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -263,6 +300,10 @@ def main():
             cases.append(r)
         print(json.dumps({"metric": "neighbour search + FP32 density, 4M points", "unit": "ms per search",
                           "cases": cases}))
+        return
+
+    if args.workload == "ensemble":
+        print(json.dumps(ensemble_bench(args.scenes, 10, args.steps if args.steps != 150 else 20, args.threads)))
         return
 
     sc = scenes.dam_break_1m(per_side=args.per_side)

@@ -262,6 +262,15 @@ int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3],
 int sph_dfsph_init(sph_dfsph_t s);
 /* n_steps calls of TimeStepDFSPH::step(). */
 int sph_dfsph_step(sph_dfsph_t s, int n_steps);
+/* Steps n independent scenes together (no reference counterpart; the
+ * ensemble that generates training data): every handle advances n_steps
+ * steps, the handles are shared out over `threads` host threads (0: up to
+ * 16) by particle count, and each thread interleaves its scenes step by step,
+ * so the scenes' kernels and host round trips overlap on their own streams.
+ * Each scene's results are those of sph_dfsph_step on it alone. A handle may
+ * appear once; the first error of any scene is returned. */
+int sph_dfsph_step_batch(sph_dfsph_t *handles, size_t n, int n_steps,
+                         int threads);
 /* One sub-step, see enum sph_dfsph_stage. */
 int sph_dfsph_stage(sph_dfsph_t s, int stage);
 int sph_dfsph_num_particles(sph_dfsph_t s, size_t *out);

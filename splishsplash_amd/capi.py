@@ -71,6 +71,7 @@ _SIGS = {
     "sph_dfsph_add_volume_map_sphere": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_double, _up]),
     "sph_dfsph_init": (C.c_int, [_vp]),
     "sph_dfsph_step": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_step_batch": (C.c_int, [C.POINTER(_vp), C.c_size_t, C.c_int, C.c_int]),
     "sph_dfsph_stage": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_num_particles": (C.c_int, [_vp, _szp]),
     "sph_dfsph_get_field": (C.c_int, [_vp, C.c_char_p, _vp, C.c_size_t]),
@@ -180,6 +181,12 @@ def mlp_train(theta, x, t, steps=1000, batch=4096, lr=1e-3, seed=0):
     _chk(lib().sph_mlp_train(th.ctypes.data_as(_fp), xx.ctypes.data_as(_fp), tt.ctypes.data_as(_fp), len(xx),
                              steps, batch, lr, seed, C.byref(loss)))
     return th, loss.value
+
+
+def step_batch(sims, n_steps=1, threads=0):
+    """Steps the TimeStepDFSPH handles `sims` together (sph_dfsph_step_batch)."""
+    arr = (_vp * len(sims))(*[s.h for s in sims])
+    _chk(lib().sph_dfsph_step_batch(arr, len(sims), int(n_steps), int(threads)))
 
 
 class PointSet:

@@ -1,6 +1,12 @@
 // dfsph_api.hip -- sph_dfsph_* : C-ABI over the device-resident DFSPH step
 // (TimeStepDFSPH and the state it steps; see include/sph_capi.h).
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
+#include <set>
+#include <thread>
+#include <vector>
 
 #include "dfsph.hpp"
 
@@ -110,6 +116,55 @@ int sph_dfsph_step(sph_dfsph_t s, int n_steps) {
         SPH_REQUIRE(n_steps >= 0, SPH_ERR_INVALID, "n_steps must be >= 0");
         for (int i = 0; i < n_steps; i++) s->s->step();
         s->s->sync();
+    });
+}
+
+int sph_dfsph_step_batch(sph_dfsph_t *handles, size_t n, int n_steps, int threads) {
+    return guarded([&] {
+        SPH_REQUIRE(handles || n == 0, SPH_ERR_INVALID, "null handle array");
+        SPH_REQUIRE(n_steps >= 0, SPH_ERR_INVALID, "n_steps must be >= 0");
+        SPH_REQUIRE(threads >= 0, SPH_ERR_INVALID, "threads must be >= 0");
+        if (n == 0 || n_steps == 0) return;
+        std::set<sph_dfsph_t> seen;
+        for (size_t k = 0; k < n; k++) {
+            DCHK(handles[k]);
+            SPH_REQUIRE(seen.insert(handles[k]).second, SPH_ERR_INVALID, "a handle appears twice in the batch");
+        }
+        const size_t T = std::min<size_t>(n, threads > 0 ? (size_t)threads : 16);
+        // longest-processing-time assignment by particle count
+        std::vector<size_t> order(n);
+        for (size_t k = 0; k < n; k++) order[k] = k;
+        std::sort(order.begin(), order.end(),
+                  [&](size_t a, size_t b) { return handles[a]->s->n() > handles[b]->s->n(); });
+        std::vector<std::vector<size_t>> work(T);
+        std::vector<size_t> load(T, 0);
+        for (size_t k : order) {
+            const size_t t = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            work[t].push_back(k);
+            load[t] += std::max<size_t>(handles[k]->s->n(), 1);
+        }
+        std::atomic<bool> failed{false};
+        std::mutex mu;
+        int code = SPH_OK;
+        std::string msg;
+        auto run = [&](size_t t) {
+            try {
+                for (int st = 0; st < n_steps && !failed.load(); st++)
+                    for (size_t k : work[t]) handles[k]->s->step();
+                for (size_t k : work[t]) handles[k]->s->sync();
+            } catch (const Error &e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!failed.exchange(true)) code = e.code, msg = e.what();
+            } catch (const std::exception &e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!failed.exchange(true)) code = SPH_ERR_INVALID, msg = e.what();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < T; t++) pool.emplace_back(run, t);
+        run(0);
+        for (auto &th : pool) th.join();
+        if (failed) throw Error(code, "batch: " + msg);
     });
 }
 

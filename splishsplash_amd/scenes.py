@@ -137,21 +137,58 @@ def dam_break_variant(seed, per_side=None, aspect=None, n_spheres=None, spacing=
     ns = int(n_spheres if n_spheres is not None else rng.integers(0, 9))
     b = n * spacing
     hi = (max(a * b, b + 0.4), b + 0.4, b + 0.2)
-    spheres = []
-    tries = 0
-    while len(spheres) < ns and tries < 1000:
-        tries += 1
-        r = rng.uniform(0.1, 0.4)
-        c = np.array([rng.uniform(r, hi[0] - r), rng.uniform(r, hi[1] - r), rng.uniform(r, hi[2] - r)])
-        # clear of the block [0.05, b + 0.15]^3 and of the other spheres
-        q = np.clip(c, 0.05, b + 0.15)
-        if np.linalg.norm(c - q) < r + 0.05:
-            continue
-        if any(np.linalg.norm(c - np.array(s[:3])) < r + s[3] + 0.05 for s in spheres):
-            continue
-        spheres.append((float(c[0]), float(c[1]), float(c[2]), float(r)))
+    spheres = place_spheres(rng, ns, hi, (0.05, 0.05, 0.05), (b + 0.15, b + 0.15, b + 0.15))
     sc = dam_break_1m(per_side=n, seed=seed, box_hi=hi, vm_cell=vm_cell, **kw)
     sc["spheres"] = spheres
     sc["sphere_res"] = vm_cell
     sc["variant"] = {"seed": seed, "per_side": n, "aspect": a, "spheres": spheres}
+    return sc
+
+
+def place_spheres(rng, count, box_hi, block_lo, block_hi, rmin=0.1, rmax=0.4, gap=0.05, tries=1000):
+    """Up to `count` spheres of radius U[rmin, rmax] inside [0, box_hi], at
+    least `gap` clear of the block [block_lo, block_hi] and of each other."""
+    spheres = []
+    block_lo, block_hi = np.asarray(block_lo, float), np.asarray(block_hi, float)
+    for _ in range(tries):
+        if len(spheres) >= count:
+            break
+        r = rng.uniform(rmin, rmax)
+        if min(box_hi) < 2 * r:
+            continue
+        c = np.array([rng.uniform(r, box_hi[k] - r) for k in range(3)])
+        q = np.clip(c, block_lo, block_hi)
+        if np.linalg.norm(c - q) < r + gap:
+            continue
+        if any(np.linalg.norm(c - np.array(s[:3])) < r + s[3] + gap for s in spheres):
+            continue
+        spheres.append((float(c[0]), float(c[1]), float(c[2]), float(r)))
+    return spheres
+
+
+def ensemble_scene(seed, nmin=20_000, nmax=500_000, spacing=0.05, vm_cell=0.05, **kw):
+    """One scene of the batched ensemble: a fluid block of about N particles,
+    N log-uniform in [nmin, nmax], with seeded proportions (each axis x
+    U[0.7, 1.4]), in a seeded box (length 1.5-3x the block's, height and
+    depth 1.1-1.6x), placed at a seeded position on the floor, with 0-8
+    static spheres of radius 0.1-0.4 clear of it; solver settings of the 1M
+    dam break."""
+    rng = np.random.default_rng(104729 * seed + 3)
+    n = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+    f = rng.uniform(0.7, 1.4, 3)
+    f /= np.prod(f) ** (1.0 / 3.0)
+    dims = tuple(max(4, int(round(n ** (1.0 / 3.0) * fk))) for fk in f)
+    ext = np.array(dims) * spacing
+    hi = (float(ext[0] * rng.uniform(1.5, 3.0) + 0.2), float(ext[1] * rng.uniform(1.1, 1.6) + 0.2),
+          float(ext[2] * rng.uniform(1.1, 1.6) + 0.2))
+    corner = (float(rng.uniform(0.1, hi[0] - ext[0] - 0.1)), 0.1, float(rng.uniform(0.1, hi[2] - ext[2] - 0.1)))
+    ns = int(rng.integers(0, 9))
+    spheres = place_spheres(rng, ns, hi, np.array(corner) - 0.05, np.array(corner) + ext + 0.05)
+    x = lattice_block(dims, spacing, corner, jitter=0.001, seed=seed)
+    sc = dam_break_1m(per_side=dims, seed=seed, box_hi=hi, vm_cell=vm_cell, **kw)
+    sc["x"] = x
+    sc["v"] = np.zeros_like(x)
+    sc["spheres"] = spheres
+    sc["sphere_res"] = vm_cell
+    sc["variant"] = {"seed": seed, "dims": dims, "box": hi, "corner": corner, "spheres": spheres}
     return sc

@@ -141,3 +141,18 @@ def test_previous_stiffness_mode_converges_with_fewer_iterations():
     assert r["previous"]["pressure_iterations"] < r["plain"]["pressure_iterations"], r
     eta = sc["params"].max_error * 0.01 * sc["params"].density0
     assert r["previous"]["avg_overpressure_err"] <= eta
+
+
+def test_batch_step_matches_scene_by_scene():
+    # Scenes stepped together give the same results as each stepped alone.
+    scs = [scenes.ensemble_scene(s, nmin=5000, nmax=30000) for s in range(5)]
+    alone = [learned.make_sim(sc) for sc in scs]
+    batch = [learned.make_sim(sc) for sc in scs]
+    for sim in alone:
+        sim.step(6)
+    capi.step_batch(batch, 6, threads=3)
+    for a, b in zip(alone, batch):
+        np.testing.assert_array_equal(a.get("position"), b.get("position"))
+        assert a.counters()["total_iterations"] == b.counters()["total_iterations"]
+    with pytest.raises(capi.SPHError):
+        capi.step_batch([batch[0], batch[0]], 1)
