@@ -161,6 +161,7 @@ def run_path(sc, thetas, warmup, steps, breakdown_steps, timed_kernel=None, mode
            "avg_density_err": c1["avg_density_err"], "avg_overpressure_err": c1["avg_overpressure_err"],
            "time_step_size": c1["time_step_size"], "sim_time": c1["time"], "flags": c1["flags"],
            "learned_particles": c1["learned_particles"], "lanes_per_particle": c1["lanes_per_particle"],
+           "lanes_per_particle_accel": c1["lanes_per_particle_accel"],
            "host_syncs_per_step": (c1["host_syncs"] - c0["host_syncs"]) / steps}
     K = float(sim.get("fluid neighbor count").mean())
     del sim

@@ -190,7 +190,8 @@ typedef struct sph_dfsph_counters {
     int    learned_solves;   /* pressure solves started from the network     */
     int    learned_solves_v; /* divergence solves started from the network   */
     int    lanes_per_particle; /* lanes sharing one particle's neighbour sums
-                                  in the solver sweeps (1, 2 or 4)           */
+                                  in the Jacobi updates and the other sweeps
+                                  (1, 2 or 4)                               */
     /* Excess of the last iteration: rho0 * mean over particles that keep
      * pressure of the positive residual (s - A p)_i, i.e. expansion the
      * pressure overshoots. The reference's error (above) counts only
@@ -200,6 +201,8 @@ typedef struct sph_dfsph_counters {
     double avg_overpressure_err_v;
     long long host_syncs;    /* host waits on the device so far (solver
                                 chunk ends, list capacity checks)           */
+    int    lanes_per_particle_accel; /* same as lanes_per_particle, for the
+                                  acceleration sweeps of the solvers        */
 } sph_dfsph_counters_t;
 
 /* Sub-steps of TimeStepDFSPH::step() (TimeStepDFSPH.cpp:117-249), callable
@@ -301,8 +304,10 @@ int sph_dfsph_enable_timing(sph_dfsph_t s, int enable);
  * or empty: all), so that a timed run records two events per launch of
  * those kernels only. */
 int sph_dfsph_time_only(sph_dfsph_t s, const char *names);
-/* Lanes per particle of the solver sweeps: 1, 2 or 4, or 0 (default) to let
- * the first solve time the choices on the live state and keep the fastest. */
+/* Lanes per particle of the sweeps: 1, 2 or 4 for all of them, or 0
+ * (default) to let the first solve time the choices on the live state and
+ * keep the fastest, separately for the acceleration sweeps and the Jacobi
+ * update (whose choice the other sweeps follow). */
 int sph_dfsph_set_lanes_per_particle(sph_dfsph_t s, int lanes);
 int sph_dfsph_kernel_stats(sph_dfsph_t s, int k, const char **name,
                            long long *launches, double *total_ms,

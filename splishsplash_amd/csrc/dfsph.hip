@@ -177,14 +177,17 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
 }
 
 // ------------------------------------------------- pressure accelerations
-// computePressureAccel (TimeStepDFSPH.cpp:1299-1367) for particle i with
-// pressure field P.
-__device__ __forceinline__ void accel_pair(const Dev &D, const float4 &xi, float pi, unsigned j, float pj, float &ax,
+// computePressureAccel (TimeStepDFSPH.cpp:1299-1367) for particle i. During
+// a solve the w component of every fluid position holds the solver's current
+// stiffness (p / rho^2 or p_v / rho^2, kept equal to the stiffness array by
+// k_stiff_to_w and the Jacobi update), so one 16-byte load per neighbour
+// gives both the position and the stiffness.
+__device__ __forceinline__ void accel_pair(const Dev &D, const float4 &xi, float pi, const float4 &xj, float &ax,
                                            float &ay, float &az) {
-    const float ps = pi + pj;
+    const float ps = pi + xj.w;
     if (fabsf(ps) > D.eps) {
         float3 r;
-        const float g = kernel_G(xi, D.x[j], D.K, r);
+        const float g = kernel_G(xi, xj, D.K, r);
         ax += ps * (-D.V * (g * r.x));
         ay += ps * (-D.V * (g * r.y));
         az += ps * (-D.V * (g * r.z));
@@ -223,25 +226,23 @@ __device__ __forceinline__ void accel_boundary(const Dev &D, unsigned i, const f
 __device__ __forceinline__ float3 accel_i(const Dev &D, unsigned i, const float *__restrict__ P) {
     const float4 xi = D.x[i];
     const float pi = P[i];
+    const float4 *__restrict__ X = D.x;
     float ax = 0.f, ay = 0.f, az = 0.f;
     const unsigned cnt = D.cf[i];
     const unsigned *__restrict__ L = D.nf + i;
     const size_t S = D.stride;
     unsigned k = 0;
-    // four neighbours per trip: the index and pressure loads of all four are
+    // four neighbours per trip: the index and position loads of all four are
     // issued before the first use, so each wave keeps more gathers in flight
     for (; k + 4 <= cnt; k += 4) {
         const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
-        const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
-        accel_pair(D, xi, pi, j0, p0, ax, ay, az);
-        accel_pair(D, xi, pi, j1, p1, ax, ay, az);
-        accel_pair(D, xi, pi, j2, p2, ax, ay, az);
-        accel_pair(D, xi, pi, j3, p3, ax, ay, az);
+        const float4 x0 = X[j0], x1 = X[j1], x2 = X[j2], x3 = X[j3];
+        accel_pair(D, xi, pi, x0, ax, ay, az);
+        accel_pair(D, xi, pi, x1, ax, ay, az);
+        accel_pair(D, xi, pi, x2, ax, ay, az);
+        accel_pair(D, xi, pi, x3, ax, ay, az);
     }
-    for (; k < cnt; k++) {
-        const unsigned j = L[k * S];
-        accel_pair(D, xi, pi, j, P[j], ax, ay, az);
-    }
+    for (; k < cnt; k++) accel_pair(D, xi, pi, X[L[k * S]], ax, ay, az);
     float3 a = make_float3(ax, ay, az);
     accel_boundary(D, i, xi, pi, a);
     return a;
@@ -297,16 +298,13 @@ __global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
         for (; k + 3 * LPP < cnt; k += 4 * LPP) {
             const unsigned j0 = L[k * S], j1 = L[(k + LPP) * S], j2 = L[(k + 2 * LPP) * S],
                            j3 = L[(k + 3 * LPP) * S];
-            const float p0 = P[j0], p1 = P[j1], p2 = P[j2], p3 = P[j3];
-            accel_pair(D, xi, pi, j0, p0, ax, ay, az);
-            accel_pair(D, xi, pi, j1, p1, ax, ay, az);
-            accel_pair(D, xi, pi, j2, p2, ax, ay, az);
-            accel_pair(D, xi, pi, j3, p3, ax, ay, az);
+            const float4 x0 = D.x[j0], x1 = D.x[j1], x2 = D.x[j2], x3 = D.x[j3];
+            accel_pair(D, xi, pi, x0, ax, ay, az);
+            accel_pair(D, xi, pi, x1, ax, ay, az);
+            accel_pair(D, xi, pi, x2, ax, ay, az);
+            accel_pair(D, xi, pi, x3, ax, ay, az);
         }
-        for (; k < cnt; k += LPP) {
-            const unsigned j = L[k * S];
-            accel_pair(D, xi, pi, j, P[j], ax, ay, az);
-        }
+        for (; k < cnt; k += LPP) accel_pair(D, xi, pi, D.x[L[k * S]], ax, ay, az);
     }
     if (LPP > 1) {
         ax = lane_sum<LPP>(ax);
@@ -414,6 +412,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
             const float res = fminf(si - aij, 0.0f);
             const float pn = fmaxf(D.p[i] - 0.5f * (si - aij) * alpha, 0.0f);
             D.p[i] = pn;
+            reinterpret_cast<float *>(D.x + i)[3] = pn;
             err = -(double)D.rho0 * (double)res;
             if (pn > 0.0f && si - aij > 0.0f) over = (double)D.rho0 * (double)(si - aij);
         } else {
@@ -423,6 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
             if (nnb < 20u) res = 0.0f;
             const float pn = fmaxf(D.pv[i] - 0.5f * (si - aij) * alpha, 0.0f);
             D.pv[i] = pn;
+            reinterpret_cast<float *>(D.x + i)[3] = pn;
             err = -(double)D.rho0 * (double)res;
             if (pn > 0.0f && si - aij > 0.0f && nnb >= 20u) over = (double)D.rho0 * (double)(si - aij);
         }
@@ -481,6 +481,25 @@ __global__ __launch_bounds__(RBLOCK) void k_jacobi_reduce(Dev D, unsigned nparts
     }
 }
 
+// Copies the solver's stiffness into the w component of the fluid positions
+// (read by the acceleration sweeps with the neighbour's position). Launched
+// at the start of a solve and before a stand-alone iteration or
+// finalisation; the Jacobi update keeps the copy current.
+template <bool PRESSURE>
+__global__ __launch_bounds__(BLOCK) void k_stiff_to_w(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    reinterpret_cast<float *>(D.x + i)[3] = PRESSURE ? D.p[i] : D.pv[i];
+}
+
+// Tuning input: positions with a unit stiffness in w.
+__global__ __launch_bounds__(BLOCK) void k_unit_w(const float4 *__restrict__ x, float4 *__restrict__ y, unsigned n) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = x[i];
+    y[i] = make_float4(a.x, a.y, a.z, 1.0f);
+}
+
 // divergenceSolve / pressureSolve finalisation (TimeStepDFSPH.cpp:345-383,
 // 500-540): v += h a_p; the divergence solve also restores alpha *= h.
 template <bool PRESSURE, int LPP>
@@ -498,7 +517,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev D) {
         if (LPP == 1) {
             a = accel_i(D, i, P);
         } else {
-            fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) { accel_pair(D, xi, pi, j, P[j], a.x, a.y, a.z); });
+            fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) { accel_pair(D, xi, pi, D.x[j], a.x, a.y, a.z); });
         }
     }
     if (LPP > 1) {
@@ -1746,19 +1765,22 @@ void DFSPH::volume_maps() {
         SPH_HIP(hipGetLastError());                               \
     } while (0)
 
-// Launch of kernel template K<..., LPP> with the handle's lanes per particle.
-#define LAUNCH_LPP(name, K, ...)                                                   \
+// Launch of kernel template K<..., LPP> with `lanes` lanes per particle.
+#define LAUNCH_LPP_AS(name, lanes, K, ...)                                         \
     do {                                                                           \
         TimedScope ts_(timer, stream, name);                                       \
-        const unsigned nb_ = nblocks((size_t)n_ * lpp_);                           \
-        if (lpp_ == 4)                                                             \
+        const int l_ = (lanes);                                                    \
+        const unsigned nb_ = nblocks((size_t)n_ * l_);                             \
+        if (l_ == 4)                                                               \
             K<__VA_ARGS__, 4><<<nb_, BLOCK, 0, stream>>>(D);                       \
-        else if (lpp_ == 2)                                                        \
+        else if (l_ == 2)                                                          \
             K<__VA_ARGS__, 2><<<nb_, BLOCK, 0, stream>>>(D);                       \
         else                                                                       \
             K<__VA_ARGS__, 1><<<nb_, BLOCK, 0, stream>>>(D);                       \
         SPH_HIP(hipGetLastError());                                                \
     } while (0)
+// ... with the handle's lanes per particle.
+#define LAUNCH_LPP(name, K, ...) LAUNCH_LPP_AS(name, lpp_, K, __VA_ARGS__)
 
 // Same for kernel templates whose only parameter is LPP.
 #define LAUNCH_LPP1(name, K)                                                       \
@@ -1808,95 +1830,111 @@ void DFSPH::sweep_init(bool dens, bool fact, bool div) {
         sweep_init_l<1>(dens, fact, div);
 }
 
-// One Jacobi iteration: accelerations, then the update, with LPP lanes per
-// particle in both neighbour sums.
-template <bool PRESSURE, int LPP>
-void DFSPH::iteration(bool force) {
+// Launches the acceleration or Jacobi sweep with `lanes` lanes per particle.
+template <bool PRESSURE, bool JACOBI>
+static void launch_sweep(int lanes, unsigned n, hipStream_t s, const Dev &D, int check_done) {
+    const unsigned nb = nblocks((size_t)n * lanes);
+    if (JACOBI) {
+        if (lanes == 4)
+            k_jacobi<PRESSURE, 4><<<nb, BLOCK, 0, s>>>(D, check_done);
+        else if (lanes == 2)
+            k_jacobi<PRESSURE, 2><<<nb, BLOCK, 0, s>>>(D, check_done);
+        else
+            k_jacobi<PRESSURE, 1><<<nb, BLOCK, 0, s>>>(D, check_done);
+    } else {
+        if (lanes == 4)
+            k_accel<PRESSURE, 4><<<nb, BLOCK, 0, s>>>(D, check_done);
+        else if (lanes == 2)
+            k_accel<PRESSURE, 2><<<nb, BLOCK, 0, s>>>(D, check_done);
+        else
+            k_accel<PRESSURE, 1><<<nb, BLOCK, 0, s>>>(D, check_done);
+    }
+    SPH_HIP(hipGetLastError());
+}
+
+// One Jacobi iteration: accelerations, then the update (each sweep with its
+// own lanes per particle).
+template <bool PRESSURE>
+void DFSPH::iteration_any(bool force) {
     const Dev D = dev();
-    const unsigned nb = nblocks((size_t)n_ * LPP);
     {
         TimedScope ts(timer, stream, PRESSURE ? "pressure_accel" : "pressure_accel_v");
-        k_accel<PRESSURE, LPP><<<nb, BLOCK, 0, stream>>>(D, force ? 0 : 1);
-        SPH_HIP(hipGetLastError());
+        launch_sweep<PRESSURE, false>(lppA_, (unsigned)n_, stream, D, force ? 0 : 1);
     }
     {
         TimedScope ts(timer, stream, PRESSURE ? "jacobi" : "jacobi_v");
-        k_jacobi<PRESSURE, LPP><<<nb, BLOCK, 0, stream>>>(D, force ? 0 : 1);
-        SPH_HIP(hipGetLastError());
+        launch_sweep<PRESSURE, true>(lpp_, (unsigned)n_, stream, D, force ? 0 : 1);
     }
-    REDUCE("jacobi_reduce", k_jacobi_reduce<PRESSURE>, D, nb, force ? 0 : 1);
-}
-
-template <bool PRESSURE>
-void DFSPH::iteration_any(bool force) {
-    if (lpp_ == 4)
-        iteration<PRESSURE, 4>(force);
-    else if (lpp_ == 2)
-        iteration<PRESSURE, 2>(force);
-    else
-        iteration<PRESSURE, 1>(force);
+    REDUCE("jacobi_reduce", k_jacobi_reduce<PRESSURE>, D, nblocks((size_t)n_ * lpp_), force ? 0 : 1);
 }
 
 void DFSPH::div_iteration(bool force) { iteration_any<false>(force); }
 
 void DFSPH::pressure_iteration(bool force) { iteration_any<true>(force); }
 
-// Picks the lanes per particle of the solver sweeps by timing the pressure
-// acceleration sweep for 1, 2 and 4 lanes on the current neighbour lists,
-// best of three launches each. The sweep reads a uniform pressure (every
-// pair active) from scratch and writes to scratch, so the solver state is
-// untouched.
+// Picks the lanes per particle of the acceleration sweeps and of the Jacobi
+// update (used for the other sweeps too) by timing each sweep for 1, 2 and 4
+// lanes on the current neighbour lists, best of three launches each. The
+// sweeps read a uniform stiffness (every pair active) from scratch copies of
+// the positions and stiffness and write to those copies, the acceleration
+// array and the reduction partials, none of which holds state between
+// solves.
 void DFSPH::tune_lanes() {
     tuned_ = true;
     if (n_ < 65536 || lpp_fixed_) return;
     Dev D = dev();
     fill_u32(stream, reinterpret_cast<unsigned *>(p2_.get()), n_, 0x3f800000u);  // 1.0f
+    k_unit_w<<<nblocks(n_), BLOCK, 0, stream>>>(x_.get(), x2_.get(), (unsigned)n_);
+    SPH_HIP(hipGetLastError());
     D.p = p2_.get();
-    D.ap = x2_.get();
+    D.x = x2_.get();
+    D.ap = ap_.get();
     hipEvent_t a, b;
     SPH_HIP(hipEventCreate(&a));
     SPH_HIP(hipEventCreate(&b));
-    float best = 1e30f;
-    int pick = 1;
-    for (int L : {1, 2, 4}) {
-        const unsigned nb = nblocks((size_t)n_ * L);
-        float tmin = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {
-            SPH_HIP(hipEventRecord(a, stream));
-            if (L == 1)
-                k_accel<true, 1><<<nb, BLOCK, 0, stream>>>(D, 0);
-            else if (L == 2)
-                k_accel<true, 2><<<nb, BLOCK, 0, stream>>>(D, 0);
-            else
-                k_accel<true, 4><<<nb, BLOCK, 0, stream>>>(D, 0);
-            SPH_HIP(hipGetLastError());
-            SPH_HIP(hipEventRecord(b, stream));
-            SPH_HIP(hipEventSynchronize(b));
-            float ms = 0.f;
-            SPH_HIP(hipEventElapsedTime(&ms, a, b));
-            tmin = std::min(tmin, ms);
+    int pick[2] = {1, 1};
+    for (int sweep = 0; sweep < 2; sweep++) {
+        float best = 1e30f;
+        for (int L : {1, 2, 4}) {
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                SPH_HIP(hipEventRecord(a, stream));
+                if (sweep == 0)
+                    launch_sweep<true, false>(L, (unsigned)n_, stream, D, 0);
+                else
+                    launch_sweep<true, true>(L, (unsigned)n_, stream, D, 0);
+                SPH_HIP(hipEventRecord(b, stream));
+                SPH_HIP(hipEventSynchronize(b));
+                float ms = 0.f;
+                SPH_HIP(hipEventElapsedTime(&ms, a, b));
+                tmin = std::min(tmin, ms);
+            }
+            tune_ms_[sweep][L == 1 ? 0 : (L == 2 ? 1 : 2)] = tmin;
+            if (tmin < best) {
+                best = tmin;
+                pick[sweep] = L;
+            }
         }
-        tune_ms_[L == 1 ? 0 : (L == 2 ? 1 : 2)] = tmin;
-        if (tmin < best) {
-            best = tmin;
-            pick = L;
-        }
+        // the Jacobi sweeps are timed on the accelerations of the chosen lanes
+        if (sweep == 0) launch_sweep<true, false>(pick[0], (unsigned)n_, stream, D, 0);
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    lpp_ = pick;
+    lppA_ = pick[0];
+    lpp_ = pick[1];
 }
 
 void DFSPH::set_lanes_per_particle(int lpp) {
     SPH_REQUIRE(lpp == 0 || lpp == 1 || lpp == 2 || lpp == 4, SPH_ERR_INVALID, "lanes per particle must be 0, 1, 2 or 4");
     lpp_fixed_ = lpp != 0;
     lpp_ = lpp ? lpp : 1;
+    lppA_ = lpp_;
     tuned_ = lpp_fixed_;
 }
 
 void DFSPH::div_finish(bool scale_pv) {
     const Dev D = dev();
-    LAUNCH_LPP("divergence_finish", k_finish, false);
+    LAUNCH_LPP_AS("divergence_finish", lppA_, k_finish, false);
     if (scale_pv) LAUNCH("scale", k_scale, D, pv_.get(), 1);
 }
 
@@ -2042,7 +2080,7 @@ void DFSPH::ensure_learn_buffers() {
 
 void DFSPH::pressure_finish(bool finalize_too) {
     const Dev D = dev();
-    LAUNCH_LPP("pressure_finish", k_finish, true);
+    LAUNCH_LPP_AS("pressure_finish", lppA_, k_finish, true);
     if (finalize_too) finalize(true, true);
 }
 
@@ -2097,8 +2135,17 @@ void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, 
     if (vupd) v_.swap(v2_);
 }
 
+void DFSPH::stiff_to_w(bool pressure) {
+    const Dev D = dev();
+    if (pressure)
+        LAUNCH("stiffness_copy", k_stiff_to_w<true>, D);
+    else
+        LAUNCH("stiffness_copy", k_stiff_to_w<false>, D);
+}
+
 int DFSPH::run_solver(bool pressure, int already) {
     if (!tuned_) tune_lanes();
+    stiff_to_w(pressure);
     const int maxIt = pressure ? P.max_iterations : P.max_iterations_v;
     int launched = already;
     int chunk = std::max(1, pressure ? std::max(lastIt_, P.min_iterations) : lastItV_);
@@ -2205,8 +2252,14 @@ void DFSPH::stage(int s) {
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
             sweep_init(false, false, true);
             break;
-        case SPH_STAGE_DIVERGENCE_ITER: div_iteration(true); break;
-        case SPH_STAGE_DIVERGENCE_FINISH: div_finish(true); break;
+        case SPH_STAGE_DIVERGENCE_ITER:
+            stiff_to_w(false);
+            div_iteration(true);
+            break;
+        case SPH_STAGE_DIVERGENCE_FINISH:
+            stiff_to_w(false);
+            div_finish(true);
+            break;
         case SPH_STAGE_DIVERGENCE_SOLVE:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 1);
             sweep_init(false, false, true);
@@ -2227,8 +2280,12 @@ void DFSPH::stage(int s) {
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
             pressure_setup();
             break;
-        case SPH_STAGE_PRESSURE_ITER: pressure_iteration(true); break;
+        case SPH_STAGE_PRESSURE_ITER:
+            stiff_to_w(true);
+            pressure_iteration(true);
+            break;
         case SPH_STAGE_PRESSURE_FINISH:
+            stiff_to_w(true);
             pressure_finish(false);
             finalize(true, false);
             break;
@@ -2438,6 +2495,7 @@ void DFSPH::counters(sph_dfsph_counters_t *c) {
     c->learned_particles_v = engaged_[1] ? (int)hctl_->nlearn_v : 0;
     c->learned_solves = learnedSolves_;
     c->lanes_per_particle = lpp_;
+    c->lanes_per_particle_accel = lppA_;
     c->host_syncs = syncs_;
     c->learned_solves_v = learnedSolvesV_;
     c->capped_solves = capped_;

@@ -170,16 +170,16 @@ private:
     void pressure_finish(bool finalize);
     void finalize(bool scale_p, bool update_x);
     int run_solver(bool pressure, int already = 0);
-    template <bool PRESSURE, int LPP>
-    void iteration(bool force);
+    void stiff_to_w(bool pressure);
     template <bool PRESSURE>
     void iteration_any(bool force);
     void tune_lanes();
-    int lpp_ = 1;              // lanes per particle of the solver sweeps
+    int lpp_ = 1;              // lanes per particle of the Jacobi update and the other sweeps
+    int lppA_ = 1;             // lanes per particle of the acceleration sweeps (iterations, finish)
     bool tuned_ = false, lpp_fixed_ = false;
 
 public:
-    float tune_ms_[3] = {0, 0, 0};  // accel sweep time at 1, 2, 4 lanes per particle
+    float tune_ms_[2][3] = {};  // acceleration / Jacobi sweep time at 1, 2, 4 lanes per particle
     int lanes_per_particle() const { return lpp_; }
     // 1, 2 or 4 fixes the lanes per particle; 0 lets the first solve pick.
     void set_lanes_per_particle(int lpp);

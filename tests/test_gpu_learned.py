@@ -67,17 +67,21 @@ def test_untrained_network_still_converges():
     assert abs(yl - yr) < 0.01 * yr
 
 
-def test_zero_network_equals_plain_start():
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_zero_network_equals_plain_start(lanes):
     # At the first step every stored stiffness is 0, so the reference's start
     # and the prediction are both 0 and the first iteration keeps the
     # reference's start: the learned path must reproduce the plain one bit
-    # for bit (same kernels' arithmetic, same iteration count).
+    # for bit (same kernels' arithmetic, same iteration count), also with the
+    # neighbour sums split over 2 or 4 lanes per particle.
     sc = small_variant(3)
     th0 = {s: np.zeros(capi.MLP_NPARAM, np.float32) for s in learned.SOLVERS}
     a = learned.make_sim(sc, th0)
     for s in learned.SOLVERS:
         a.set_warm_start_policy(s, 0)
     b = learned.make_sim(sc)
+    a.set_lanes_per_particle(lanes)
+    b.set_lanes_per_particle(lanes)
     a.step(1)
     b.step(1)
     assert a.counters()["learned_solves"] == 1
@@ -116,6 +120,32 @@ def test_engagement_policy():
     b.set_warm_start_policy("pressure", 0)
     b.step(30)
     assert b.counters()["learned_solves"] == 30
+
+
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_learned_start_with_split_neighbour_sums(lanes):
+    # The learned-start kernels (features, both candidate accelerations, the
+    # first iteration) with 2 or 4 lanes per particle: every solve reaches the
+    # tolerance, the network is used in every solve, and the motion and
+    # iteration totals stay those of one lane per particle.
+    sc = small_variant(5)
+    eta = sc["params"].max_error * 0.01 * sc["params"].density0
+    th = learned.load()
+    runs = {}
+    for L in (1, lanes):
+        sim = learned.make_sim(sc, th)
+        sim.set_lanes_per_particle(L)
+        for s in learned.SOLVERS:
+            sim.set_warm_start_policy(s, 0)
+        for _ in range(40):
+            sim.step(1)
+            c = sim.counters()
+            assert (c["avg_density_err"] <= eta and c["avg_overpressure_err"] <= eta) or c["iterations"] == 100
+        assert c["flags"] == 0 and c["learned_solves"] == 40 and c["lanes_per_particle"] == L
+        runs[L] = (sim.get("position")[:, 1].mean(), c["total_iterations"])
+    (y1, i1), (yl, il) = runs[1], runs[lanes]
+    assert abs(yl - y1) < 0.005 * y1
+    assert abs(il - i1) <= 0.1 * i1, runs
 
 
 def test_packaged_networks_gain_on_heldout_variant():
