@@ -124,10 +124,10 @@ def test_engagement_policy():
 
 @pytest.mark.parametrize("lanes", [2, 4])
 def test_learned_start_with_split_neighbour_sums(lanes):
-    # The learned-start kernels (features, both candidate accelerations, the
-    # first iteration) with 2 or 4 lanes per particle: every solve reaches the
-    # tolerance, the network is used in every solve, and the motion and
-    # iteration totals stay those of one lane per particle.
+    # The learned start followed by solver sweeps with 2 or 4 lanes per
+    # particle: every solve reaches the tolerance, the network is used in
+    # every solve, and the motion and iteration totals stay those of one
+    # lane per particle.
     sc = small_variant(5)
     eta = sc["params"].max_error * 0.01 * sc["params"].density0
     th = learned.load()
