@@ -39,8 +39,9 @@ BYTES = {
     "pressure_finish": lambda K: 4 * K + 72,
     "post_forces": lambda K: 4 * K + 104,
     "pressure_setup": lambda K: 4 * K + 68,
-    "weiler_diag": lambda K: 4 * K + 96,
-    "cg_init": lambda K: 4 * K + 120,
+    "cg_init": lambda K: 4 * K + 136,  # block diagonal, its inverse, x0, r, z, p in one sweep
+    "mass_density_copy": lambda K: 8,
+    "stiffness_copy": lambda K: 8,
     "cg_matvec": lambda K: 4 * K + 60,
     "cg_update": lambda K: 108,
     "cg_pupdate": lambda K: 36,

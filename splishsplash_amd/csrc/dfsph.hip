@@ -171,7 +171,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
             if (nnb < 20u) dAdv = 0.0f;
             const float pv0 = D.pv[i];
             if (D.kprev) D.kprev[i] = pv0;
-            D.pv[i] = (dAdv > 0.0f) ? 0.5f * fminf(pv0, 0.5f) * invH : 0.0f;
+            const float pv1 = (dAdv > 0.0f) ? 0.5f * fminf(pv0, 0.5f) * invH : 0.0f;
+            D.pv[i] = pv1;
+            reinterpret_cast<float *>(D.x + i)[3] = pv1;  // the solve's stiffness copy
         }
     }
 }
@@ -492,6 +494,15 @@ __global__ __launch_bounds__(BLOCK) void k_stiff_to_w(Dev D) {
     reinterpret_cast<float *>(D.x + i)[3] = PRESSURE ? D.p[i] : D.pv[i];
 }
 
+// Outside the pressure solves the viscosity and XSPH sweeps read m_j / rho_j
+// from the w component of the fluid positions (the same FP32 quotient as
+// computed in place), so one 16-byte load per neighbour gives both.
+__global__ __launch_bounds__(BLOCK) void k_mrho_to_w(Dev D) {
+    const unsigned i = gtid();
+    if (i >= D.n) return;
+    reinterpret_cast<float *>(D.x + i)[3] = D.mass / D.rho[i];
+}
+
 // Tuning input: positions with a unit stiffness in w.
 __global__ __launch_bounds__(BLOCK) void k_unit_w(const float4 *__restrict__ x, float4 *__restrict__ y, unsigned n) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -568,9 +579,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
             xi = D.x[i];
             if (D.xsph != 0.f)
                 fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
-                    const float4 vj = D.v[j];
-                    const float w = kernel_W(xi, D.x[j], D.K);
-                    const float s = invH * D.xsph * (D.mass / D.rho[j]);
+                    const float4 vj = D.v[j], xj = D.x[j];  // xj.w: m_j / rho_j
+                    const float w = kernel_W(xi, xj, D.K);
+                    const float s = invH * D.xsph * xj.w;
                     sx -= s * (vi.x - vj.x) * w;
                     sy -= s * (vi.y - vj.y) * w;
                     sz -= s * (vi.z - vj.z) * w;
@@ -711,7 +722,9 @@ __global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
     D.alpha[i] *= invH2;
     const float p0 = D.p[i];
     if (D.kprev) D.kprev[i] = p0;
-    D.p[i] = (radv > 1.0f) ? 0.5f * fminf(p0, 0.00025f) * invH2 : 0.0f;
+    const float p1 = (radv > 1.0f) ? 0.5f * fminf(p0, 0.00025f) * invH2 : 0.0f;
+    D.p[i] = p1;
+    reinterpret_cast<float *>(D.x + i)[3] = p1;  // the solve's stiffness copy
 }
 
 // p *= h^2 and the compensated position update x += h_step v (Kahan), then
@@ -775,89 +788,71 @@ __device__ __forceinline__ void ortho(const float3 &n, float3 &t1, float3 &t2) {
     t2 = make_float3(t2.x / l2, t2.y / l2, t2.z / l2);
 }
 
-template <int LPP>
-__global__ __launch_bounds__(BLOCK) void k_weiler_diag(Dev D) {
-    const unsigned t = gtid();
-    const unsigned i = t / LPP, hl = t % LPP;
-    const bool act = i < D.n;
+// Boundary samples of the Weiler2018 operator for particle i: calls
+// f(xs, c) for every boundary sample position xs with its coefficient
+// c = d mu_b V_b (Akinci particles: V_b = rho0 V_b / rho_i, as the reference
+// weighs them; volume maps: four samples of V_b / 4 on the tangent plane at
+// h / 2 from the boundary point, Viscosity_Weiler2018.cpp:327-438, 672-740).
+template <class F>
+__device__ __forceinline__ void weiler_boundary(const Dev &D, unsigned i, const float4 &xi, float rhoi, F &&f) {
     const float dd = 10.0f;
-    float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (act) {
-        xi = D.x[i];
-        fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
-            float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
-            const float s = dd * D.mu * (D.mass / D.rho[j]) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-            const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
-            m[0] += s * (gx * r.x);
-            m[1] += s * (gx * r.y);
-            m[2] += s * (gx * r.z);
-            m[3] += s * (gy * r.y);
-            m[4] += s * (gy * r.z);
-            m[5] += s * (gz * r.z);
-        });
-    }
-    if (LPP > 1)
-#pragma unroll
-        for (int q = 0; q < 6; q++) m[q] = lane_sum<LPP>(m[q]);
-    if (!act || hl != 0) return;
-    const float rhoi = D.rho[i];
-    if (D.mub != 0.f) {
-        if (D.bmode == 1) {
-            const unsigned cb = D.cb[i];
-            for (unsigned k = 0; k < cb; k++) {
-                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
-                float3 r;
-                const float g = kernel_G(xi, xb, D.K, r);
-                const float s = dd * D.mub * (D.rho0 * xb.w / rhoi) / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-                const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
-                m[0] += s * (gx * r.x);
-                m[1] += s * (gx * r.y);
-                m[2] += s * (gx * r.z);
-                m[3] += s * (gy * r.y);
-                m[4] += s * (gy * r.z);
-                m[5] += s * (gz * r.z);
-            }
-        } else if (D.bmode == 2) {
-            for (unsigned o = 0; o < D.nvm; o++) {
-                const float4 xb = D.vmb[(size_t)o * D.stride + i];
-                if (!(xb.w > 0.f)) break;
-                float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
-                const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
-                if (nl > 0.0001f) {
-                    nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
-                    float3 t1, t2;
-                    ortho(nrm, t1, t2);
-                    const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
-                    for (int q = 0; q < 4; q++) {
-                        const float3 t = q < 2 ? t1 : t2;
-                        const float sg = (q % 2 == 0) ? -1.f : 1.f;
-                        const float4 xs = make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist,
-                                                      xb.z + sg * t.z * dist, 0.f);
-                        float3 r;
-                        const float g = kernel_G(xi, xs, D.K, r);
-                        const float s = dd * D.mub * vol / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-                        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
-                        m[0] += s * (gx * r.x);
-                        m[1] += s * (gx * r.y);
-                        m[2] += s * (gx * r.z);
-                        m[3] += s * (gy * r.y);
-                        m[4] += s * (gy * r.z);
-                        m[5] += s * (gz * r.z);
-                    }
+    if (D.mub == 0.f) return;
+    if (D.bmode == 1) {
+        const unsigned cb = D.cb[i];
+        for (unsigned k = 0; k < cb; k++) {
+            const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+            f(xb, dd * D.mub * (D.rho0 * xb.w / rhoi));
+        }
+    } else if (D.bmode == 2) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
+            float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
+            const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
+            if (nl > 0.0001f) {
+                nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
+                float3 t1, t2;
+                ortho(nrm, t1, t2);
+                const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
+                for (int q = 0; q < 4; q++) {
+                    const float3 t = q < 2 ? t1 : t2;
+                    const float sg = (q % 2 == 0) ? -1.f : 1.f;
+                    const float4 xs = make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist,
+                                                  xb.z + sg * t.z * dist, 0.f);
+                    f(xs, dd * D.mub * vol);
                 }
             }
         }
     }
-    const double f = (double)D.ctl->h / (double)rhoi;
-    double md[6] = {1.0 - f * m[0], -f * m[1], -f * m[2], 1.0 - f * m[3], -f * m[4], 1.0 - f * m[5]};
-    double inv[6];
-    sym_inverse(md, inv);
-    D.minv[2 * i] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
-    D.minv[2 * i + 1] = make_float4((float)inv[4], (float)inv[5], 0.f, 0.f);
-    const float4 v = D.v[i], vd = D.vdiff[i];
-    D.cgx[i] = make_float4(v.x + vd.x, v.y + vd.y, v.z + vd.z, 0.f);
+}
+
+// One pair of the operator: with c the pair's coefficient (fluid:
+// d mu m_j / rho_j), adds c g_ij (r r^T) / (|r|^2 + 0.01 h^2) to the block
+// diagonal m (xx xy xz yy yz zz; DIAG) and c (dv . r) / (|r|^2 + 0.01 h^2)
+// g_ij r to the product a (AV).
+template <bool DIAG, bool AV>
+__device__ __forceinline__ void weiler_pair(const Dev &D, const float4 &xi, const float4 &xj, float c, const float4 &vi,
+                                            const float4 &vj, float *m, float &ax, float &ay, float &az) {
+    float3 r;
+    const float g = kernel_G(xi, xj, D.K, r);
+    const float den = sqn(r.x, r.y, r.z) + 0.01f * D.hs2;
+    if (DIAG) {
+        const float s = c / den;
+        const float gx = g * r.x, gy = g * r.y, gz = g * r.z;
+        m[0] += s * (gx * r.x);
+        m[1] += s * (gx * r.y);
+        m[2] += s * (gx * r.z);
+        m[3] += s * (gy * r.y);
+        m[4] += s * (gy * r.z);
+        m[5] += s * (gz * r.z);
+    }
+    if (AV) {
+        const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
+        const float s = c * dot / den;
+        ax += s * (g * r.x);
+        ay += s * (g * r.y);
+        az += s * (g * r.z);
+    }
 }
 
 __device__ __forceinline__ float3 minv_apply(const Dev &D, unsigned i, const float3 &r) {
@@ -868,7 +863,8 @@ __device__ __forceinline__ float3 minv_apply(const Dev &D, unsigned i, const flo
 
 // matrixVecProd (Viscosity_Weiler2018.cpp:271-441): (A vec)_i.
 // For LPP > 1 every lane of the particle calls this (the fluid sum is split
-// over them); the result is valid in lane hl == 0 only.
+// over them); the result is valid in lane hl == 0 only. The fluid
+// positions' w holds m_j / rho_j.
 template <int LPP>
 __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, unsigned hl, bool act,
                                             const float4 *__restrict__ vec) {
@@ -879,14 +875,8 @@ __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, unsigned h
         xi = D.x[i];
         vi = vec[i];
         fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
-            const float4 vj = vec[j];
-            float3 r;
-            const float g = kernel_G(xi, D.x[j], D.K, r);
-            const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
-            const float s = dd * D.mu * (D.mass / D.rho[j]) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-            ax += s * (g * r.x);
-            ay += s * (g * r.y);
-            az += s * (g * r.z);
+            const float4 vj = vec[j], xj = D.x[j];
+            weiler_pair<false, true>(D, xi, xj, dd * D.mu * xj.w, vi, vj, nullptr, ax, ay, az);
         });
     }
     if (LPP > 1) {
@@ -896,70 +886,74 @@ __device__ __forceinline__ float3 weiler_Av(const Dev &D, unsigned i, unsigned h
     }
     if (!act || hl != 0) return make_float3(0.f, 0.f, 0.f);
     const float rhoi = D.rho[i];
-    if (D.mub != 0.f) {
-        if (D.bmode == 1) {
-            const unsigned cb = D.cb[i];
-            for (unsigned k = 0; k < cb; k++) {
-                const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
-                float3 r;
-                const float g = kernel_G(xi, xb, D.K, r);
-                const float dot = (vi.x * r.x + vi.y * r.y) + vi.z * r.z;
-                const float s = dd * D.mub * (D.rho0 * xb.w / rhoi) * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-                ax += s * (g * r.x);
-                ay += s * (g * r.y);
-                az += s * (g * r.z);
-            }
-        } else if (D.bmode == 2) {
-            for (unsigned o = 0; o < D.nvm; o++) {
-                const float4 xb = D.vmb[(size_t)o * D.stride + i];
-                if (!(xb.w > 0.f)) break;
-                float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
-                const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
-                if (nl > 0.0001f) {
-                    nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
-                    float3 t1, t2;
-                    ortho(nrm, t1, t2);
-                    const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
-                    for (int q = 0; q < 4; q++) {
-                        const float3 t = q < 2 ? t1 : t2;
-                        const float sg = (q % 2 == 0) ? -1.f : 1.f;
-                        const float4 xs = make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist,
-                                                      xb.z + sg * t.z * dist, 0.f);
-                        float3 r;
-                        const float g = kernel_G(xi, xs, D.K, r);
-                        const float dot = (vi.x * r.x + vi.y * r.y) + vi.z * r.z;
-                        const float s = dd * D.mub * vol * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
-                        ax += s * (g * r.x);
-                        ay += s * (g * r.y);
-                        az += s * (g * r.z);
-                    }
-                }
-            }
-        }
-    }
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    weiler_boundary(D, i, xi, rhoi, [&](const float4 &xs, float c) {
+        weiler_pair<false, true>(D, xi, xs, c, vi, z4, nullptr, ax, ay, az);
+    });
     const float f = (float)D.ctl->h / rhoi;
     return make_float3(vi.x - f * ax, vi.y - f * ay, vi.z - f * az);
 }
 
-// Initial residual r = b - A x (b = v for static boundaries), z = M^-1 r,
-// p = z; k_cg_init_reduce then sets up Eigen's CG state
-// (ConjugateGradient.h:43-70).
+// Set-up of the Weiler2018 solve in one neighbour sweep: the block diagonal
+// (diagonalMatrixElement, Viscosity_Weiler2018.cpp:605-742) inverted for the
+// block-Jacobi preconditioner (MatrixFreeSolver.h:231-245), the warm start
+// x0 = v + vDiff (:1244-1251), the residual r = b - A x0 (b = v for static
+// boundaries), z = M^-1 r and p = z; k_cg_init_reduce then sets up Eigen's
+// CG state (ConjugateGradient.h:43-70). Neighbours' x0 are formed from v and
+// vDiff in place (the same FP32 sums).
 template <int LPP>
-__global__ __launch_bounds__(BLOCK) void k_cg_init(Dev D) {
+__global__ __launch_bounds__(BLOCK) void k_weiler_init(Dev D) {
     __shared__ double lds[BLOCK / 64];
     const unsigned t = gtid();
     const unsigned i = t / LPP, hl = t % LPP;
     const bool act = i < D.n;
+    const float dd = 10.0f;
+    float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f), x0i = xi, vi = xi;
+    if (act) {
+        xi = D.x[i];
+        vi = D.v[i];
+        const float4 vd = D.vdiff[i];
+        x0i = make_float4(vi.x + vd.x, vi.y + vd.y, vi.z + vd.z, 0.f);
+        fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
+            const float4 xj = D.x[j], vj = D.v[j], vdj = D.vdiff[j];
+            const float4 x0j = make_float4(vj.x + vdj.x, vj.y + vdj.y, vj.z + vdj.z, 0.f);
+            weiler_pair<true, true>(D, xi, xj, dd * D.mu * xj.w, x0i, x0j, m, ax, ay, az);
+        });
+    }
+    if (LPP > 1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) m[q] = lane_sum<LPP>(m[q]);
+        ax = lane_sum<LPP>(ax);
+        ay = lane_sum<LPP>(ay);
+        az = lane_sum<LPP>(az);
+    }
     double b2 = 0.0, r2 = 0.0, rz = 0.0;
-    const float3 Ax = weiler_Av<LPP>(D, i, hl, act, D.cgx);
     if (act && hl == 0) {
-        const float4 b = D.v[i];
-        const float3 r = make_float3(b.x - Ax.x, b.y - Ax.y, b.z - Ax.z);
-        const float3 z = minv_apply(D, i, r);
+        const float rhoi = D.rho[i];
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        weiler_boundary(D, i, xi, rhoi, [&](const float4 &xs, float c) {
+            weiler_pair<true, true>(D, xi, xs, c, x0i, z4, m, ax, ay, az);
+        });
+        const double f = (double)D.ctl->h / (double)rhoi;
+        double md[6] = {1.0 - f * m[0], -f * m[1], -f * m[2], 1.0 - f * m[3], -f * m[4], 1.0 - f * m[5]};
+        double inv[6];
+        sym_inverse(md, inv);
+        const float4 ma = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
+        const float4 mb = make_float4((float)inv[4], (float)inv[5], 0.f, 0.f);
+        D.minv[2 * i] = ma;
+        D.minv[2 * i + 1] = mb;
+        D.cgx[i] = x0i;
+        const float fa = (float)D.ctl->h / rhoi;
+        const float3 Ax = make_float3(x0i.x - fa * ax, x0i.y - fa * ay, x0i.z - fa * az);
+        const float3 r = make_float3(vi.x - Ax.x, vi.y - Ax.y, vi.z - Ax.z);
+        const float3 z = make_float3((ma.x * r.x + ma.y * r.y) + ma.z * r.z, (ma.y * r.x + ma.w * r.y) + mb.x * r.z,
+                                     (ma.z * r.x + mb.x * r.y) + mb.y * r.z);
         D.cgr[i] = make_float4(r.x, r.y, r.z, 0.f);
         D.cgz[i] = make_float4(z.x, z.y, z.z, 0.f);
         D.cgp[i] = make_float4(z.x, z.y, z.z, 0.f);
-        b2 = ((double)b.x * b.x + (double)b.y * b.y) + (double)b.z * b.z;
+        b2 = ((double)vi.x * vi.x + (double)vi.y * vi.y) + (double)vi.z * vi.z;
         r2 = ((double)r.x * r.x + (double)r.y * r.y) + (double)r.z * r.z;
         rz = ((double)r.x * z.x + (double)r.y * z.y) + (double)r.z * z.z;
     }
@@ -1227,6 +1221,7 @@ __global__ __launch_bounds__(BLOCK) void k_pred_start(Dev D) {
         if (!(D.radv[i] > 0.0f) || nnb < 20u) b = q = 0.0f;
         D.pv[i] = b;
     }
+    reinterpret_cast<float *>(D.x + i)[3] = b;
     D.pred[i] = q;
 }
 
@@ -1382,6 +1377,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
             D.p[i] = pn;
         else
             D.pv[i] = pn;
+        reinterpret_cast<float *>(D.x + i)[3] = pn;
         err = -(double)D.rho0 * (double)res;
         if (pn > 0.0f && rr > 0.0f && (PRESSURE || nnb >= 20u)) over = (double)D.rho0 * (double)rr;
     }
@@ -2096,6 +2092,7 @@ void DFSPH::finalize(bool scale_p, bool update_x) {
 
 void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, bool scale_pv) {
     const Dev D = dev();
+    if (xsph && P.xsph != 0.0 && !mrho_w_) LAUNCH("mass_density_copy", k_mrho_to_w, D);
     // the step uses the full fusion; single stages use the one-flag forms
     if (clear && !apply && !xsph && !cfl && !vupd && !scale_pv)
         LAUNCH_LPP("clear_accel", k_post, true, false, false, false, false, false);
@@ -2143,9 +2140,9 @@ void DFSPH::stiff_to_w(bool pressure) {
         LAUNCH("stiffness_copy", k_stiff_to_w<false>, D);
 }
 
-int DFSPH::run_solver(bool pressure, int already) {
+int DFSPH::run_solver(bool pressure, int already, bool copy_w) {
     if (!tuned_) tune_lanes();
-    stiff_to_w(pressure);
+    if (copy_w) stiff_to_w(pressure);
     const int maxIt = pressure ? P.max_iterations : P.max_iterations_v;
     int launched = already;
     int chunk = std::max(1, pressure ? std::max(lastIt_, P.min_iterations) : lastItV_);
@@ -2179,9 +2176,9 @@ int DFSPH::run_solver(bool pressure, int already) {
 
 int DFSPH::run_cg() {
     const Dev D = dev();
-    LAUNCH_LPP1("weiler_diag", k_weiler_diag);
+    if (!mrho_w_) LAUNCH("mass_density_copy", k_mrho_to_w, D);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 3);
-    LAUNCH_LPP1("cg_init", k_cg_init);
+    LAUNCH_LPP1("cg_init", k_weiler_init);
     REDUCE("cg_reduce", k_cg_init_reduce, D, nblocks((size_t)n_ * lpp_));
     int launched = 0, chunk = std::max(1, lastCg_ + 1);
     for (;;) {
@@ -2216,7 +2213,7 @@ void DFSPH::step() {
     sweep_init(true, true, div);
     if (div) {
         const bool lv = learned_start(false);
-        itVLast_ = run_solver(false, lv ? 1 : 0);
+        itVLast_ = run_solver(false, lv ? 1 : 0, false);  // the set-up wrote the copy
         lastItV_ = std::max(1, itVLast_);
         totItV_ += itVLast_;
         div_finish(false);
@@ -2224,14 +2221,20 @@ void DFSPH::step() {
         itVLast_ = 0;
     }
     const bool weiler = P.viscosity_method == SPH_VISCOSITY_WEILER2018;
+    if ((weiler || P.xsph != 0.0) && n_) {
+        const Dev D = dev();
+        LAUNCH("mass_density_copy", k_mrho_to_w, D);
+        mrho_w_ = true;
+    }
     if (weiler) viscosity(false);
     post_forces(true, weiler, P.xsph != 0.0 || P.xsph_boundary != 0.0, P.cfl_method == 1 || P.cfl_method == 2, true,
                 div);
+    mrho_w_ = false;
     if (div) collect(false);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
     pressure_setup();
     const bool lp = learned_start(true);
-    itLast_ = run_solver(true, lp ? 1 : 0);
+    itLast_ = run_solver(true, lp ? 1 : 0, false);
     lastIt_ = std::max(P.min_iterations, itLast_);
     totIt_ += itLast_;
     pressure_finish(true);

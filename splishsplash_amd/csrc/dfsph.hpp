@@ -169,7 +169,10 @@ private:
     void pressure_iteration(bool force);
     void pressure_finish(bool finalize);
     void finalize(bool scale_p, bool update_x);
-    int run_solver(bool pressure, int already = 0);
+    // copy_w: copy the stiffness into the position w first (the set-up
+    // sweeps of a step write it themselves)
+    int run_solver(bool pressure, int already = 0, bool copy_w = true);
+    bool mrho_w_ = false;  // position w holds m / rho (within a step's non-pressure forces)
     void stiff_to_w(bool pressure);
     template <bool PRESSURE>
     void iteration_any(bool force);
