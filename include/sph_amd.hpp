@@ -184,6 +184,11 @@ public:
         check(sph_dfsph_set_warm_start_policy(h_, solver, min_prev_iterations), "set_warm_start_policy");
     }
     void init() { check(sph_dfsph_init(h_), "init"); }
+    void reset() { check(sph_dfsph_reset(h_), "reset"); }
+    // Emitter support: emittedParticles + resize of the reference's TimeStep.
+    void emittedParticles(const float *x, const float *v, std::size_t n) {
+        check(sph_dfsph_emit_particles(h_, x, v, n), "emit_particles");
+    }
     void step() { check(sph_dfsph_step(h_, 1), "step"); }
     void step(int n) { check(sph_dfsph_step(h_, n), "step"); }
     // sub-steps, TimeStepDFSPH.cpp:117-249

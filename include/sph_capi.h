@@ -265,6 +265,20 @@ int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3],
 int sph_dfsph_init(sph_dfsph_t s);
 /* n_steps calls of TimeStepDFSPH::step(). */
 int sph_dfsph_step(sph_dfsph_t s, int n_steps);
+/* TimeStepDFSPH::reset (TimeStepDFSPH.cpp:710-716, SimulationDataDFSPH.cpp:
+ * 64-81): zeroes the solver's per-particle fields (factor, advected density,
+ * p / rho^2, p_v / rho^2, pressure acceleration) and the iteration counters.
+ * Positions, velocities, densities, time and the viscosity warm start are
+ * kept, as in the reference. */
+int sph_dfsph_reset(sph_dfsph_t s);
+/* Particles activated by an emitter (Emitter.cpp:180-225) followed by
+ * TimeStepDFSPH::emittedParticles and resize (TimeStepDFSPH.cpp:723-731,
+ * SimulationDataDFSPH.cpp:103-112): appends n particles with positions x and
+ * velocities v (3n floats each; v may be null for zero). They get ids
+ * num_particles() .. num_particles() + n - 1 and zero stiffness; the other
+ * particles keep their state. */
+int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v,
+                             size_t n);
 /* Steps n independent scenes together (no reference counterpart; the
  * ensemble that generates training data): every handle advances n_steps
  * steps, the handles are shared out over `threads` host threads (0: up to

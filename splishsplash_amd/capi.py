@@ -66,6 +66,8 @@ _SIGS = {
     "sph_dfsph_create": (C.c_int, [C.POINTER(DFSPHParams), C.POINTER(_vp)]),
     "sph_dfsph_destroy": (C.c_int, [_vp]),
     "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
+    "sph_dfsph_reset": (C.c_int, [_vp]),
+    "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_boundary_particles": (C.c_int, [_vp, _fp, C.c_size_t]),
     "sph_dfsph_set_volume_map_box": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _up]),
     "sph_dfsph_add_volume_map_sphere": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_double, _up]),
@@ -368,6 +370,18 @@ class TimeStepDFSPH:
 
     def init(self):
         _chk(lib().sph_dfsph_init(self.h))
+
+    def reset(self):
+        """TimeStepDFSPH::reset: solver fields and iteration counters to zero."""
+        _chk(lib().sph_dfsph_reset(self.h))
+
+    def emit_particles(self, x, v=None):
+        """Appends particles (emitter + emittedParticles): ids continue from
+        the current count, stiffness starts at zero."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+        _chk(lib().sph_dfsph_emit_particles(self.h, x.ctypes.data_as(_fp), v.ctypes.data_as(_fp), len(x)))
+        self.n += len(x)
 
     def step(self, n=1):
         _chk(lib().sph_dfsph_step(self.h, n))

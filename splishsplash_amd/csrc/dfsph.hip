@@ -1572,14 +1572,11 @@ void DFSPH::upload_ctl_field(size_t off, const void *src, size_t bytes) {
     SPH_HIP(hipStreamSynchronize(stream));
 }
 
-void DFSPH::init() {
-    SPH_REQUIRE(n_ > 0, SPH_ERR_STATE, "set_fluid must be called first");
-    const size_t n = n_;
-    stride_ = (unsigned)((n + 255) / 256 * 256);
+// Per-particle device buffers for stride_ slots (contents not preserved).
+void DFSPH::alloc_particles() {
     const size_t S = stride_;
     for (DBuf<float4> *b : {&x_, &v_, &v2_, &a_, &ap_, &vdiff_, &xc_, &x2_, &vdiff2_, &xc2_}) b->reserve(S);
-    const bool bender = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019;
-    const size_t nobj = bender ? vm_shapes_.size() : 0;
+    const size_t nobj = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 ? vm_shapes_.size() : 0;
     vmb_.reserve(std::max<size_t>(nobj, 1) * S);
     if (nobj > 1) vmraw_.reserve(nobj * S);
     for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_, &p2_, &pv2_}) b->reserve(S);
@@ -1588,25 +1585,53 @@ void DFSPH::init() {
     cf_.reserve(S);
     cb_.reserve(S);
     partial_.reserve(12 * (size_t)nblocks(S) + 64);  // up to three values per block of up to 4 lanes per particle
-    {
-        std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
-        std::vector<unsigned> ids(n);
-        for (size_t i = 0; i < n; i++) {
-            hx[i] = make_float4(hx_[3 * i], hx_[3 * i + 1], hx_[3 * i + 2], 0.f);
-            hv[i] = make_float4(hv_[3 * i], hv_[3 * i + 1], hv_[3 * i + 2], 0.f);
-            ids[i] = (unsigned)i;
-        }
-        std::vector<float> zf(n, 0.f);
-        SPH_HIP(hipMemcpy(x_.get(), hx.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        SPH_HIP(hipMemcpy(v_.get(), hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        for (DBuf<float4> *b : {&a_, &ap_, &vdiff_, &xc_})
-            SPH_HIP(hipMemcpy(b->get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        for (size_t o = 0; o < std::max<size_t>(nobj, 1); o++)
-            SPH_HIP(hipMemcpy(vmb_.get() + o * S, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_})
-            SPH_HIP(hipMemcpy(b->get(), zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
-        SPH_HIP(hipMemcpy(id_.get(), ids.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    if (initialized_ && (mode_[0] || mode_[1] || stride_s_[0] || stride_s_[1])) ensure_learn_buffers();
+    if (P.viscosity_method == SPH_VISCOSITY_WEILER2018) {
+        minv_.reserve(2 * S);
+        for (DBuf<float4> *b : {&cgx_, &cgr_, &cgp_, &cgz_, &cgt_}) b->reserve(S);
     }
+}
+
+// Uploads n particles in id order (ids 0..n-1): positions and velocities
+// (3 floats each) and the persistent per-particle state (null: zero); the
+// other fields start at zero.
+void DFSPH::upload_particles(size_t n, const float *x, const float *v, const float4 *xc, const float4 *vdiff,
+                             const float *p, const float *pv) {
+    const size_t S = stride_;
+    const size_t nobj = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 ? vm_shapes_.size() : 0;
+    std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<unsigned> ids(n);
+    for (size_t i = 0; i < n; i++) {
+        hx[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.f);
+        hv[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f);
+        ids[i] = (unsigned)i;
+    }
+    std::vector<float> zf(n, 0.f);
+    SPH_HIP(hipStreamSynchronize(stream));
+    SPH_HIP(hipMemcpy(x_.get(), hx.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(v_.get(), hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (DBuf<float4> *b : {&a_, &ap_})
+        SPH_HIP(hipMemcpy(b->get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(vdiff_.get(), vdiff ? vdiff : zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(xc_.get(), xc ? xc : zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (size_t o = 0; o < std::max<size_t>(nobj, 1); o++)
+        SPH_HIP(hipMemcpy(vmb_.get() + o * S, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    for (DBuf<float> *b : {&rho_, &alpha_, &radv_})
+        SPH_HIP(hipMemcpy(b->get(), zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(p_.get(), p ? p : zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(pv_.get(), pv ? pv : zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(id_.get(), ids.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+}
+
+void DFSPH::init() {
+    SPH_REQUIRE(n_ > 0, SPH_ERR_STATE, "set_fluid must be called first");
+    const size_t n = n_;
+    stride_ = (unsigned)((n + 255) / 256 * 256);
+    const size_t S = stride_;
+    const bool bender = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019;
+    const size_t nobj = bender ? vm_shapes_.size() : 0;
+    alloc_particles();
+    upload_particles(n, hx_.data(), hv_.data(), nullptr, nullptr, nullptr, nullptr);
     // grid over fluid, boundary particles and the volume-map box
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     auto ext = [&](const std::vector<float> &a) {
@@ -1672,12 +1697,92 @@ void DFSPH::init() {
     nvm_ = (unsigned)nobj;
     initialized_ = true;
     if (mode_[0] || mode_[1] || stride_s_[0] || stride_s_[1]) ensure_learn_buffers();
-    if (P.viscosity_method == SPH_VISCOSITY_WEILER2018) {
-        minv_.reserve(2 * S);
-        for (DBuf<float4> *b : {&cgx_, &cgr_, &cgp_, &cgz_, &cgt_}) b->reserve(S);
-    }
-    initialized_ = true;
     have_lists_ = false;
+    (void)S;
+}
+
+// TimeStepDFSPH::reset (TimeStepDFSPH.cpp:710-716) with
+// SimulationDataDFSPH::reset (SimulationDataDFSPH.cpp:64-81): the solver's
+// per-particle fields and iteration counters return to zero; positions,
+// velocities and the viscosity warm start (Viscosity_Weiler2018::reset
+// keeps vDiff) are unchanged.
+void DFSPH::reset() {
+    itLast_ = itVLast_ = cgLast_ = 0;
+    lastIt_ = 2;
+    lastItV_ = 1;
+    lastCg_ = 4;
+    totIt_ = totItV_ = totVisco_ = 0;
+    steps_ = 0;
+    capped_ = cappedV_ = 0;
+    learnedSolves_ = learnedSolvesV_ = 0;
+    engaged_[0] = engaged_[1] = false;
+    if (!initialized_) return;
+    const size_t n = n_;
+    for (DBuf<float> *b : {&alpha_, &radv_, &p_, &pv_})
+        SPH_HIP(hipMemsetAsync(b->get(), 0, n * sizeof(float), stream));
+    SPH_HIP(hipMemsetAsync(ap_.get(), 0, n * sizeof(float4), stream));
+    for (int which : {0, 1, 3}) k_reset<<<1, 1, 0, stream>>>(ctl_.get(), which);
+    SPH_HIP(hipGetLastError());
+    SPH_HIP(hipStreamSynchronize(stream));
+}
+
+// Particles added by an emitter (Emitter.cpp:180-225 activates them and
+// calls emittedParticles, which zeroes their stiffness: TimeStepDFSPH.cpp:
+// 723-731, SimulationDataDFSPH.cpp:103-112). The existing particles keep
+// their state; the device arrays grow when needed (resize). The state is
+// rebuilt in id order with the new particles appended; the cell grid stays
+// that of init() (particles outside it fall into its border cells, which
+// the search handles exactly, only with more candidates).
+void DFSPH::emit_particles(const float *x, const float *v, size_t k) {
+    if (k == 0) return;
+    SPH_REQUIRE(x != nullptr, SPH_ERR_INVALID, "null positions");
+    SPH_REQUIRE(n_ + k < (size_t)0x7fffffff, SPH_ERR_INVALID, "too many particles");
+    if (!initialized_) {
+        hx_.insert(hx_.end(), x, x + 3 * k);
+        if (v)
+            hv_.insert(hv_.end(), v, v + 3 * k);
+        else
+            hv_.insert(hv_.end(), 3 * k, 0.f);
+        n_ += k;
+        return;
+    }
+    const size_t n = n_, m = n + k;
+    SPH_HIP(hipStreamSynchronize(stream));
+    std::vector<float4> dx(n), dv(n), dxc(n), dvd(n);
+    std::vector<float> dp(n), dpv(n);
+    std::vector<unsigned> id(n);
+    SPH_HIP(hipMemcpy(dx.data(), x_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(dv.data(), v_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(dxc.data(), xc_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(dvd.data(), vdiff_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(dp.data(), p_.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(dpv.data(), pv_.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    SPH_HIP(hipMemcpy(id.data(), id_.get(), n * sizeof(unsigned), hipMemcpyDeviceToHost));
+    std::vector<float> hx(3 * m), hv(3 * m, 0.f), hp(m, 0.f), hpv(m, 0.f);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    std::vector<float4> hxc(m, z4), hvd(m, z4);
+    for (size_t d = 0; d < n; d++) {
+        const unsigned t = id[d];
+        SPH_REQUIRE(t < n, SPH_ERR_STATE, "particle ids are not a permutation");
+        hx[3 * t] = dx[d].x, hx[3 * t + 1] = dx[d].y, hx[3 * t + 2] = dx[d].z;
+        hv[3 * t] = dv[d].x, hv[3 * t + 1] = dv[d].y, hv[3 * t + 2] = dv[d].z;
+        hxc[t] = dxc[d];
+        hvd[t] = dvd[d];
+        hp[t] = dp[d];
+        hpv[t] = dpv[d];
+    }
+    std::copy(x, x + 3 * k, hx.begin() + 3 * n);
+    if (v) std::copy(v, v + 3 * k, hv.begin() + 3 * n);
+    n_ = m;
+    if (m > stride_) {
+        stride_ = (unsigned)((m + m / 4 + 255) / 256 * 256);  // headroom for further emission
+        alloc_particles();
+    }
+    upload_particles(m, hx.data(), hv.data(), hxc.data(), hvd.data(), hp.data(), hpv.data());
+    hx_.assign(hx.begin(), hx.end());
+    hv_.assign(hv.begin(), hv.end());
+    have_lists_ = false;
+    tuned_ = lpp_fixed_;
 }
 
 void DFSPH::build_lists() {

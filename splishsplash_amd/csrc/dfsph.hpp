@@ -89,6 +89,8 @@ public:
                             const unsigned res[3]);
     void add_volume_map(const Shape &shape, const unsigned res[3]);
     void init();
+    void reset();
+    void emit_particles(const float *x, const float *v, size_t n);
     void step();
     void stage(int s);
 
@@ -151,6 +153,9 @@ private:
     void ensure_learn_buffers();
 
     Dev dev() const;
+    void alloc_particles();
+    void upload_particles(size_t n, const float *x, const float *v, const float4 *xc, const float4 *vdiff,
+                          const float *p, const float *pv);
     void upload_ctl_field(size_t off, const void *src, size_t bytes);
     void read_ctl();
     void neighborhood_search();

@@ -110,6 +110,21 @@ int sph_dfsph_init(sph_dfsph_t s) {
     });
 }
 
+int sph_dfsph_reset(sph_dfsph_t s) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->reset();
+    });
+}
+
+int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v, size_t n) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(x || n == 0, SPH_ERR_INVALID, "null positions");
+        s->s->emit_particles(x, v, n);
+    });
+}
+
 int sph_dfsph_step(sph_dfsph_t s, int n_steps) {
     return guarded([&] {
         DCHK(s);

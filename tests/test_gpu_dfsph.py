@@ -277,3 +277,74 @@ def test_dam_break_runs_stably():
     assert np.isfinite(x).all() and c["flags"] == 0
     assert (x > -0.03).all() and (x[:, 0] < 4.03).all() and (x[:, 1] < 3.03).all() and (x[:, 2] < 2.03).all()
     assert x[:, 1].mean() < sc["x"][:, 1].mean()
+
+
+def test_reset_zeroes_solver_fields_and_counters():
+    # TimeStepDFSPH::reset: the solver fields and iteration counters return
+    # to zero, positions, velocities and densities stay; the next step is the
+    # oracle's step from that state.
+    sc = scenes.dam_break_small(per_side=16)
+    g, o = make_pair(sc["params"], sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    g.step(60)  # past the start, where the lattice (0.8 rho0) is under-dense and p = 0
+    x, v, rho = g.get("position"), g.get("velocity"), g.get("density")
+    assert g.get("p / rho^2").max() > 0 and g.counters()["total_iterations"] > 0
+    g.reset()
+    for nm in ("p / rho^2", "p_v / rho^2", "factor", "advected density", "pressure acceleration"):
+        assert not g.get(nm).any(), nm
+    c = g.counters()
+    assert c["iterations"] == 0 and c["iterations_v"] == 0 and c["total_iterations"] == 0 and c["steps"] == 0
+    np.testing.assert_array_equal(g.get("position"), x)
+    np.testing.assert_array_equal(g.get("velocity"), v)
+    np.testing.assert_array_equal(g.get("density"), rho)
+    for _ in range(2):
+        sync_state(g, o)
+        g.step(1)
+        o.step(1)
+        assert g.counters()["iterations"] == o.counters()["iterations"]
+        for nm in ("position", "velocity"):
+            assert rel(g.get(nm), o.get(nm)) <= 1e-4, nm
+
+
+def test_emitted_particles_keep_state_and_join_the_step():
+    # Emission (emitter + emittedParticles + resize): the existing particles
+    # keep every field, the new ones get the next ids, their positions and
+    # velocities and zero stiffness, and the following steps treat them like
+    # any other particle (oracle from identical states).
+    sc = scenes.dam_break_small(per_side=16)
+    g, o = make_pair(sc["params"], sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    g.step(60)
+    assert g.get("p / rho^2").max() > 0
+    keep = ("position", "velocity", "p / rho^2", "p_v / rho^2", "velocity difference")
+    before = {nm: g.get(nm) for nm in keep}
+    n = g.n
+    s = np.arange(5, dtype=np.float32) * 0.05
+    new = np.stack(np.meshgrid(s, s, s, indexing="ij"), -1).reshape(-1, 3) + np.float32([1.5, 1.2, 0.6])
+    vnew = np.tile(np.float32([0.0, -1.0, 0.0]), (len(new), 1))
+    for _ in range(3):  # the first emission grows the device arrays, the others use the headroom
+        g.emit_particles(new, vnew)
+        new = new + np.float32([0.3, 0.0, 0.0])
+    assert g.n == n + 3 * 125
+    for nm in keep:
+        np.testing.assert_array_equal(g.get(nm)[:n], before[nm])
+    assert not g.get("p / rho^2")[n:].any() and not g.get("p_v / rho^2")[n:].any()
+    np.testing.assert_array_equal(g.get("velocity")[n:], np.tile(vnew, (3, 1)))
+    o.set_fluid(g.get("position"), g.get("velocity"))
+    for step in range(3):
+        sync_state(g, o)
+        g.step(1)
+        o.step(1)
+        assert g.counters()["iterations"] == o.counters()["iterations"], step
+        for nm in ("position", "velocity"):
+            assert rel(g.get(nm), o.get(nm)) <= 1e-4, (step, nm)
+    # emission before the first step is the same as handing all particles
+    # to set_fluid
+    a = capi.TimeStepDFSPH(sc["params"])
+    a.set_fluid(sc["x"][:-100], sc["v"][:-100])
+    a.emit_particles(sc["x"][-100:], sc["v"][-100:])
+    a.add_boundary_particles(sc["boundary_x"])
+    b = capi.TimeStepDFSPH(sc["params"])
+    b.set_fluid(sc["x"], sc["v"])
+    b.add_boundary_particles(sc["boundary_x"])
+    a.step(3)
+    b.step(3)
+    np.testing.assert_array_equal(a.get("position"), b.get("position"))
