@@ -67,7 +67,7 @@ SYMBOL = {"jacobi": "k_jacobi", "jacobi_v": "k_jacobi", "pressure_accel": "k_acc
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     record (FETCH_SIZE/WRITE_SIZE scaled by a copy calibration; the commands
-    are in profiles/dam1m_traffic_sweeps.md), or None."""
+    are in profiles/dam1m_traffic_kernels.md), or None."""
     path = os.path.join(ROOT, "profiles", "dam1m_traffic.json")
     try:
         rec = json.load(open(path))
