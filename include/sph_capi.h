@@ -55,6 +55,7 @@ extern "C" {
 #define SPH_BOUNDARY_AKINCI2012  0
 #define SPH_BOUNDARY_BENDER2019  2
 #define SPH_VISCOSITY_NONE       0
+#define SPH_VISCOSITY_STANDARD   1   /* Viscosity_Standard (explicit)        */
 #define SPH_VISCOSITY_WEILER2018 6
 
 const char *sph_last_error(void);
@@ -153,8 +154,8 @@ typedef struct sph_dfsph_params {
     double xsph;                /* XSPH.xsph                                */
     double xsph_boundary;       /* XSPH.xsphBoundary                        */
     int    viscosity_method;    /* SPH_VISCOSITY_*                          */
-    double viscosity;           /* Weiler2018 viscosity (kinematic)         */
-    double viscosity_boundary;  /* Weiler2018 viscosityBoundary             */
+    double viscosity;           /* viscosity of the chosen method           */
+    double viscosity_boundary;  /* its viscosityBoundary                    */
     int    visco_max_iter;
     double visco_max_error;
     int    boundary_handling_method; /* SPH_BOUNDARY_*                      */

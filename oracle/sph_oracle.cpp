@@ -1135,6 +1135,68 @@ void Sim::viscosity_weiler() {
         }
 }
 
+// Viscosity_Standard::step, scalar branch (Viscosity_Standard.cpp:240-400),
+// static boundaries (boundary point velocities zero).
+void Sim::viscosity_standard() {
+    if (P.viscosity_method != SPH_VISCOSITY_STANDARD || n == 0) return;
+    const double h = hs, h2 = h * h, density0 = P.density0;
+    const double nu = P.viscosity, nub = P.viscosity_boundary, d = 10.0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i], *vi = &v[3 * i];
+        double *ai = &a[3 * i];
+        const double density_i = rho[i];
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double *xj = &x[3 * j], *vj = &v[3 * j];
+            const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            const double dv[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+            double g[3];
+            K.gradW(xixj, g);
+            const double s = d * nu * (mass / rho[j]) * dot3(dv, xixj) / (sqn3(xixj) + 0.01 * h2);
+            for (int c = 0; c < 3; c++) ai[c] += s * g[c];
+        }
+        if (nub == 0.0) continue;
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double *xj = &bx[3 * (size_t)j];
+                const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double g[3];
+                K.gradW(xixj, g);
+                const double s = d * nub * (density0 * bV[j] / density_i) * dot3(vi, xixj) / (sqn3(xixj) + 0.01 * h2);
+                for (int c = 0; c < 3; c++) ai[c] += s * g[c];
+            }
+        } else if (has_volume_map()) {
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (!(Vj > 0.0)) continue;
+                const double *xj = &vmX[3 * (o * n + i)];
+                double normal[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+                const double nl = std::sqrt(sqn3(normal));
+                if (!(nl > 0.0001)) continue;
+                for (int c = 0; c < 3; c++) normal[c] /= nl;
+                double t1[3], t2[3];
+                orthogonal_vectors(normal, t1, t2);
+                const double dist = (1.0 - nl / h) * h;
+                const double vol = 0.25 * Vj;
+                double sum[3] = {0.0, 0.0, 0.0};  // a1 + a2 + a3 + a4
+                for (int sidx = 0; sidx < 4; sidx++) {
+                    const double *t = (sidx < 2) ? t1 : t2;
+                    const double sg = (sidx % 2 == 0) ? -1.0 : 1.0;
+                    double xix[3];
+                    for (int c = 0; c < 3; c++) xix[c] = xi[c] - (xj[c] + sg * t[c] * dist);
+                    double g[3];
+                    K.gradW(xix, g);
+                    const double s = d * nub * vol * dot3(vi, xix) / (sqn3(xix) + 0.01 * h2);
+                    for (int c = 0; c < 3; c++) sum[c] += s * g[c];
+                }
+                for (int c = 0; c < 3; c++) ai[c] += sum[c];
+            }
+        }
+    }
+}
+
 // XSPH::step, scalar branch (XSPH.cpp:138-216), static boundaries.
 void Sim::xsph() {
     const double cf = P.xsph, cb = P.xsph_boundary;
@@ -1297,6 +1359,7 @@ void Sim::step() {
     else
         iterationsV = 0;
     clear_accelerations();
+    viscosity_standard();
     viscosity_weiler();
     xsph();
     cfl();
@@ -1317,7 +1380,10 @@ int Sim::stage(int s) {
         case SPH_STAGE_DIVERGENCE_FINISH: divergence_finish(); break;
         case SPH_STAGE_DIVERGENCE_SOLVE: divergence_solve(); break;
         case SPH_STAGE_CLEAR_ACCEL: clear_accelerations(); break;
-        case SPH_STAGE_VISCOSITY: viscosity_weiler(); break;
+        case SPH_STAGE_VISCOSITY:
+            viscosity_standard();
+            viscosity_weiler();
+            break;
         case SPH_STAGE_XSPH: xsph(); break;
         case SPH_STAGE_CFL: cfl(); break;
         case SPH_STAGE_UPDATE_VELOCITY: update_velocity(); break;

@@ -147,6 +147,7 @@ public:
     void divergence_solve();
     void clear_accelerations();
     void viscosity_weiler();
+    void viscosity_standard();
     void xsph();
     void cfl();
     void update_velocity();

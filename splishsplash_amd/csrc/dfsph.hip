@@ -560,6 +560,65 @@ __global__ __launch_bounds__(BLOCK) void k_scale(Dev D, float *q, int which) {
 // clearAccelerations (TimeStep.cpp:35-50), Viscosity_Weiler2018::applyForces
 // (:928-940), XSPH::step (XSPH.cpp:138-216), updateTimeStepSizeCFL
 // (Simulation.cpp:415-493) and v += h a (TimeStepDFSPH.cpp:192-208).
+// Tangent frame of MathFunctions::getOrthogonalVectors (MathFunctions.cpp:234-247).
+__device__ __forceinline__ void ortho(const float3 &n, float3 &t1, float3 &t2) {
+    float3 v = make_float3(1.f, 0.f, 0.f);
+    if (fabsf(n.x) > 0.999f) v = make_float3(0.f, 1.f, 0.f);
+    t1 = make_float3(n.y * v.z - n.z * v.y, n.z * v.x - n.x * v.z, n.x * v.y - n.y * v.x);
+    t2 = make_float3(n.y * t1.z - n.z * t1.y, n.z * t1.x - n.x * t1.z, n.x * t1.y - n.y * t1.x);
+    const float l1 = sqrtf(sqn(t1.x, t1.y, t1.z)), l2 = sqrtf(sqn(t2.x, t2.y, t2.z));
+    t1 = make_float3(t1.x / l1, t1.y / l1, t1.z / l1);
+    t2 = make_float3(t2.x / l2, t2.y / l2, t2.z / l2);
+}
+
+// Boundary part of Viscosity_Standard for static boundaries
+// (Viscosity_Standard.cpp:293-380): Akinci particles with
+// d nu_b rho0 V_b / rho_i, volume maps through four samples of V_b / 4 on
+// the tangent plane at (1 - |x_b - x_i| / h) h from the boundary point.
+__device__ __forceinline__ void std_visc_boundary(const Dev &D, unsigned i, const float4 &xi, const float4 &vi,
+                                                  float4 &ai) {
+    const float h = D.K.h;
+    auto term = [&](const float4 &xs, float c, float &bx, float &by, float &bz) {
+        float3 r;
+        const float g = kernel_G(xi, xs, D.K, r);
+        const float dot = (vi.x * r.x + vi.y * r.y) + vi.z * r.z;
+        const float s = c * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+        bx += s * (g * r.x);
+        by += s * (g * r.y);
+        bz += s * (g * r.z);
+    };
+    if (D.bmode == 1) {
+        const float rhoi = D.rho[i];
+        const unsigned cb = D.cb[i];
+        for (unsigned k = 0; k < cb; k++) {
+            const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
+            term(xb, D.sviscB * (D.rho0 * xb.w / rhoi), ai.x, ai.y, ai.z);
+        }
+    } else if (D.bmode == 2) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
+            float3 nrm = make_float3(xb.x - xi.x, xb.y - xi.y, xb.z - xi.z);
+            const float nl = sqrtf(sqn(nrm.x, nrm.y, nrm.z));
+            if (!(nl > 0.0001f)) continue;
+            nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
+            float3 t1, t2;
+            ortho(nrm, t1, t2);
+            const float dist = (1.0f - nl / h) * h, c = D.sviscB * (0.25f * xb.w);
+            float bx = 0.f, by = 0.f, bz = 0.f;  // a1 + a2 + a3 + a4
+            for (int q = 0; q < 4; q++) {
+                const float3 t = q < 2 ? t1 : t2;
+                const float sg = (q % 2 == 0) ? -1.f : 1.f;
+                term(make_float4(xb.x + sg * t.x * dist, xb.y + sg * t.y * dist, xb.z + sg * t.z * dist, 0.f), c, bx,
+                     by, bz);
+            }
+            ai.x += bx;
+            ai.y += by;
+            ai.z += bz;
+        }
+    }
+}
+
 template <bool CLEAR, bool APPLY, bool XSPH, bool CFL, bool VUPD, bool SCALEPV, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
     __shared__ double lds[BLOCK / 64];
@@ -573,18 +632,33 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
     double vm = 0.0;
     float4 vi = make_float4(0.f, 0.f, 0.f, 0.f), xi = vi;
     float sx = 0.f, sy = 0.f, sz = 0.f;  // XSPH fluid sum
+    float ux = 0.f, uy = 0.f, uz = 0.f;  // standard viscosity fluid sum
     if (act) {
         vi = D.v[i];
         if (XSPH) {
             xi = D.x[i];
-            if (D.xsph != 0.f)
+            const bool xs = D.xsph != 0.f, sv = D.svisc != 0.f;
+            if (xs || sv)
                 fluid_nbrs<LPP>(D, i, hl, D.cf[i], [&](unsigned j) {
                     const float4 vj = D.v[j], xj = D.x[j];  // xj.w: m_j / rho_j
-                    const float w = kernel_W(xi, xj, D.K);
-                    const float s = invH * D.xsph * xj.w;
-                    sx -= s * (vi.x - vj.x) * w;
-                    sy -= s * (vi.y - vj.y) * w;
-                    sz -= s * (vi.z - vj.z) * w;
+                    if (sv) {
+                        // Viscosity_Standard (Viscosity_Standard.cpp:281-289):
+                        // d nu m_j/rho_j (v_ij . x_ij) / (|x_ij|^2 + 0.01 h^2) gradW
+                        float3 r;
+                        const float g = kernel_G(xi, xj, D.K, r);
+                        const float dot = ((vi.x - vj.x) * r.x + (vi.y - vj.y) * r.y) + (vi.z - vj.z) * r.z;
+                        const float s = D.svisc * xj.w * dot / (sqn(r.x, r.y, r.z) + 0.01f * D.hs2);
+                        ux += s * (g * r.x);
+                        uy += s * (g * r.y);
+                        uz += s * (g * r.z);
+                    }
+                    if (xs) {
+                        const float w = kernel_W(xi, xj, D.K);
+                        const float s = invH * D.xsph * xj.w;
+                        sx -= s * (vi.x - vj.x) * w;
+                        sy -= s * (vi.y - vj.y) * w;
+                        sz -= s * (vi.z - vj.z) * w;
+                    }
                 });
         }
     }
@@ -592,6 +666,9 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
         sx = lane_sum<LPP>(sx);
         sy = lane_sum<LPP>(sy);
         sz = lane_sum<LPP>(sz);
+        ux = lane_sum<LPP>(ux);
+        uy = lane_sum<LPP>(uy);
+        uz = lane_sum<LPP>(uz);
     }
     if (act && hl == 0) {
         float4 ai = CLEAR ? make_float4(D.grav.x, D.grav.y, D.grav.z, 0.f) : D.a[i];
@@ -605,6 +682,13 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
             D.vdiff[i] = make_float4(dx, dy, dz, 0.f);
         }
         if (XSPH) {
+            // computeViscosity before computeXSPH (Simulation.cpp:495-507)
+            if (D.svisc != 0.f) {
+                ai.x += ux;
+                ai.y += uy;
+                ai.z += uz;
+                if (D.sviscB != 0.f) std_visc_boundary(D, i, xi, vi, ai);
+            }
             ai.x += sx;
             ai.y += sy;
             ai.z += sz;
@@ -777,16 +861,6 @@ __device__ __forceinline__ void sym_inverse(const double m[6], double inv[6]) {
     inv[5] = (a * d - b * b) * id;
 }
 
-// Tangent frame of MathFunctions::getOrthogonalVectors (MathFunctions.cpp:234-247).
-__device__ __forceinline__ void ortho(const float3 &n, float3 &t1, float3 &t2) {
-    float3 v = make_float3(1.f, 0.f, 0.f);
-    if (fabsf(n.x) > 0.999f) v = make_float3(0.f, 1.f, 0.f);
-    t1 = make_float3(n.y * v.z - n.z * v.y, n.z * v.x - n.x * v.z, n.x * v.y - n.y * v.x);
-    t2 = make_float3(n.y * t1.z - n.z * t1.y, n.z * t1.x - n.x * t1.z, n.x * t1.y - n.y * t1.x);
-    const float l1 = sqrtf(sqn(t1.x, t1.y, t1.z)), l2 = sqrtf(sqn(t2.x, t2.y, t2.z));
-    t1 = make_float3(t1.x / l1, t1.y / l1, t1.z / l1);
-    t2 = make_float3(t2.x / l2, t2.y / l2, t2.z / l2);
-}
 
 // Boundary samples of the Weiler2018 operator for particle i: calls
 // f(xs, c) for every boundary sample position xs with its coefficient
@@ -1423,6 +1497,12 @@ __global__ __launch_bounds__(BLOCK) void k_collect(Dev D) {
 DFSPH::DFSPH(const sph_dfsph_params_t &p) : P(p) {
     require_device();
     SPH_REQUIRE(P.particle_radius > 0 && P.density0 > 0, SPH_ERR_INVALID, "particle_radius and density0 must be > 0");
+    SPH_REQUIRE(P.viscosity_method == SPH_VISCOSITY_NONE || P.viscosity_method == SPH_VISCOSITY_STANDARD ||
+                    P.viscosity_method == SPH_VISCOSITY_WEILER2018,
+                SPH_ERR_INVALID, "viscosity_method: only none (0), standard (1) and Weiler2018 (6) are implemented");
+    SPH_REQUIRE(P.boundary_handling_method == SPH_BOUNDARY_NONE || P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 ||
+                    P.boundary_handling_method == SPH_BOUNDARY_BENDER2019,
+                SPH_ERR_INVALID, "boundary_handling_method: only none (-1), Akinci2012 (0) and Bender2019 (2)");
     r = P.particle_radius;
     hs = 4.0 * r;
     const double diam = 2.0 * r;
@@ -1523,6 +1603,9 @@ Dev DFSPH::dev() const {
     D.maxErrorV = P.max_error_v;
     D.xsph = (float)P.xsph;
     D.xsphB = (float)P.xsph_boundary;
+    const bool stdv = P.viscosity_method == SPH_VISCOSITY_STANDARD;
+    D.svisc = stdv ? (float)(10.0 * P.viscosity) : 0.f;
+    D.sviscB = stdv ? (float)(10.0 * P.viscosity_boundary) : 0.f;
     D.cflMethod = P.cfl_method;
     D.cflFactor = P.cfl_factor;
     D.cflMin = P.cfl_min_time_step_size;
@@ -2195,9 +2278,12 @@ void DFSPH::finalize(bool scale_p, bool update_x) {
         LAUNCH("finalize", (k_final<false, true>), D);
 }
 
-void DFSPH::post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupd, bool scale_pv) {
-    const Dev D = dev();
-    if (xsph && P.xsph != 0.0 && !mrho_w_) LAUNCH("mass_density_copy", k_mrho_to_w, D);
+void DFSPH::post_forces(bool clear, bool apply, int expl, bool cfl, bool vupd, bool scale_pv) {
+    Dev D = dev();
+    if (!(expl & 1)) D.xsph = D.xsphB = 0.f;
+    if (!(expl & 2)) D.svisc = D.sviscB = 0.f;
+    const bool xsph = D.xsph != 0.f || D.xsphB != 0.f || D.svisc != 0.f;
+    if ((D.xsph != 0.f || D.svisc != 0.f) && !mrho_w_) LAUNCH("mass_density_copy", k_mrho_to_w, D);
     // the step uses the full fusion; single stages use the one-flag forms
     if (clear && !apply && !xsph && !cfl && !vupd && !scale_pv)
         LAUNCH_LPP("clear_accel", k_post, true, false, false, false, false, false);
@@ -2301,12 +2387,21 @@ int DFSPH::run_cg() {
     return hctl_->cg_it;
 }
 
+// computeViscosity. Weiler2018: the implicit solve (its result is applied
+// with the other forces of the step, or here for a stand-alone stage);
+// the explicit standard viscosity is part of the step's force sweep and
+// runs here only as a stand-alone stage.
 void DFSPH::viscosity(bool apply_separately) {
-    if (P.viscosity_method != SPH_VISCOSITY_WEILER2018 || n_ == 0) return;
+    if (n_ == 0) return;
+    if (P.viscosity_method == SPH_VISCOSITY_STANDARD) {
+        if (apply_separately) post_forces(false, false, 2, false, false, false);
+        return;
+    }
+    if (P.viscosity_method != SPH_VISCOSITY_WEILER2018) return;
     cgLast_ = run_cg();
     lastCg_ = cgLast_;
     totVisco_ += cgLast_;
-    if (apply_separately) post_forces(false, true, false, false, false, false);
+    if (apply_separately) post_forces(false, true, 0, false, false, false);
 }
 
 void DFSPH::step() {
@@ -2326,14 +2421,15 @@ void DFSPH::step() {
         itVLast_ = 0;
     }
     const bool weiler = P.viscosity_method == SPH_VISCOSITY_WEILER2018;
-    if ((weiler || P.xsph != 0.0) && n_) {
+    const bool stdv = P.viscosity_method == SPH_VISCOSITY_STANDARD;
+    if ((weiler || stdv || P.xsph != 0.0) && n_) {
         const Dev D = dev();
         LAUNCH("mass_density_copy", k_mrho_to_w, D);
         mrho_w_ = true;
     }
     if (weiler) viscosity(false);
-    post_forces(true, weiler, P.xsph != 0.0 || P.xsph_boundary != 0.0, P.cfl_method == 1 || P.cfl_method == 2, true,
-                div);
+    const int expl = ((P.xsph != 0.0 || P.xsph_boundary != 0.0) ? 1 : 0) | (stdv ? 2 : 0);
+    post_forces(true, weiler, expl, P.cfl_method == 1 || P.cfl_method == 2, true, div);
     mrho_w_ = false;
     if (div) collect(false);
     k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
@@ -2375,15 +2471,15 @@ void DFSPH::stage(int s) {
             totItV_ += itVLast_;
             div_finish(true);
             break;
-        case SPH_STAGE_CLEAR_ACCEL: post_forces(true, false, false, false, false, false); break;
+        case SPH_STAGE_CLEAR_ACCEL: post_forces(true, false, 0, false, false, false); break;
         case SPH_STAGE_VISCOSITY: viscosity(true); break;
         case SPH_STAGE_XSPH:
-            if (P.xsph != 0.0 || P.xsph_boundary != 0.0) post_forces(false, false, true, false, false, false);
+            if (P.xsph != 0.0 || P.xsph_boundary != 0.0) post_forces(false, false, 1, false, false, false);
             break;
         case SPH_STAGE_CFL:
-            if (P.cfl_method == 1 || P.cfl_method == 2) post_forces(false, false, false, true, false, false);
+            if (P.cfl_method == 1 || P.cfl_method == 2) post_forces(false, false, 0, true, false, false);
             break;
-        case SPH_STAGE_UPDATE_VELOCITY: post_forces(false, false, false, false, true, false); break;
+        case SPH_STAGE_UPDATE_VELOCITY: post_forces(false, false, 0, false, true, false); break;
         case SPH_STAGE_PRESSURE_SETUP:
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 0);
             pressure_setup();

@@ -55,6 +55,7 @@ struct Dev {
     int minIt, maxIt, maxItV;
     double maxError, maxErrorV;
     float xsph, xsphB;
+    float svisc, sviscB;  // Viscosity_Standard: d nu, d nu_b (0: off)
     int cflMethod;
     double cflFactor, cflMin, cflMax, radius;
     // Weiler2018
@@ -169,7 +170,8 @@ private:
     void scale(float *p, int which);
     void clear_accel();
     void viscosity(bool apply_separately);
-    void post_forces(bool clear, bool apply, bool xsph, bool cfl, bool vupdate, bool scale_pv);
+    // expl: explicit neighbour forces, bit 0 XSPH, bit 1 standard viscosity
+    void post_forces(bool clear, bool apply, int expl, bool cfl, bool vupdate, bool scale_pv);
     void pressure_setup();
     void pressure_iteration(bool force);
     void pressure_finish(bool finalize);

@@ -9,6 +9,7 @@ BOUNDARY_NONE = -1
 BOUNDARY_AKINCI2012 = 0
 BOUNDARY_BENDER2019 = 2
 VISCOSITY_NONE = 0
+VISCOSITY_STANDARD = 1
 VISCOSITY_WEILER2018 = 6
 
 STAGES = {

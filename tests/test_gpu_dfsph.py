@@ -10,7 +10,7 @@ import pytest
 
 from oracle import oracle
 from splishsplash_amd import capi, scenes
-from splishsplash_amd.params import BOUNDARY_BENDER2019, VISCOSITY_WEILER2018, default_params
+from splishsplash_amd.params import BOUNDARY_BENDER2019, VISCOSITY_STANDARD, VISCOSITY_WEILER2018, default_params
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,7 @@ OUTPUTS = {
     "divergence_iter": ["pressure acceleration", "p_v / rho^2"],
     "divergence_finish": ["velocity", "factor", "p_v / rho^2"],
     "clear_accel": ["acceleration"],
+    "viscosity": ["acceleration"],
     "xsph": ["acceleration"],
     "update_velocity": ["velocity"],
     "pressure_setup": ["advected density", "factor", "p / rho^2"],
@@ -72,7 +73,7 @@ def perturbed_state(x, seed=0):
     return v, p, pv
 
 
-def run_stage_sequence(g, o, seq, extra_state=()):
+def run_stage_sequence(g, o, seq, extra_state=(), tol=None):
     report = {}
     for st in seq:
         sync_state(g, o, extra_state)
@@ -81,7 +82,8 @@ def run_stage_sequence(g, o, seq, extra_state=()):
         for nm in OUTPUTS.get(st, []):
             e = rel(g.get(nm), o.get(nm))
             report[(st, nm)] = e
-            assert e <= TOL, f"{st}: {nm} rel err {e:.3e}"
+            bound = (tol or {}).get(st, TOL)
+            assert e <= bound, f"{st}: {nm} rel err {e:.3e}"
         if st == "cfl":
             hg, ho = g.counters()["time_step_size"], o.counters()["time_step_size"]
             assert hg == pytest.approx(ho, rel=1e-6)
@@ -348,3 +350,49 @@ def test_emitted_particles_keep_state_and_join_the_step():
     a.step(3)
     b.step(3)
     np.testing.assert_array_equal(a.get("position"), b.get("position"))
+
+
+@pytest.mark.parametrize("bmode", ["akinci", "bender"])
+def test_standard_viscosity_matches_oracle(bmode):
+    # Viscosity_Standard (explicit, the method of the reference's own dam
+    # break scenes) with fluid and boundary viscosity: the viscosity stage
+    # and every other stage from identical states, then whole steps.
+    if bmode == "akinci":
+        sc = scenes.dam_break_small(per_side=20)
+        x, prm = sc["x"] - np.float32(0.06), sc["params"]
+        kw = dict(boundary_x=sc["boundary_x"])
+        extra = ()
+    else:
+        x, prm, vm = vm_scene()
+        kw = dict(vm=vm)
+        extra = ("boundary volume", "boundary x")
+    prm.viscosity_method = VISCOSITY_STANDARD
+    prm.viscosity = 0.05
+    prm.viscosity_boundary = 0.3
+    v, p, pv = perturbed_state(x, seed=6)
+    g, o = make_pair(prm, x, v, **kw)
+    g.set_time(0.0, 0.004)
+    o.set_time(0.0, 0.004)
+    g.stage("neighborhood_search")
+    o.stage("neighborhood_search")
+    if bmode == "bender":
+        sync_state(g, o)
+        g.stage("volume_maps")
+        o.stage("volume_maps")
+    g.set("p / rho^2", p)
+    g.set("p_v / rho^2", pv)
+    seq = list(STEP_SEQ)
+    seq.insert(seq.index("clear_accel") + 1, "viscosity")
+    # With volume maps the boundary term samples four points on the tangent
+    # plane of each boundary point; the device forms them in FP32, the
+    # oracle in FP64, so a sample can fall on the other side of a kernel
+    # table step (a 1e-4 change of that term), as for the Weiler2018 samples.
+    run_stage_sequence(g, o, seq, extra_state=extra, tol={"viscosity": 1e-4} if bmode == "bender" else None)
+    step_tol = 5e-4 if bmode == "bender" else 1e-4  # the sample-point effect above, through a whole step
+    for step in range(4):
+        sync_state(g, o, extra)
+        g.step(1)
+        o.step(1)
+        assert g.counters()["iterations"] == o.counters()["iterations"], step
+        for nm in ("position", "velocity"):
+            assert rel(g.get(nm), o.get(nm)) <= step_tol, (step, nm)
