@@ -393,6 +393,43 @@ int sph_dfsph_get_training(sph_dfsph_t s, int solver, float *x, float *t,
 int sph_kernel_eval(double support_radius, const float *r, size_t n,
                     float *w, float *g);
 
+/* ------------------------------------------------------------------------ */
+/* Scene front-end                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct sph_scene *sph_scene_t;
+
+/* SceneLoader::readScene (SPlisHSPlasH/Utilities/SceneLoader.cpp:13-150)
+ * for what this time step runs. Read: "Configuration" (particleRadius,
+ * timeStepSize, gravitation, cflMethod, cflFactor, cflMin/MaxTimeStepSize,
+ * boundaryHandlingMethod and the "DFSPH" section), the fluid's "Materials"
+ * entry (density0, viscosityMethod 0/1/6 with its "Standard viscosity" or
+ * "Weiler et al. 2018" section, the "XSPH" section), "FluidBlocks" sampled
+ * as SimulatorBase::createFluidBlocks (Simulator/SimulatorBase.cpp:
+ * 1418-1516) with their initial velocities (:1079-1093), and static
+ * "RigidBodies" whose OBJ geometry is an axis-aligned box or a sphere
+ * (Bender2019: one volume map each with mapInvert / mapThickness /
+ * mapResolution; Akinci2012: their surfaces sampled at 2r). Other keys that
+ * do not change the simulation are ignored; anything that would (dynamic
+ * bodies, emitters, fluid models from files, other methods, several fluid
+ * ids) fails with SPH_ERR_INVALID naming the key. Relative geometry paths
+ * are resolved against the scene file's folder. */
+int sph_scene_load(const char *path, sph_scene_t *out);
+int sph_scene_destroy(sph_scene_t s);
+int sph_scene_params(sph_scene_t s, sph_dfsph_params_t *params);
+int sph_scene_counts(sph_scene_t s, size_t *n_fluid,
+                     size_t *n_boundary_particles, int *n_bodies);
+/* Fluid positions and velocities (3 floats each, block by block in the
+ * reference's sampling order) and Akinci boundary particles; any pointer
+ * may be null. */
+int sph_scene_particles(sph_scene_t s, float *x, float *v, float *boundary_x);
+/* Body k: kind 0 box (lo, hi) or 1 sphere (centre, radius), and its map
+ * settings. */
+int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3],
+                   double centre[3], double *radius, int *invert,
+                   double *thickness, unsigned resolution[3]);
+/* A DFSPH handle with the scene's parameters, fluid and boundaries. */
+int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out);
+
 #ifdef __cplusplus
 }
 #endif

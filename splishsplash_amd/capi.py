@@ -67,6 +67,13 @@ _SIGS = {
     "sph_dfsph_destroy": (C.c_int, [_vp]),
     "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_reset": (C.c_int, [_vp]),
+    "sph_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "sph_scene_destroy": (C.c_int, [_vp]),
+    "sph_scene_params": (C.c_int, [_vp, C.c_void_p]),
+    "sph_scene_counts": (C.c_int, [_vp, _szp, _szp, C.POINTER(C.c_int)]),
+    "sph_scene_particles": (C.c_int, [_vp, _fp, _fp, _fp]),
+    "sph_scene_body": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, C.POINTER(C.c_int), _dp, _up]),
+    "sph_scene_create_dfsph": (C.c_int, [_vp, C.POINTER(_vp)]),
     "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_boundary_particles": (C.c_int, [_vp, _fp, C.c_size_t]),
     "sph_dfsph_set_volume_map_box": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _up]),
@@ -322,17 +329,70 @@ class NeighborhoodSearch:
         return off, idx
 
 
+class Scene:
+    """A SPlisHSPlasH scene file read by the library (sph_scene_load):
+    parameters, sampled fluid blocks, static boxes and spheres."""
+
+    def __init__(self, path):
+        h = _vp()
+        _chk(lib().sph_scene_load(os.fsencode(path), C.byref(h)))
+        self.h = h
+        self.params = DFSPHParams()
+        _chk(lib().sph_scene_params(self.h, C.byref(self.params)))
+        nf, nb, nbod = C.c_size_t(), C.c_size_t(), C.c_int()
+        _chk(lib().sph_scene_counts(self.h, C.byref(nf), C.byref(nb), C.byref(nbod)))
+        self.n, self.nb, self.nbodies = nf.value, nb.value, nbod.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.sph_scene_destroy(self.h)
+            self.h = None
+
+    def particles(self):
+        """(x, v, boundary_x) as float32 arrays of shape (n, 3)."""
+        x = np.zeros((self.n, 3), np.float32)
+        v = np.zeros((self.n, 3), np.float32)
+        bx = np.zeros((self.nb, 3), np.float32)
+        _chk(lib().sph_scene_particles(self.h, x.ctypes.data_as(_fp), v.ctypes.data_as(_fp), bx.ctypes.data_as(_fp)))
+        return x, v, bx
+
+    def body(self, k):
+        kind, inv = C.c_int(), C.c_int()
+        lo, hi, cen = (np.zeros(3) for _ in range(3))
+        rad, th = C.c_double(), C.c_double()
+        res = np.zeros(3, np.uint32)
+        _chk(lib().sph_scene_body(self.h, k, C.byref(kind), lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp),
+                                  cen.ctypes.data_as(_dp), C.byref(rad), C.byref(inv), C.byref(th),
+                                  res.ctypes.data_as(_up)))
+        return {"kind": "box" if kind.value == 0 else "sphere", "lo": lo, "hi": hi, "centre": cen,
+                "radius": rad.value, "invert": bool(inv.value), "thickness": th.value, "resolution": res}
+
+    def create_dfsph(self):
+        """A TimeStepDFSPH with the scene's parameters, fluid and boundaries."""
+        h = _vp()
+        _chk(lib().sph_scene_create_dfsph(self.h, C.byref(h)))
+        return TimeStepDFSPH(self.params, _handle=h, _n=self.n, _nb=self.nb)
+
+
+def load_scene(path):
+    """Reads a scene file and returns a ready-to-step TimeStepDFSPH."""
+    return Scene(path).create_dfsph()
+
+
 class TimeStepDFSPH:
     """Device-resident DFSPH time step (SPH::TimeStepDFSPH) with the fluid,
     Akinci2012 boundary particles or a Bender2019 volume-map box."""
 
-    def __init__(self, params=None, **kw):
+    def __init__(self, params=None, _handle=None, _n=0, _nb=0, **kw):
         self.params = params if params is not None else default_params(**kw)
-        h = _vp()
-        _chk(lib().sph_dfsph_create(C.byref(self.params), C.byref(h)))
+        if _handle is None:
+            h = _vp()
+            _chk(lib().sph_dfsph_create(C.byref(self.params), C.byref(h)))
+        else:
+            h = _handle
         self.h = h
-        self.n = 0
-        self.nb = 0
+        self.n = _n
+        self.nb = _nb
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -1,0 +1,642 @@
+// scene.hip -- scene front-end (host code): reads a SPlisHSPlasH scene file
+// (JSON, the keys of SceneLoader::readScene, SPlisHSPlasH/Utilities/
+// SceneLoader.cpp:13-150, with the parameter names of the objects it fills),
+// samples its fluid blocks as SimulatorBase::createFluidBlocks does
+// (Simulator/SimulatorBase.cpp:1418-1516), turns its static boxes and
+// spheres into the handle's boundaries and creates a DFSPH handle.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sph {
+namespace {
+
+// ------------------------------------------------------------------ JSON
+struct JValue {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    bool b = false;
+    double num = 0.0;
+    std::string str;
+    std::vector<JValue> arr;
+    std::vector<std::pair<std::string, JValue>> obj;
+    const JValue *find(const std::string &key) const {
+        if (kind != OBJ) return nullptr;
+        for (const auto &kv : obj)
+            if (kv.first == key) return &kv.second;
+        return nullptr;
+    }
+};
+
+class JParser {
+public:
+    explicit JParser(const std::string &t) : s_(t) {}
+    JValue parse() {
+        JValue v = value();
+        ws();
+        if (p_ != s_.size()) fail("trailing characters");
+        return v;
+    }
+
+private:
+    const std::string &s_;
+    size_t p_ = 0;
+    [[noreturn]] void fail(const char *what) const {
+        size_t line = 1;
+        for (size_t k = 0; k < p_ && k < s_.size(); k++) line += s_[k] == '\n';
+        throw Error(SPH_ERR_INVALID, std::string("scene JSON: ") + what + " at line " + std::to_string(line));
+    }
+    void ws() {
+        while (p_ < s_.size() && (s_[p_] == ' ' || s_[p_] == '\t' || s_[p_] == '\n' || s_[p_] == '\r')) p_++;
+    }
+    bool eat(char c) {
+        ws();
+        if (p_ < s_.size() && s_[p_] == c) {
+            p_++;
+            return true;
+        }
+        return false;
+    }
+    void expect(char c) {
+        if (!eat(c)) fail((std::string("expected '") + c + "'").c_str());
+    }
+    bool word(const char *w) {
+        const size_t n = std::strlen(w);
+        if (s_.compare(p_, n, w) == 0) {
+            p_ += n;
+            return true;
+        }
+        return false;
+    }
+    std::string string_lit() {
+        expect('"');
+        std::string out;
+        while (p_ < s_.size() && s_[p_] != '"') {
+            char c = s_[p_++];
+            if (c == '\\') {
+                if (p_ >= s_.size()) fail("bad escape");
+                const char e = s_[p_++];
+                switch (e) {
+                    case 'n': out += '\n'; break;
+                    case 't': out += '\t'; break;
+                    case 'r': out += '\r'; break;
+                    case 'b': out += '\b'; break;
+                    case 'f': out += '\f'; break;
+                    case 'u': {
+                        if (p_ + 4 > s_.size()) fail("bad \\u escape");
+                        const unsigned cp = (unsigned)std::strtoul(s_.substr(p_, 4).c_str(), nullptr, 16);
+                        p_ += 4;
+                        if (cp < 0x80) {
+                            out += (char)cp;
+                        } else if (cp < 0x800) {
+                            out += (char)(0xC0 | (cp >> 6));
+                            out += (char)(0x80 | (cp & 0x3F));
+                        } else {
+                            out += (char)(0xE0 | (cp >> 12));
+                            out += (char)(0x80 | ((cp >> 6) & 0x3F));
+                            out += (char)(0x80 | (cp & 0x3F));
+                        }
+                        break;
+                    }
+                    default: out += e;  // \" \\ \/
+                }
+            } else {
+                out += c;
+            }
+        }
+        if (p_ >= s_.size()) fail("unterminated string");
+        p_++;
+        return out;
+    }
+    JValue value() {
+        ws();
+        if (p_ >= s_.size()) fail("unexpected end");
+        JValue v;
+        const char c = s_[p_];
+        if (c == '{') {
+            p_++;
+            v.kind = JValue::OBJ;
+            if (eat('}')) return v;
+            do {
+                ws();
+                std::string k = string_lit();
+                expect(':');
+                v.obj.emplace_back(std::move(k), value());
+            } while (eat(','));
+            expect('}');
+        } else if (c == '[') {
+            p_++;
+            v.kind = JValue::ARR;
+            if (eat(']')) return v;
+            do v.arr.push_back(value());
+            while (eat(','));
+            expect(']');
+        } else if (c == '"') {
+            v.kind = JValue::STR;
+            v.str = string_lit();
+        } else if (word("true")) {
+            v.kind = JValue::BOOL;
+            v.b = true;
+        } else if (word("false")) {
+            v.kind = JValue::BOOL;
+        } else if (word("null")) {
+            v.kind = JValue::NUL;
+        } else {
+            const char *b0 = s_.c_str() + p_;
+            char *e0 = nullptr;
+            v.num = std::strtod(b0, &e0);
+            if (e0 == b0) fail("unexpected character");
+            v.kind = JValue::NUM;
+            p_ += (size_t)(e0 - b0);
+        }
+        return v;
+    }
+};
+
+// typed reads with the reference's defaults when a key is absent
+double num(const JValue &o, const char *k, double def) {
+    const JValue *v = o.find(k);
+    if (!v || v->kind == JValue::NUL) return def;
+    if (v->kind == JValue::BOOL) return v->b ? 1.0 : 0.0;
+    if (v->kind != JValue::NUM) throw Error(SPH_ERR_INVALID, std::string("scene: \"") + k + "\" must be a number");
+    return v->num;
+}
+bool flag(const JValue &o, const char *k, bool def) { return num(o, k, def ? 1.0 : 0.0) != 0.0; }
+void vec3(const JValue &o, const char *k, double out[3]) {
+    const JValue *v = o.find(k);
+    if (!v || v->kind == JValue::NUL) return;
+    if (v->kind != JValue::ARR || v->arr.size() < 3)
+        throw Error(SPH_ERR_INVALID, std::string("scene: \"") + k + "\" must be an array of 3 numbers");
+    for (int c = 0; c < 3; c++) {
+        if (v->arr[c].kind != JValue::NUM)
+            throw Error(SPH_ERR_INVALID, std::string("scene: \"") + k + "\" must be an array of 3 numbers");
+        out[c] = v->arr[c].num;
+    }
+}
+std::string text(const JValue &o, const char *k, const std::string &def) {
+    const JValue *v = o.find(k);
+    if (!v || v->kind == JValue::NUL) return def;
+    if (v->kind == JValue::NUM) {  // ids are sometimes numbers
+        std::ostringstream s;
+        s << v->num;
+        return s.str();
+    }
+    if (v->kind != JValue::STR) throw Error(SPH_ERR_INVALID, std::string("scene: \"") + k + "\" must be a string");
+    return v->str;
+}
+[[noreturn]] void unsupported(const std::string &what) {
+    throw Error(SPH_ERR_INVALID, "scene: " + what + " is not supported by this time step");
+}
+
+std::string dir_of(const std::string &path) {
+    const size_t s = path.find_last_of('/');
+    return s == std::string::npos ? std::string(".") : path.substr(0, s);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- the scene
+struct SceneBody {
+    int kind = 0;  // 0 box, 1 sphere
+    double lo[3], hi[3], centre[3], radius = 0.0;
+    bool invert = false;
+    double thickness = 0.0;
+    unsigned res[3] = {20, 20, 20};
+};
+
+struct Scene {
+    sph_dfsph_params_t params{};
+    std::vector<float> x, v, bx;  // fluid positions, velocities, Akinci particles
+    std::vector<SceneBody> bodies;
+};
+
+// SimulatorBase::createFluidBlocks (SimulatorBase.cpp:1418-1516) for one
+// block in 3D, in double precision (the reference's Real), then rounded to
+// the device's FP32; the velocity is SimulatorBase::setInitialVelocity
+// (:1079-1093): v0 + w0 x (x - centre of the block's particles).
+static void sample_block(double r, int mode, const double bmin[3], const double bmax[3], const double tr[3],
+                         const double sc[3], const double v0[3], const double w0[3], std::vector<float> &X,
+                         std::vector<float> &V) {
+    const double diam = 2.0 * r, eps = 1.0e-9;
+    double xshift = diam, yshift = diam;
+    if (mode == 1) {
+        yshift = std::sqrt(3.0) * r + eps;
+    } else if (mode == 2) {
+        xshift = std::sqrt(6.0) * diam / 3.0 + eps;
+        yshift = std::sqrt(3.0) * r + eps;
+    }
+    double minX[3], maxX[3], diff[3];
+    for (int c = 0; c < 3; c++) {
+        minX[c] = sc[c] * bmin[c] + tr[c];
+        maxX[c] = sc[c] * bmax[c] + tr[c];
+        diff[c] = maxX[c] - minX[c];
+    }
+    if (mode == 1) {
+        diff[0] -= diam;
+        diff[2] -= diam;
+    } else if (mode == 2) {
+        diff[0] -= xshift;
+        diff[2] -= diam;
+    }
+    const int sx = (int)std::round(diff[0] / xshift) - 1;
+    const int sy = (int)std::round(diff[1] / yshift) - 1;
+    const int sz = (int)std::round(diff[2] / diam) - 1;
+    if (sx <= 1 || sy <= 1 || sz <= 1) return;  // the reference skips an empty block
+    const double start[3] = {minX[0] + diam, minX[1] + diam, minX[2] + diam};
+    std::vector<double> P;
+    P.reserve(3 * (size_t)sx * sy * sz);
+    for (int j = 0; j < sx; j++)
+        for (int k = 0; k < sy; k++)
+            for (int l = 0; l < sz; l++) {
+                double p[3] = {j * xshift + start[0], k * yshift + start[1], l * diam + start[2]};
+                if (mode == 1) {
+                    if (k % 2 == 0)
+                        p[2] += r;
+                    else
+                        p[0] += r;
+                } else if (mode == 2) {
+                    p[2] += r;
+                    double s[3] = {0.0, 0.0, 0.0};
+                    if (j % 2) s[2] += diam / (2.0 * (k % 2 ? -1 : 1));
+                    if (k % 2 == 0) s[0] += xshift / 2.0;
+                    for (int c = 0; c < 3; c++) p[c] += s[c];
+                }
+                P.insert(P.end(), p, p + 3);
+            }
+    const size_t m = P.size() / 3;
+    double com[3] = {0.0, 0.0, 0.0};
+    for (size_t i = 0; i < m; i++)
+        for (int c = 0; c < 3; c++) com[c] += P[3 * i + c];
+    for (int c = 0; c < 3; c++) com[c] /= (double)m;
+    for (size_t i = 0; i < m; i++) {
+        const double d[3] = {P[3 * i] - com[0], P[3 * i + 1] - com[1], P[3 * i + 2] - com[2]};
+        const double vv[3] = {v0[0] + (w0[1] * d[2] - w0[2] * d[1]), v0[1] + (w0[2] * d[0] - w0[0] * d[2]),
+                              v0[2] + (w0[0] * d[1] - w0[1] * d[0])};
+        for (int c = 0; c < 3; c++) {
+            X.push_back((float)P[3 * i + c]);
+            V.push_back((float)vv[c]);
+        }
+    }
+}
+
+// Vertices of an OBJ mesh ("v x y z" lines).
+static std::vector<double> obj_vertices(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw Error(SPH_ERR_INVALID, "scene: cannot open geometry file " + path);
+    std::vector<double> out;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.size() < 2 || line[0] != 'v' || (line[1] != ' ' && line[1] != '\t')) continue;
+        std::istringstream s(line.substr(2));
+        double a, b, c;
+        if (s >> a >> b >> c) out.insert(out.end(), {a, b, c});
+    }
+    if (out.empty()) throw Error(SPH_ERR_INVALID, "scene: no vertices in " + path);
+    return out;
+}
+
+// A static rigid body of the scene (StaticBoundarySimulator.cpp:42-160:
+// mesh scaled, rotated about `rotationAxis` by `rotationAngle`, translated)
+// recognised as an axis-aligned box (every vertex on a corner of the
+// bounding box) or a sphere (every vertex at the same distance from the
+// centre), the shapes whose distance fields the volume maps build exactly.
+static SceneBody body_from_mesh(const std::vector<double> &V, const double sc[3], const double tr[3],
+                                const double axis[3], double angle, const std::string &name) {
+    // rotation matrix (Rodrigues)
+    double n[3] = {axis[0], axis[1], axis[2]};
+    const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nl > 0.0)
+        for (double &c : n) c /= nl;
+    const double cs = std::cos(angle), sn = std::sin(angle), t = 1.0 - cs;
+    const double R[9] = {t * n[0] * n[0] + cs,        t * n[0] * n[1] - sn * n[2], t * n[0] * n[2] + sn * n[1],
+                         t * n[0] * n[1] + sn * n[2], t * n[1] * n[1] + cs,        t * n[1] * n[2] - sn * n[0],
+                         t * n[0] * n[2] - sn * n[1], t * n[1] * n[2] + sn * n[0], t * n[2] * n[2] + cs};
+    const size_t m = V.size() / 3;
+    std::vector<double> W(V.size());
+    SceneBody b;
+    for (int c = 0; c < 3; c++) {
+        b.lo[c] = 1e300;
+        b.hi[c] = -1e300;
+    }
+    for (size_t i = 0; i < m; i++) {
+        const double p[3] = {sc[0] * V[3 * i], sc[1] * V[3 * i + 1], sc[2] * V[3 * i + 2]};
+        for (int r0 = 0; r0 < 3; r0++) {
+            const double q = R[3 * r0] * p[0] + R[3 * r0 + 1] * p[1] + R[3 * r0 + 2] * p[2] + tr[r0];
+            W[3 * i + r0] = q;
+            b.lo[r0] = std::min(b.lo[r0], q);
+            b.hi[r0] = std::max(b.hi[r0], q);
+        }
+    }
+    double ext = 0.0;
+    for (int c = 0; c < 3; c++) ext = std::max(ext, b.hi[c] - b.lo[c]);
+    const double tol = 1e-5 * ext;
+    // a box: all eight corners of the bounding box are vertices and every
+    // vertex lies on its surface (a subdivided box included)
+    bool box = true;
+    unsigned corners = 0;
+    for (size_t i = 0; i < m && box; i++) {
+        int code = 0, faces = 0;
+        for (int c = 0; c < 3; c++) {
+            const bool atlo = std::fabs(W[3 * i + c] - b.lo[c]) <= tol, athi = std::fabs(W[3 * i + c] - b.hi[c]) <= tol;
+            faces += atlo || athi;
+            code |= (athi ? 1 : 0) << c;
+        }
+        if (faces == 0) box = false;
+        if (faces == 3) corners |= 1u << code;
+    }
+    if (box && corners == 0xffu) {
+        b.kind = 0;
+        return b;
+    }
+    double dmin = 1e300, dmax = 0.0, dsum = 0.0;
+    for (int c = 0; c < 3; c++) b.centre[c] = 0.5 * (b.lo[c] + b.hi[c]);
+    for (size_t i = 0; i < m; i++) {
+        const double d = std::sqrt((W[3 * i] - b.centre[0]) * (W[3 * i] - b.centre[0]) +
+                                   (W[3 * i + 1] - b.centre[1]) * (W[3 * i + 1] - b.centre[1]) +
+                                   (W[3 * i + 2] - b.centre[2]) * (W[3 * i + 2] - b.centre[2]));
+        dmin = std::min(dmin, d);
+        dmax = std::max(dmax, d);
+        dsum += d;
+    }
+    bool round = m >= 12 && dmax > 0.0 && (dmax - dmin) <= 1e-3 * dmax;
+    // the vertices reach (up to the tessellation) the bounding box on every
+    // axis, which a polyhedron with all vertices on a sphere, such as a cube,
+    // does not
+    for (int c = 0; c < 3 && round; c++)
+        if (std::fabs(0.5 * (b.hi[c] - b.lo[c]) - dmax) > 0.02 * dmax) round = false;
+    if (round) {
+        b.kind = 1;
+        b.radius = dsum / (double)m;
+        return b;
+    }
+    unsupported("geometry \"" + name + "\" (only axis-aligned boxes and spheres have exact distance fields here)");
+}
+
+// Akinci boundary particles on the surface of a body: a regular grid of
+// spacing 2r on each face of a box, a Fibonacci lattice of about the same
+// spacing on a sphere. (The reference samples the mesh surface with Poisson
+// disks, StaticBoundarySimulator.cpp:96-120; any even sampling at ~2r
+// spacing gives the same boundary volumes up to the sampling noise.)
+static void sample_surface(const SceneBody &b, double r, std::vector<float> &out) {
+    const double sp = 2.0 * r;
+    if (b.kind == 0) {
+        int n[3];
+        for (int c = 0; c < 3; c++) n[c] = std::max(1, (int)std::lround((b.hi[c] - b.lo[c]) / sp));
+        for (int i = 0; i <= n[0]; i++)
+            for (int j = 0; j <= n[1]; j++)
+                for (int k = 0; k <= n[2]; k++) {
+                    if (i != 0 && i != n[0] && j != 0 && j != n[1] && k != 0 && k != n[2]) continue;
+                    const int id[3] = {i, j, k};
+                    for (int c = 0; c < 3; c++) out.push_back((float)(b.lo[c] + (b.hi[c] - b.lo[c]) * id[c] / n[c]));
+                }
+    } else {
+        const double area = 4.0 * M_PI * b.radius * b.radius;
+        const int m = std::max(12, (int)std::lround(area / (sp * sp)));
+        const double ga = M_PI * (3.0 - std::sqrt(5.0));
+        for (int i = 0; i < m; i++) {
+            const double y = 1.0 - 2.0 * (i + 0.5) / m, rr = std::sqrt(std::max(0.0, 1.0 - y * y)), th = ga * i;
+            out.push_back((float)(b.centre[0] + b.radius * rr * std::cos(th)));
+            out.push_back((float)(b.centre[1] + b.radius * y));
+            out.push_back((float)(b.centre[2] + b.radius * rr * std::sin(th)));
+        }
+    }
+}
+
+static Scene *read_scene(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw Error(SPH_ERR_INVALID, "scene: cannot open " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string txt = ss.str();
+    const JValue root = JParser(txt).parse();
+    if (root.kind != JValue::OBJ) throw Error(SPH_ERR_INVALID, "scene: the file is not a JSON object");
+    std::unique_ptr<Scene> S(new Scene());
+    sph_dfsph_params_t &P = S->params;
+    sph_dfsph_default_params(&P);
+    static const JValue empty_obj = [] {
+        JValue v;
+        v.kind = JValue::OBJ;
+        return v;
+    }();
+
+    // Configuration: Simulation, TimeManager, TimeStepDFSPH's "DFSPH" section
+    const JValue *cfgp = root.find("Configuration");
+    const JValue &cfg = cfgp ? *cfgp : empty_obj;
+    if (num(cfg, "simulationMethod", 4) != 4) unsupported("\"simulationMethod\" other than DFSPH (4)");
+    if (flag(cfg, "sim2D", false)) unsupported("\"sim2D\"");
+    for (const char *k : {"kernel", "gradKernel"})
+        if (cfg.find(k) && num(cfg, k, 4) != 4) unsupported(std::string("\"") + k + "\" other than the precomputed cubic spline (4)");
+    P.particle_radius = num(cfg, "particleRadius", P.particle_radius);
+    P.time_step_size = num(cfg, "timeStepSize", P.time_step_size);
+    vec3(cfg, "gravitation", P.gravity);
+    P.cfl_method = (int)num(cfg, "cflMethod", P.cfl_method);
+    P.cfl_factor = num(cfg, "cflFactor", P.cfl_factor);
+    P.cfl_min_time_step_size = num(cfg, "cflMinTimeStepSize", P.cfl_min_time_step_size);
+    P.cfl_max_time_step_size = num(cfg, "cflMaxTimeStepSize", P.cfl_max_time_step_size);
+    P.boundary_handling_method = (int)num(cfg, "boundaryHandlingMethod", P.boundary_handling_method);
+    if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012 && P.boundary_handling_method != SPH_BOUNDARY_BENDER2019)
+        unsupported("\"boundaryHandlingMethod\" " + std::to_string(P.boundary_handling_method));
+    if (const JValue *d = cfg.find("DFSPH")) {
+        P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
+        P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
+        P.max_error = num(*d, "maxError", P.max_error);
+        P.max_iterations_v = (int)num(*d, "maxIterationsV", P.max_iterations_v);
+        P.max_error_v = num(*d, "maxErrorV", P.max_error_v);
+        P.enable_divergence_solver = flag(*d, "enableDivergenceSolver", P.enable_divergence_solver != 0) ? 1 : 0;
+    }
+    const double r = P.particle_radius;
+
+    for (const char *k : {"FluidModels", "Emitters", "AnimationFields"}) {
+        const JValue *v = root.find(k);
+        if (v && v->kind == JValue::ARR && !v->arr.empty()) unsupported(std::string("\"") + k + "\"");
+    }
+
+    // FluidBlocks (one fluid id: one fluid model)
+    std::string fluid_id;
+    if (const JValue *blocks = root.find("FluidBlocks")) {
+        for (const JValue &b : blocks->arr) {
+            const std::string id = text(b, "id", "Fluid");
+            if (!fluid_id.empty() && id != fluid_id) unsupported("more than one fluid id (\"" + fluid_id + "\", \"" + id + "\")");
+            fluid_id = id;
+            double bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1}, tr[3] = {0, 0, 0}, sc[3] = {1, 1, 1}, v0[3] = {0, 0, 0},
+                   w0[3] = {0, 0, 0};
+            vec3(b, "start", bmin);
+            vec3(b, "end", bmax);
+            vec3(b, "translation", tr);
+            vec3(b, "scale", sc);
+            vec3(b, "initialVelocity", v0);
+            vec3(b, "initialAngularVelocity", w0);
+            const int mode = (int)num(b, "denseMode", 0);
+            if (mode < 0 || mode > 2) unsupported("\"denseMode\" " + std::to_string(mode));
+            sample_block(r, mode, bmin, bmax, tr, sc, v0, w0, S->x, S->v);
+        }
+    }
+    if (S->x.empty()) throw Error(SPH_ERR_INVALID, "scene: no fluid particles (\"FluidBlocks\" missing or empty)");
+    if (fluid_id.empty()) fluid_id = "Fluid";
+
+    // Materials: the fluid's density, viscosity and XSPH sections
+    if (const JValue *mats = root.find("Materials")) {
+        for (const JValue &m : mats->arr) {
+            if (text(m, "id", "") != fluid_id) continue;
+            P.density0 = num(m, "density0", P.density0);
+            for (const char *k : {"surfaceTensionMethod", "vorticityMethod", "dragMethod", "elasticityMethod"})
+                if (num(m, k, 0) != 0) unsupported(std::string("\"") + k + "\"");
+            const int vm = (int)num(m, "viscosityMethod", 0);
+            if (vm == SPH_VISCOSITY_STANDARD) {
+                const JValue *sec = m.find("Standard viscosity");
+                P.viscosity_method = vm;
+                P.viscosity = sec ? num(*sec, "viscosity", 0.01) : 0.01;
+                P.viscosity_boundary = sec ? num(*sec, "viscosityBoundary", 0.0) : 0.0;
+            } else if (vm == SPH_VISCOSITY_WEILER2018) {
+                const JValue *sec = m.find("Weiler et al. 2018");
+                P.viscosity_method = vm;
+                P.viscosity = sec ? num(*sec, "viscosity", 0.01) : 0.01;
+                P.viscosity_boundary = sec ? num(*sec, "viscosityBoundary", 0.0) : 0.0;
+                P.visco_max_iter = sec ? (int)num(*sec, "viscoMaxIter", 100) : 100;
+                P.visco_max_error = sec ? num(*sec, "viscoMaxError", 0.01) : 0.01;
+            } else if (vm != 0) {
+                unsupported("\"viscosityMethod\" " + std::to_string(vm));
+            }
+            if (const JValue *x = m.find("XSPH")) {
+                P.xsph = num(*x, "xsph", 0.0);
+                P.xsph_boundary = num(*x, "xsphBoundary", 0.0);
+            }
+        }
+    }
+
+    // RigidBodies: static boxes and spheres
+    const std::string base = dir_of(path);
+    if (const JValue *rbs = root.find("RigidBodies")) {
+        for (const JValue &b : rbs->arr) {
+            if (flag(b, "isDynamic", false)) unsupported("a dynamic rigid body");
+            if (flag(b, "isAnimated", false)) unsupported("an animated rigid body");
+            if (!text(b, "particleFile", "").empty()) unsupported("\"particleFile\" boundary samples");
+            if (!text(b, "mapFile", "").empty()) unsupported("\"mapFile\" precomputed volume maps");
+            std::string mesh = text(b, "geometryFile", "");
+            if (mesh.empty()) throw Error(SPH_ERR_INVALID, "scene: a rigid body without \"geometryFile\"");
+            if (mesh[0] != '/') mesh = base + "/" + mesh;
+            double tr[3] = {0, 0, 0}, sc[3] = {1, 1, 1}, ax[3] = {1, 0, 0};
+            vec3(b, "translation", tr);
+            vec3(b, "scale", sc);
+            vec3(b, "rotationAxis", ax);
+            const double angle = num(b, "rotationAngle", 0.0);
+            SceneBody body = body_from_mesh(obj_vertices(mesh), sc, tr, ax, angle, mesh);
+            body.invert = flag(b, "mapInvert", false);
+            body.thickness = num(b, "mapThickness", 0.0);
+            double res[3] = {20, 20, 20};
+            vec3(b, "mapResolution", res);
+            for (int c = 0; c < 3; c++) body.res[c] = (unsigned)std::max(1.0, res[c]);
+            if (P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012) sample_surface(body, r, S->bx);
+            S->bodies.push_back(body);
+        }
+    }
+    return S.release();
+}
+
+}  // namespace sph
+
+using sph::guarded;
+
+struct sph_scene {
+    std::unique_ptr<sph::Scene> s;
+};
+
+extern "C" {
+
+int sph_scene_load(const char *path, sph_scene_t *out) {
+    return guarded([&] {
+        SPH_REQUIRE(path && out, SPH_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<sph_scene> h(new sph_scene());
+        h->s.reset(sph::read_scene(path));
+        *out = h.release();
+    });
+}
+
+int sph_scene_destroy(sph_scene_t s) {
+    delete s;
+    return SPH_OK;
+}
+
+int sph_scene_params(sph_scene_t s, sph_dfsph_params_t *p) {
+    return guarded([&] {
+        SPH_REQUIRE(s && p, SPH_ERR_INVALID, "null argument");
+        *p = s->s->params;
+    });
+}
+
+int sph_scene_counts(sph_scene_t s, size_t *n_fluid, size_t *n_boundary_particles, int *n_bodies) {
+    return guarded([&] {
+        SPH_REQUIRE(s, SPH_ERR_INVALID, "null scene");
+        if (n_fluid) *n_fluid = s->s->x.size() / 3;
+        if (n_boundary_particles) *n_boundary_particles = s->s->bx.size() / 3;
+        if (n_bodies) *n_bodies = (int)s->s->bodies.size();
+    });
+}
+
+int sph_scene_particles(sph_scene_t s, float *x, float *v, float *boundary_x) {
+    return guarded([&] {
+        SPH_REQUIRE(s, SPH_ERR_INVALID, "null scene");
+        const sph::Scene &S = *s->s;
+        if (x) std::copy(S.x.begin(), S.x.end(), x);
+        if (v) std::copy(S.v.begin(), S.v.end(), v);
+        if (boundary_x) std::copy(S.bx.begin(), S.bx.end(), boundary_x);
+    });
+}
+
+int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3], double centre[3], double *radius,
+                   int *invert, double *thickness, unsigned resolution[3]) {
+    return guarded([&] {
+        SPH_REQUIRE(s && k >= 0 && k < (int)s->s->bodies.size(), SPH_ERR_INVALID, "body index out of range");
+        const sph::SceneBody &b = s->s->bodies[(size_t)k];
+        if (kind) *kind = b.kind;
+        for (int c = 0; c < 3; c++) {
+            if (lo) lo[c] = b.lo[c];
+            if (hi) hi[c] = b.hi[c];
+            if (centre) centre[c] = b.kind == 1 ? b.centre[c] : 0.5 * (b.lo[c] + b.hi[c]);
+            if (resolution) resolution[c] = b.res[c];
+        }
+        if (radius) *radius = b.radius;
+        if (invert) *invert = b.invert ? 1 : 0;
+        if (thickness) *thickness = b.thickness;
+    });
+}
+
+int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out) {
+    return guarded([&] {
+        SPH_REQUIRE(s && out, SPH_ERR_INVALID, "null argument");
+        *out = nullptr;
+        const sph::Scene &S = *s->s;
+        sph_dfsph_t h = nullptr;
+        int rc = sph_dfsph_create(&S.params, &h);
+        if (rc != SPH_OK) throw sph::Error(rc, sph_last_error());
+        auto chk = [&](int code) {
+            if (code != SPH_OK) {
+                const std::string m = sph_last_error();
+                sph_dfsph_destroy(h);
+                throw sph::Error(code, m);
+            }
+        };
+        chk(sph_dfsph_set_fluid(h, S.x.data(), S.v.data(), S.x.size() / 3));
+        if (S.params.boundary_handling_method == SPH_BOUNDARY_AKINCI2012) {
+            if (!S.bx.empty()) chk(sph_dfsph_add_boundary_particles(h, S.bx.data(), S.bx.size() / 3));
+        } else {
+            for (const sph::SceneBody &b : S.bodies) {
+                if (b.kind == 0)
+                    chk(sph_dfsph_set_volume_map_box(h, b.lo, b.hi, b.invert ? 1 : 0, b.thickness, b.res));
+                else
+                    chk(sph_dfsph_add_volume_map_sphere(h, b.centre, b.radius, b.invert ? 1 : 0, b.thickness, b.res));
+            }
+        }
+        *out = h;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+"""Scene front-end (sph_scene_*): the reference's scene keys, its fluid-block
+sampler (against the oracle restatement, itself pinned by lattice
+geometry), static boxes and spheres from OBJ meshes, and the rejection of
+content this time step does not run. CPU only: the reader is host code."""
+import json
+import math
+import os
+import shutil
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from oracle import scene_oracle
+from splishsplash_amd import capi
+
+DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "splishsplash_amd", "data")
+
+
+def write_scene(tmp_path, scene, meshes=("unit_box.obj", "unit_sphere.obj")):
+    for m in meshes:
+        shutil.copy(os.path.join(DATA, m), tmp_path / m)
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(scene))
+    return str(p)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_block_sampler_oracle_geometry(mode):
+    # The lattices of createFluidBlocks: regular cubic (mode 0) and offset
+    # layers (mode 1, layers sqrt(3) r apart, shifted by r) with nearest
+    # neighbours at 2r; the dense mode 2 (columns sqrt(6)/3 2r apart, layers
+    # sqrt(3) r apart, shifted by half a column and half a diameter) with
+    # nearest neighbours at sqrt(11/12) 2r.
+    r = 0.025
+    x, v = scene_oracle.fluid_block(r, mode, (0.0, 0.0, 0.0), (0.6, 0.5, 0.55))
+    assert len(x) > 500 and not v.any()
+    d, _ = cKDTree(x).query(x, k=2)
+    np.testing.assert_allclose(d[:, 1], 2 * r * (math.sqrt(11.0 / 12.0) if mode == 2 else 1.0), rtol=1e-6)
+    assert (x >= 2 * r - 1e-12).all() and (x <= np.array([0.6, 0.5, 0.55])).all()  # inside the block
+    if mode == 0:
+        np.testing.assert_allclose(x[0], [2 * r] * 3, rtol=0, atol=1e-15)
+        assert len(x) == 11 * 9 * 10
+    if mode == 2:
+        t = cKDTree(x)
+        shell = 2 * r * math.sqrt(11.0 / 12.0) * (1 + 1e-6)
+        # on the block's x and y faces: one layer neighbour and, in the next
+        # column, the particles half a diameter above and below (the corner
+        # particle has only the one above)
+        assert t.query_ball_point(x[0], shell, return_length=True) - 1 == 2
+        assert t.query_ball_point(x[1], shell, return_length=True) - 1 == 3
+
+
+def test_block_sampler_matches_oracle(tmp_path):
+    blocks = [
+        dict(denseMode=0, start=[-0.5, 0.0, -0.5], end=[0.5, 1.0, 0.5], translation=[-1.45, 0.05, 0.0]),
+        dict(denseMode=1, start=[0.0, 0.0, 0.0], end=[0.4, 0.3, 0.35], scale=[1.5, 1, 1], translation=[0.3, 0.1, -0.4],
+             initialVelocity=[0.5, -1.0, 0.25]),
+        dict(denseMode=2, start=[0.0, 0.0, 0.0], end=[0.3, 0.3, 0.3], translation=[1.0, 0.2, 0.1],
+             initialAngularVelocity=[0.0, 2.0, 1.0]),
+    ]
+    path = write_scene(tmp_path, {"Configuration": {"particleRadius": 0.02}, "FluidBlocks": blocks}, meshes=())
+    sc = capi.Scene(path)
+    x, v, _ = sc.particles()
+    xs, vs = [], []
+    for b in blocks:
+        xo, vo = scene_oracle.fluid_block(0.02, b["denseMode"], b["start"], b["end"], b.get("translation", (0, 0, 0)),
+                                          b.get("scale", (1, 1, 1)), b.get("initialVelocity", (0, 0, 0)),
+                                          b.get("initialAngularVelocity", (0, 0, 0)))
+        xs.append(xo)
+        vs.append(vo)
+    xo, vo = np.concatenate(xs), np.concatenate(vs)
+    assert len(x) == len(xo) == sc.n
+    np.testing.assert_array_equal(x, xo.astype(np.float32))
+    np.testing.assert_allclose(v, vo, rtol=1e-6, atol=1e-7)
+
+
+def test_scene_parameters_and_defaults(tmp_path):
+    scene = {
+        "Configuration": {
+            "particleRadius": 0.03, "timeStepSize": 0.002, "gravitation": [0.5, -9.0, 0.25], "cflMethod": 2,
+            "cflFactor": 0.7, "cflMinTimeStepSize": 0.0002, "cflMaxTimeStepSize": 0.004, "boundaryHandlingMethod": 0,
+            "simulationMethod": 4, "numberOfStepsPerRenderUpdate": 4,
+            "DFSPH": {"minIterations": 3, "maxIterations": 77, "maxError": 0.05, "maxIterationsV": 55,
+                      "maxErrorV": 0.2, "enableDivergenceSolver": False},
+        },
+        "Materials": [{"id": "Fluid", "density0": 998.0, "viscosityMethod": 6, "colorMapType": 1,
+                       "Weiler et al. 2018": {"viscosity": 0.02, "viscosityBoundary": 0.1, "viscoMaxIter": 50,
+                                              "viscoMaxError": 0.001},
+                       "XSPH": {"xsph": 0.03, "xsphBoundary": 0.04}}],
+        "FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}],
+        "RigidBodies": [{"geometryFile": "unit_box.obj", "translation": [0, 1, 0], "scale": [2, 2, 2],
+                         "isWall": True, "mapInvert": True, "color": [1, 0, 0, 1]}],
+    }
+    p = capi.Scene(write_scene(tmp_path, scene)).params
+    assert (p.particle_radius, p.time_step_size, list(p.gravity)) == (0.03, 0.002, [0.5, -9.0, 0.25])
+    assert (p.cfl_method, p.cfl_factor, p.cfl_min_time_step_size, p.cfl_max_time_step_size) == (2, 0.7, 0.0002, 0.004)
+    assert p.boundary_handling_method == 0
+    assert (p.min_iterations, p.max_iterations, p.max_error, p.max_iterations_v, p.max_error_v) == (3, 77, 0.05, 55, 0.2)
+    assert p.enable_divergence_solver == 0
+    assert (p.density0, p.viscosity_method, p.viscosity, p.viscosity_boundary) == (998.0, 6, 0.02, 0.1)
+    assert (p.visco_max_iter, p.visco_max_error, p.xsph, p.xsph_boundary) == (50, 0.001, 0.03, 0.04)
+    # absent keys keep the reference's defaults
+    q = capi.Scene(write_scene(tmp_path, {"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]})).params
+    d = capi.default_params()
+    for name, _ in type(q)._fields_:
+        if name != "reserved":
+            a, b = getattr(q, name), getattr(d, name)
+            assert (list(a) == list(b)) if name == "gravity" else (a == b), name
+
+
+def test_boxes_spheres_and_akinci_sampling(tmp_path):
+    scene = {
+        "Configuration": {"boundaryHandlingMethod": 0},
+        "FluidBlocks": [{"start": [-0.2, 0.0, -0.2], "end": [0.2, 0.4, 0.2]}],
+        "RigidBodies": [
+            {"geometryFile": "unit_box.obj", "translation": [0, 1, 0], "scale": [2, 2, 1], "isWall": True,
+             "mapInvert": True, "mapResolution": [30, 30, 15], "mapThickness": 0.01},
+            {"geometryFile": "unit_sphere.obj", "translation": [0.5, 0.5, 0.0], "scale": [0.2, 0.2, 0.2],
+             "rotationAxis": [0, 1, 0], "rotationAngle": 0.7},
+        ],
+    }
+    sc = capi.Scene(write_scene(tmp_path, scene))
+    assert sc.nbodies == 2
+    b0, b1 = sc.body(0), sc.body(1)
+    assert b0["kind"] == "box" and b0["invert"] and b0["thickness"] == 0.01
+    np.testing.assert_allclose(b0["lo"], [-1, 0, -0.5])
+    np.testing.assert_allclose(b0["hi"], [1, 2, 0.5])
+    assert list(b0["resolution"]) == [30, 30, 15]
+    assert b1["kind"] == "sphere" and not b1["invert"] and b1["radius"] == pytest.approx(0.2, rel=1e-9)
+    np.testing.assert_allclose(b1["centre"], [0.5, 0.5, 0.0], atol=1e-12)
+    _, _, bx = sc.particles()
+    assert sc.nb == len(bx) > 1000
+    # every Akinci sample lies on one of the two surfaces
+    on_box = np.any(np.isclose(bx, [-1, 0, -0.5], atol=1e-6) | np.isclose(bx, [1, 2, 0.5], atol=1e-6), axis=1)
+    on_sphere = np.isclose(np.linalg.norm(bx - [0.5, 0.5, 0.0], axis=1), 0.2, atol=1e-6)
+    assert (on_box | on_sphere).all() and on_sphere.sum() > 50
+    # neighbouring samples on a face sit 2r apart
+    face = bx[np.isclose(bx[:, 2], -0.5, atol=1e-6)]
+    d, _ = cKDTree(face).query(face, k=2)
+    np.testing.assert_allclose(d[:, 1], 0.05, rtol=1e-5)
+
+
+@pytest.mark.parametrize("change, key", [
+    ({"RigidBodies": [{"geometryFile": "unit_box.obj", "isDynamic": True}]}, "dynamic"),
+    ({"Emitters": [{"width": 5}]}, "Emitters"),
+    ({"Configuration": {"simulationMethod": 1}}, "simulationMethod"),
+    ({"Configuration": {"sim2D": True}}, "sim2D"),
+    ({"Configuration": {"kernel": 0}}, "kernel"),
+    ({"Configuration": {"boundaryHandlingMethod": 1}}, "boundaryHandlingMethod"),
+    ({"Materials": [{"id": "Fluid", "viscosityMethod": 3}]}, "viscosityMethod"),
+    ({"Materials": [{"id": "Fluid", "surfaceTensionMethod": 1}]}, "surfaceTensionMethod"),
+    ({"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]},
+                      {"id": "Other", "start": [1, 0, 0], "end": [1.5, 0.5, 0.5]}]}, "fluid id"),
+    ({"RigidBodies": [{"geometryFile": "unit_box.obj", "rotationAngle": 0.3}]}, "axis-aligned"),
+    ({"RigidBodies": [{"geometryFile": "tetra.obj"}]}, "geometry"),
+])
+def test_unsupported_content_is_rejected(tmp_path, change, key):
+    (tmp_path / "tetra.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\nf 1 2 4\nf 1 3 4\nf 2 3 4\n")
+    scene = {"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]}
+    for k, v in change.items():
+        scene.setdefault(k, {}) if isinstance(v, dict) else None
+        if isinstance(v, dict):
+            scene[k].update(v)
+        else:
+            scene[k] = v
+    with pytest.raises(capi.SPHError) as e:
+        capi.Scene(write_scene(tmp_path, scene))
+    assert key in str(e.value)
+
+
+def test_malformed_files_are_errors(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{"Configuration": {"particleRadius": 0.025,}')
+    with pytest.raises(capi.SPHError) as e:
+        capi.Scene(str(p))
+    assert "line 1" in str(e.value)
+    with pytest.raises(capi.SPHError):
+        capi.Scene(str(tmp_path / "missing.json"))
+    p.write_text('{"FluidBlocks": []}')
+    with pytest.raises(capi.SPHError):
+        capi.Scene(str(p))
+
+
+def test_packaged_dam_break_scene():
+    sc = capi.Scene(os.path.join(DATA, "dam_break.json"))
+    assert sc.n == 19 ** 3 and sc.nbodies == 2
+    assert sc.params.viscosity_method == 1 and sc.params.xsph == 0.01
+    assert math.isclose(sc.body(1)["radius"], 0.25, rel_tol=1e-9)
