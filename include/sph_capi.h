@@ -430,6 +430,45 @@ int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3],
 /* A DFSPH handle with the scene's parameters, fluid and boundaries. */
 int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out);
 
+/* ------------------------------------------------------------------------ */
+/* Particle files (partio BGEO)                                              */
+/* ------------------------------------------------------------------------ */
+/* ParticleExporter_Partio::writeParticlesPartio
+ * (Simulator/Exporter/ParticleExporter_Partio.cpp): the particles in
+ * particle-id order as a gzip-compressed BGEO file with the position, "id"
+ * and the ';'-separated fields (the scene's "particleAttributes", e.g.
+ * "velocity;density"; names as in sph_dfsph_get_field). */
+int sph_dfsph_write_particles(sph_dfsph_t s, const char *path,
+                              const char *fields);
+/* SimulatorBase::saveState / loadState (SimulatorBase.cpp:1689-1872) for the
+ * fluid: <prefix>_particle_Fluid.bgeo holds the fields the reference stores
+ * (writeFluidParticlesState, :2062-2128: id, state, object_id, position,
+ * velocity, p / rho^2, p_v / rho^2 and, with Weiler2018, velocity
+ * difference), readable by the reference; <prefix>.json holds the time and
+ * time step size (the reference keeps those in its own .bin). Loading sets
+ * every field of the particle file that the handle has (point i to particle
+ * i, readFluidParticlesState :2130-2190) and the time; the particle count
+ * must match. */
+int sph_dfsph_save_state(sph_dfsph_t s, const char *prefix);
+int sph_dfsph_load_state(sph_dfsph_t s, const char *prefix);
+
+/* The file format itself: BGEO version 5 as partio writes it (big-endian,
+ * gzip; point position plus point attributes of type 0 float, 1 int or
+ * 5 vector). Reading accepts plain or gzip files. */
+typedef struct sph_bgeo *sph_bgeo_t;
+int sph_bgeo_open(const char *path, sph_bgeo_t *out);
+int sph_bgeo_close(sph_bgeo_t f);
+int sph_bgeo_count(sph_bgeo_t f, size_t *n_points, int *n_attributes);
+int sph_bgeo_attribute(sph_bgeo_t f, int k, const char **name, int *type,
+                       int *size);
+/* Attribute data (n_points * size 32-bit values), or "position" (3 floats
+ * per point). */
+int sph_bgeo_data(sph_bgeo_t f, const char *name, void *out, size_t bytes);
+int sph_bgeo_write(const char *path, size_t n_points, const float *position,
+                   int n_attributes, const char *const *names,
+                   const int *types, const int *sizes,
+                   const void *const *data);
+
 #ifdef __cplusplus
 }
 #endif

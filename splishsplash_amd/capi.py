@@ -68,6 +68,16 @@ _SIGS = {
     "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_reset": (C.c_int, [_vp]),
     "sph_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "sph_dfsph_write_particles": (C.c_int, [_vp, C.c_char_p, C.c_char_p]),
+    "sph_dfsph_save_state": (C.c_int, [_vp, C.c_char_p]),
+    "sph_dfsph_load_state": (C.c_int, [_vp, C.c_char_p]),
+    "sph_bgeo_open": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "sph_bgeo_close": (C.c_int, [_vp]),
+    "sph_bgeo_count": (C.c_int, [_vp, _szp, C.POINTER(C.c_int)]),
+    "sph_bgeo_attribute": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sph_bgeo_data": (C.c_int, [_vp, C.c_char_p, C.c_void_p, C.c_size_t]),
+    "sph_bgeo_write": (C.c_int, [C.c_char_p, C.c_size_t, _fp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_void_p)]),
     "sph_scene_destroy": (C.c_int, [_vp]),
     "sph_scene_params": (C.c_int, [_vp, C.c_void_p]),
     "sph_scene_counts": (C.c_int, [_vp, _szp, _szp, C.POINTER(C.c_int)]),
@@ -374,6 +384,50 @@ class Scene:
         return TimeStepDFSPH(self.params, _handle=h, _n=self.n, _nb=self.nb)
 
 
+def read_bgeo(path):
+    """A partio BGEO file as {"position": (n, 3) float32, attribute name:
+    array} (float attributes float32, int attributes int32, vectors (n, k))."""
+    h = _vp()
+    _chk(lib().sph_bgeo_open(os.fsencode(path), C.byref(h)))
+    try:
+        n, na = C.c_size_t(), C.c_int()
+        _chk(lib().sph_bgeo_count(h, C.byref(n), C.byref(na)))
+        out = {}
+        pos = np.zeros((n.value, 3), np.float32)
+        _chk(lib().sph_bgeo_data(h, b"position", pos.ctypes.data_as(C.c_void_p), pos.nbytes))
+        out["position"] = pos
+        for k in range(na.value):
+            nm, ty, sz = C.c_char_p(), C.c_int(), C.c_int()
+            _chk(lib().sph_bgeo_attribute(h, k, C.byref(nm), C.byref(ty), C.byref(sz)))
+            a = np.zeros((n.value, sz.value), np.int32 if ty.value == 1 else np.float32)
+            _chk(lib().sph_bgeo_data(h, nm.value, a.ctypes.data_as(C.c_void_p), a.nbytes))
+            out[nm.value.decode()] = a[:, 0] if sz.value == 1 else a
+        return out
+    finally:
+        lib().sph_bgeo_close(h)
+
+
+def write_bgeo(path, position, attrs):
+    """Writes a BGEO file: `attrs` maps names to (n,) or (n, k) arrays; int
+    arrays become int attributes, (n, 3) float arrays vectors."""
+    pos = np.ascontiguousarray(position, np.float32).reshape(-1, 3)
+    names, types, sizes, bufs = [], [], [], []
+    for nm, a in attrs.items():
+        a = np.asarray(a)
+        isint = np.issubdtype(a.dtype, np.integer)
+        a = np.ascontiguousarray(a, np.int32 if isint else np.float32).reshape(len(pos), a.shape[1] if a.ndim == 2 else 1)
+        names.append(nm.encode())
+        types.append(1 if isint else (5 if a.shape[1] == 3 else 0))
+        sizes.append(a.shape[1])
+        bufs.append(a)
+    k = len(names)
+    cn = (C.c_char_p * max(k, 1))(*names)
+    ct = (C.c_int * max(k, 1))(*types)
+    cs = (C.c_int * max(k, 1))(*sizes)
+    cd = (C.c_void_p * max(k, 1))(*[b.ctypes.data for b in bufs])
+    _chk(lib().sph_bgeo_write(os.fsencode(path), len(pos), pos.ctypes.data_as(_fp), k, cn, ct, cs, cd))
+
+
 def load_scene(path):
     """Reads a scene file and returns a ready-to-step TimeStepDFSPH."""
     return Scene(path).create_dfsph()
@@ -434,6 +488,17 @@ class TimeStepDFSPH:
     def reset(self):
         """TimeStepDFSPH::reset: solver fields and iteration counters to zero."""
         _chk(lib().sph_dfsph_reset(self.h))
+
+    def write_particles(self, path, fields="velocity"):
+        """The exporter's BGEO particle file (position, id and `fields`,
+        ';'-separated)."""
+        _chk(lib().sph_dfsph_write_particles(self.h, os.fsencode(path), fields.encode()))
+
+    def save_state(self, prefix):
+        _chk(lib().sph_dfsph_save_state(self.h, os.fsencode(prefix)))
+
+    def load_state(self, prefix):
+        _chk(lib().sph_dfsph_load_state(self.h, os.fsencode(prefix)))
 
     def emit_particles(self, x, v=None):
         """Appends particles (emitter + emittedParticles): ids continue from

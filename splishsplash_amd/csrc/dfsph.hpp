@@ -90,6 +90,9 @@ public:
                             const unsigned res[3]);
     void add_volume_map(const Shape &shape, const unsigned res[3]);
     void init();
+    void init_if_needed() {
+        if (!initialized_) init();
+    }
     void reset();
     void emit_particles(const float *x, const float *v, size_t n);
     void step();
@@ -198,5 +201,11 @@ private:
     int run_cg();
     std::vector<unsigned> ids_host();
 };
+
+// Particle files (io.hip): the exporter's particle file, and the fluid part of
+// a state file, in the partio BGEO format.
+void write_particles(DFSPH &s, const std::string &path, const char *fields);
+void write_state(DFSPH &s, const std::string &path);
+void read_state(DFSPH &s, const std::string &path);
 
 }  // namespace sph

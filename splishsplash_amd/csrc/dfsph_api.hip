@@ -273,6 +273,50 @@ int sph_dfsph_set_time(sph_dfsph_t s, double time, double time_step_size) {
     });
 }
 
+int sph_dfsph_write_particles(sph_dfsph_t s, const char *path, const char *fields) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(path, SPH_ERR_INVALID, "null path");
+        write_particles(*s->s, path, fields);
+    });
+}
+
+int sph_dfsph_save_state(sph_dfsph_t s, const char *prefix) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(prefix, SPH_ERR_INVALID, "null path");
+        const std::string pre(prefix);
+        write_state(*s->s, pre + "_particle_Fluid.bgeo");
+        sph_dfsph_counters_t c;
+        s->s->counters(&c);
+        FILE *f = std::fopen((pre + ".json").c_str(), "w");
+        SPH_REQUIRE(f, SPH_ERR_INVALID, "cannot write " + pre + ".json");
+        std::fprintf(f, "{\"time\": %.17g, \"timeStepSize\": %.17g, \"numParticles\": %zu}\n", c.time,
+                     c.time_step_size, s->s->n());
+        SPH_REQUIRE(std::fclose(f) == 0, SPH_ERR_INVALID, "write failed: " + pre + ".json");
+    });
+}
+
+int sph_dfsph_load_state(sph_dfsph_t s, const char *prefix) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(prefix, SPH_ERR_INVALID, "null path");
+        const std::string pre(prefix);
+        FILE *f = std::fopen((pre + ".json").c_str(), "r");
+        SPH_REQUIRE(f, SPH_ERR_INVALID, "cannot open " + pre + ".json");
+        double t = 0.0, h = 0.0;
+        size_t n = 0;
+        const int got = std::fscanf(f, " {\"time\": %lf, \"timeStepSize\": %lf, \"numParticles\": %zu", &t, &h, &n);
+        std::fclose(f);
+        SPH_REQUIRE(got == 3, SPH_ERR_INVALID, "malformed " + pre + ".json");
+        SPH_REQUIRE(n == s->s->n(), SPH_ERR_INVALID, "the state holds " + std::to_string(n) + " particles");
+        if (!s->s->n()) return;
+        s->s->init_if_needed();
+        read_state(*s->s, pre + "_particle_Fluid.bgeo");
+        s->s->set_time(t, h);
+    });
+}
+
 int sph_dfsph_enable_timing(sph_dfsph_t s, int enable) {
     return guarded([&] {
         DCHK(s);

@@ -1,0 +1,377 @@
+// io.hip -- particle files (host code): the partio BGEO format that the
+// reference's exporter and state files use (Simulator/Exporter/
+// ParticleExporter_Partio.cpp, SimulatorBase.cpp:2062-2190), and state
+// checkpoints of a DFSPH handle built on it.
+//
+// BGEO here is Houdini's classic binary geometry as partio writes it,
+// big-endian, gzip-compressed:
+//   "Bgeo" 'V' int32 5
+//   int32 points, primitives (1), point groups (0), primitive groups (0),
+//         point attributes, vertex attributes (0), primitive attributes (1),
+//         detail attributes (0)
+//   per point attribute: uint16 name length, name, uint16 size,
+//         int32 type (0 float, 1 int, 5 vector), size x 4-byte default
+//   per point: float x, y, z, 1 (the position), then the attribute values
+//   primitive attribute "generator" (index type 4, one string "papi")
+//   one particle-system primitive: int32 0x8000, int32 points, point
+//         indices (uint16 below 65536 points, else int32), int32 0
+//   0x00 0xff
+#include <zlib.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dfsph.hpp"
+
+namespace sph {
+
+struct BgeoAttr {
+    std::string name;
+    int type = 0;   // 0 float, 1 int, 5 vector
+    int size = 1;   // components
+    std::vector<uint32_t> data;  // raw 32-bit values, size per point
+};
+
+struct BgeoFile {
+    size_t n = 0;
+    std::vector<float> pos;  // 3 per point
+    std::vector<BgeoAttr> attrs;
+    const BgeoAttr *find(const std::string &name) const {
+        for (const auto &a : attrs)
+            if (a.name == name) return &a;
+        return nullptr;
+    }
+};
+
+namespace {
+
+void put32(std::string &b, uint32_t v) {
+    const char c[4] = {(char)(v >> 24), (char)(v >> 16), (char)(v >> 8), (char)v};
+    b.append(c, 4);
+}
+void put16(std::string &b, uint16_t v) {
+    const char c[2] = {(char)(v >> 8), (char)v};
+    b.append(c, 2);
+}
+uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+void putname(std::string &b, const std::string &s) {
+    SPH_REQUIRE(s.size() < 65536, SPH_ERR_INVALID, "BGEO: attribute name too long");
+    put16(b, (uint16_t)s.size());
+    b += s;
+}
+
+struct Reader {
+    const std::string &b;
+    size_t p = 0;
+    void need(size_t k) const {
+        if (p + k > b.size()) throw Error(SPH_ERR_INVALID, "BGEO: file ends early");
+    }
+    uint32_t u32() {
+        need(4);
+        const unsigned char *c = reinterpret_cast<const unsigned char *>(b.data() + p);
+        p += 4;
+        return (uint32_t)c[0] << 24 | (uint32_t)c[1] << 16 | (uint32_t)c[2] << 8 | c[3];
+    }
+    uint16_t u16() {
+        need(2);
+        const unsigned char *c = reinterpret_cast<const unsigned char *>(b.data() + p);
+        p += 2;
+        return (uint16_t)(c[0] << 8 | c[1]);
+    }
+    float f32() {
+        const uint32_t u = u32();
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    }
+    std::string str(size_t k) {
+        need(k);
+        std::string s = b.substr(p, k);
+        p += k;
+        return s;
+    }
+};
+
+}  // namespace
+
+std::string bgeo_encode(const BgeoFile &f) {
+    std::string b;
+    b += "BgeoV";
+    put32(b, 5);
+    put32(b, (uint32_t)f.n);
+    put32(b, 1);
+    put32(b, 0);
+    put32(b, 0);
+    put32(b, (uint32_t)f.attrs.size());
+    put32(b, 0);
+    put32(b, 1);
+    put32(b, 0);
+    for (const auto &a : f.attrs) {
+        putname(b, a.name);
+        put16(b, (uint16_t)a.size);
+        put32(b, (uint32_t)a.type);
+        for (int c = 0; c < a.size; c++) put32(b, 0);
+    }
+    for (size_t i = 0; i < f.n; i++) {
+        for (int c = 0; c < 3; c++) put32(b, fbits(f.pos[3 * i + c]));
+        put32(b, fbits(1.0f));
+        for (const auto &a : f.attrs)
+            for (int c = 0; c < a.size; c++) put32(b, a.data[(size_t)a.size * i + c]);
+    }
+    putname(b, "generator");
+    put16(b, 1);
+    put32(b, 4);
+    put32(b, 1);
+    putname(b, "papi");
+    put32(b, 0x8000);
+    put32(b, (uint32_t)f.n);
+    for (size_t i = 0; i < f.n; i++) {
+        if (f.n < 65536)
+            put16(b, (uint16_t)i);
+        else
+            put32(b, (uint32_t)i);
+    }
+    put32(b, 0);
+    b += '\0';
+    b += '\xff';
+    return b;
+}
+
+BgeoFile bgeo_decode(const std::string &bytes) {
+    Reader r{bytes};
+    if (r.str(5) != "BgeoV") throw Error(SPH_ERR_INVALID, "BGEO: bad magic");
+    const uint32_t version = r.u32();
+    SPH_REQUIRE(version == 5, SPH_ERR_INVALID, "BGEO: only version 5 files are read");
+    BgeoFile f;
+    f.n = r.u32();
+    r.u32();  // primitives
+    const uint32_t pg = r.u32(), prg = r.u32(), npa = r.u32(), nva = r.u32();
+    r.u32();  // primitive attributes
+    const uint32_t nda = r.u32();
+    SPH_REQUIRE(pg == 0 && prg == 0 && nva == 0 && nda == 0, SPH_ERR_INVALID,
+                "BGEO: groups, vertex and detail attributes are not read");
+    for (uint32_t k = 0; k < npa; k++) {
+        BgeoAttr a;
+        a.name = r.str(r.u16());
+        a.size = r.u16();
+        a.type = (int)r.u32();
+        SPH_REQUIRE(a.type == 0 || a.type == 1 || a.type == 5, SPH_ERR_INVALID,
+                    "BGEO: attribute '" + a.name + "' has an unsupported type");
+        for (int c = 0; c < a.size; c++) r.u32();  // defaults
+        f.attrs.push_back(std::move(a));
+    }
+    f.pos.resize(3 * f.n);
+    for (auto &a : f.attrs) a.data.resize((size_t)a.size * f.n);
+    for (size_t i = 0; i < f.n; i++) {
+        for (int c = 0; c < 3; c++) f.pos[3 * i + c] = r.f32();
+        r.f32();  // w
+        for (auto &a : f.attrs)
+            for (int c = 0; c < a.size; c++) a.data[(size_t)a.size * i + c] = r.u32();
+    }
+    return f;  // primitives and trailer carry no particle data
+}
+
+void gz_write(const std::string &path, const std::string &bytes) {
+    gzFile g = gzopen(path.c_str(), "wb");
+    if (!g) throw Error(SPH_ERR_INVALID, "cannot write " + path);
+    const int w = bytes.empty() ? 0 : gzwrite(g, bytes.data(), (unsigned)bytes.size());
+    const int c = gzclose(g);
+    if ((size_t)w != bytes.size() || c != Z_OK) throw Error(SPH_ERR_INVALID, "write failed: " + path);
+}
+
+std::string gz_read(const std::string &path) {
+    gzFile g = gzopen(path.c_str(), "rb");  // reads plain files too
+    if (!g) throw Error(SPH_ERR_INVALID, "cannot open " + path);
+    std::string out;
+    char buf[1 << 16];
+    int k;
+    while ((k = gzread(g, buf, sizeof buf)) > 0) out.append(buf, (size_t)k);
+    const bool bad = k < 0;
+    gzclose(g);
+    if (bad) throw Error(SPH_ERR_INVALID, "read failed: " + path);
+    return out;
+}
+
+namespace {
+
+// Field of the handle: components (3 vector, 1 scalar) and BGEO type.
+struct FieldKind {
+    int comps, type;
+};
+FieldKind field_kind(const std::string &name) {
+    if (name == "position" || name == "velocity" || name == "acceleration" || name == "pressure acceleration" ||
+        name == "velocity difference")
+        return {3, 5};
+    if (name == "id") return {1, 1};
+    return {1, 0};
+}
+
+// The handle's particles (id order) with the named fields, position first.
+BgeoFile handle_particles(DFSPH &s, const std::vector<std::string> &fields) {
+    BgeoFile f;
+    f.n = s.n();
+    f.pos.resize(3 * f.n);
+    s.get_field("position", f.pos.data(), f.pos.size() * 4);
+    for (const auto &name : fields) {
+        if (name == "position") continue;  // written as the point position
+        BgeoAttr a;
+        a.name = name;
+        if (name == "state" || name == "object_id") {  // all particles active, one object
+            a.type = 1;
+            a.size = 1;
+            a.data.assign(f.n, 0u);
+        } else {
+            const FieldKind k = field_kind(name);
+            a.type = k.type;
+            a.size = k.comps;
+            a.data.resize((size_t)k.comps * f.n);
+            s.get_field(name, a.data.data(), a.data.size() * 4);
+        }
+        f.attrs.push_back(std::move(a));
+    }
+    return f;
+}
+
+std::vector<std::string> split(const char *s, char sep) {
+    std::vector<std::string> out;
+    if (!s) return out;
+    std::string cur;
+    for (const char *p = s;; p++) {
+        if (*p == sep || *p == '\0') {
+            if (!cur.empty()) out.push_back(cur);
+            cur.clear();
+            if (*p == '\0') break;
+        } else {
+            cur += *p;
+        }
+    }
+    return out;
+}
+
+}  // namespace
+
+// ParticleExporter_Partio::writeParticlesPartio: position, id, then the
+// requested fields (';'-separated, as the scene's "particleAttributes").
+void write_particles(DFSPH &s, const std::string &path, const char *fields) {
+    std::vector<std::string> names = {"id"};
+    for (const auto &n : split(fields, ';'))
+        if (n != "position" && n != "id") names.push_back(n);
+    gz_write(path, bgeo_encode(handle_particles(s, names)));
+}
+
+// SimulatorBase::writeFluidParticlesState: the fields the reference stores
+// (FluidModel.cpp:61-66, TimeStepDFSPH.cpp:51-52, Viscosity_Weiler2018.cpp:
+// 33), in its registration order.
+void write_state(DFSPH &s, const std::string &path) {
+    std::vector<std::string> names = {"id", "state", "object_id", "velocity", "p / rho^2", "p_v / rho^2"};
+    if (s.P.viscosity_method == SPH_VISCOSITY_WEILER2018) names.push_back("velocity difference");
+    gz_write(path, bgeo_encode(handle_particles(s, names)));
+}
+
+// SimulatorBase::readFluidParticlesState: every attribute whose name is a
+// field of the handle is set, point i to particle i; the point position is
+// the position. Attributes the handle does not hold ("state", "object_id",
+// "id": the handle's ids are the point order) are skipped.
+void read_state(DFSPH &s, const std::string &path) {
+    const BgeoFile f = bgeo_decode(gz_read(path));
+    SPH_REQUIRE(f.n == s.n(), SPH_ERR_INVALID,
+                "particle file holds " + std::to_string(f.n) + " particles, the handle " + std::to_string(s.n()));
+    s.set_field("position", f.pos.data(), f.pos.size() * 4);
+    for (const auto &a : f.attrs) {
+        if (a.name == "id" || a.name == "state" || a.name == "object_id" || a.name == "position") continue;
+        const FieldKind k = field_kind(a.name);
+        SPH_REQUIRE(a.size == k.comps, SPH_ERR_INVALID, "attribute '" + a.name + "' has the wrong size");
+        s.set_field(a.name, a.data.data(), a.data.size() * 4);
+    }
+}
+
+}  // namespace sph
+
+// ------------------------------------------------------------------ C-ABI
+using sph::guarded;
+
+struct sph_bgeo {
+    sph::BgeoFile f;
+};
+
+extern "C" {
+
+int sph_bgeo_open(const char *path, sph_bgeo_t *out) {
+    return guarded([&] {
+        SPH_REQUIRE(path && out, SPH_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<sph_bgeo> h(new sph_bgeo());
+        h->f = sph::bgeo_decode(sph::gz_read(path));
+        *out = h.release();
+    });
+}
+
+int sph_bgeo_close(sph_bgeo_t f) {
+    delete f;
+    return SPH_OK;
+}
+
+int sph_bgeo_count(sph_bgeo_t f, size_t *n, int *nattr) {
+    return guarded([&] {
+        SPH_REQUIRE(f, SPH_ERR_INVALID, "null file");
+        if (n) *n = f->f.n;
+        if (nattr) *nattr = (int)f->f.attrs.size();
+    });
+}
+
+int sph_bgeo_attribute(sph_bgeo_t f, int k, const char **name, int *type, int *size) {
+    return guarded([&] {
+        SPH_REQUIRE(f && k >= 0 && k < (int)f->f.attrs.size(), SPH_ERR_INVALID, "attribute index out of range");
+        const sph::BgeoAttr &a = f->f.attrs[(size_t)k];
+        if (name) *name = a.name.c_str();
+        if (type) *type = a.type;
+        if (size) *size = a.size;
+    });
+}
+
+int sph_bgeo_data(sph_bgeo_t f, const char *name, void *out, size_t bytes) {
+    return guarded([&] {
+        SPH_REQUIRE(f && name && out, SPH_ERR_INVALID, "null argument");
+        if (std::string(name) == "position") {
+            SPH_REQUIRE(bytes == f->f.pos.size() * 4, SPH_ERR_INVALID, "size mismatch");
+            std::memcpy(out, f->f.pos.data(), bytes);
+            return;
+        }
+        const sph::BgeoAttr *a = f->f.find(name);
+        SPH_REQUIRE(a, SPH_ERR_NOTFOUND, std::string("no attribute '") + name + "'");
+        SPH_REQUIRE(bytes == a->data.size() * 4, SPH_ERR_INVALID, "size mismatch");
+        std::memcpy(out, a->data.data(), bytes);
+    });
+}
+
+int sph_bgeo_write(const char *path, size_t n, const float *position, int nattr, const char *const *names,
+                   const int *types, const int *sizes, const void *const *data) {
+    return guarded([&] {
+        SPH_REQUIRE(path && (position || n == 0) && nattr >= 0, SPH_ERR_INVALID, "null argument");
+        sph::BgeoFile f;
+        f.n = n;
+        f.pos.assign(position, position + 3 * n);
+        for (int k = 0; k < nattr; k++) {
+            SPH_REQUIRE(names[k] && data[k], SPH_ERR_INVALID, "null attribute");
+            SPH_REQUIRE(types[k] == 0 || types[k] == 1 || types[k] == 5, SPH_ERR_INVALID, "attribute type 0, 1 or 5");
+            SPH_REQUIRE(sizes[k] >= 1 && sizes[k] <= 16, SPH_ERR_INVALID, "attribute size out of range");
+            sph::BgeoAttr a;
+            a.name = names[k];
+            a.type = types[k];
+            a.size = sizes[k];
+            const uint32_t *d = static_cast<const uint32_t *>(data[k]);
+            a.data.assign(d, d + (size_t)a.size * n);
+            f.attrs.push_back(std::move(a));
+        }
+        sph::gz_write(path, sph::bgeo_encode(f));
+    });
+}
+
+}  // extern "C"
