@@ -259,6 +259,17 @@ int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3],
                                     double radius, int invert,
                                     double thickness,
                                     const unsigned resolution[3]);
+/* Adds a Bender2019 boundary object for a static closed triangle mesh
+ * (vertices: 3 doubles each, world coordinates; faces: 3 vertex indices
+ * each). The distance field at the grid nodes is the distance to the
+ * nearest triangle, negative inside (generalized winding number above 1/2;
+ * positive inside when `invert`), minus `thickness`; the volume field
+ * integrates the interpolated distance field, as SimulatorBase::
+ * initVolumeMap does with Discregrid's (SimulatorBase.cpp:2566-2704). */
+int sph_dfsph_add_volume_map_mesh(sph_dfsph_t s, const double *vertices,
+                                  size_t n_vertices, const unsigned *faces,
+                                  size_t n_faces, int invert, double thickness,
+                                  const unsigned resolution[3]);
 /* Deferred initialisation (SimulatorBase::deferredInit,
  * StaticBoundarySimulator.cpp:181-199): Akinci boundary volumes
  * (Simulation::updateBoundaryVolume) and device buffers. Called by the first
@@ -406,9 +417,10 @@ typedef struct sph_scene *sph_scene_t;
  * "Weiler et al. 2018" section, the "XSPH" section), "FluidBlocks" sampled
  * as SimulatorBase::createFluidBlocks (Simulator/SimulatorBase.cpp:
  * 1418-1516) with their initial velocities (:1079-1093), and static
- * "RigidBodies" whose OBJ geometry is an axis-aligned box or a sphere
- * (Bender2019: one volume map each with mapInvert / mapThickness /
- * mapResolution; Akinci2012: their surfaces sampled at 2r). Other keys that
+ * "RigidBodies" from OBJ meshes (Bender2019: one volume map each with
+ * mapInvert / mapThickness / mapResolution, analytic for an axis-aligned box
+ * or a sphere, a mesh map otherwise; Akinci2012: box and sphere surfaces
+ * sampled at 2r). Other keys that
  * do not change the simulation are ignored; anything that would (dynamic
  * bodies, emitters, fluid models from files, other methods, several fluid
  * ids) fails with SPH_ERR_INVALID naming the key. Relative geometry paths
@@ -422,8 +434,8 @@ int sph_scene_counts(sph_scene_t s, size_t *n_fluid,
  * reference's sampling order) and Akinci boundary particles; any pointer
  * may be null. */
 int sph_scene_particles(sph_scene_t s, float *x, float *v, float *boundary_x);
-/* Body k: kind 0 box (lo, hi) or 1 sphere (centre, radius), and its map
- * settings. */
+/* Body k: kind 0 box (lo, hi), 1 sphere (centre, radius) or 2 mesh (lo, hi:
+ * its bounding box), and its map settings. */
 int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3],
                    double centre[3], double *radius, int *invert,
                    double *thickness, unsigned resolution[3]);

@@ -45,6 +45,8 @@ def lib():
             "orc_add_boundary_particles": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
             "orc_set_volume_map_box": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _up]),
             "orc_add_volume_map_sphere": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int, C.c_double, _up]),
+            "orc_add_volume_map_mesh": (C.c_int, [C.c_void_p, _dp, C.c_size_t, _up, C.c_size_t, C.c_int, C.c_double, _up]),
+            "orc_shape_sdf_mesh": (C.c_int, [_dp, C.c_size_t, _up, C.c_size_t, _dp, C.c_size_t, _dp]),
             "orc_init": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
@@ -257,6 +259,13 @@ class OracleDFSPH:
         _chk(lib().orc_add_volume_map_sphere(self.h, _ptr(_d(centre), _dp), float(radius), int(invert), thickness,
                                              _ptr(r, _up)))
 
+    def add_volume_map_mesh(self, vertices, faces, res, invert=False, thickness=0.0):
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        r = np.ascontiguousarray(res, dtype=np.uint32)
+        _chk(lib().orc_add_volume_map_mesh(self.h, _ptr(v, _dp), len(v), _ptr(f, _up), len(f), int(invert), thickness,
+                                           _ptr(r, _up)))
+
     def init(self):
         _chk(lib().orc_init(self.h))
 
@@ -302,3 +311,14 @@ class OracleDFSPH:
         _chk(lib().orc_get_neighbors(self.h, int(boundary), _ptr(off, _up), _ptr(idx, _up), idx.size,
                                      C.byref(nnz)))
         return off, idx[: nnz.value]
+
+
+def mesh_sdf(vertices, faces, x):
+    """Signed distance of a closed triangle mesh at points x (negative
+    inside)."""
+    v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+    p = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    out = np.zeros(len(p))
+    _chk(lib().orc_shape_sdf_mesh(_ptr(v, _dp), len(v), _ptr(f, _up), len(f), _ptr(p, _dp), len(p), _ptr(out, _dp)))
+    return out

@@ -233,9 +233,72 @@ void gauss_legendre(int n, double *x, double *w) {
 }
 
 // ---------------------------------------------------------- shape SDF
+// Squared distance from p to a triangle: closest point by the Voronoi
+// regions of its vertices, edges and face (Ericson, Real-Time Collision
+// Detection, sec. 5.1.5). The closest point is unique, so this is the
+// unsigned distance any mesh-distance routine returns.
+static double triangle_dist2(const double *p, const double *t) {
+    const double *a = t, *b = t + 3, *c = t + 6;
+    double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+    for (int k = 0; k < 3; k++) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ap[k] = p[k] - a[k];
+        bp[k] = p[k] - b[k];
+        cp[k] = p[k] - c[k];
+    }
+    const double d1 = dot3(ab, ap), d2 = dot3(ac, ap), d3 = dot3(ab, bp), d4 = dot3(ac, bp), d5 = dot3(ab, cp),
+                 d6 = dot3(ac, cp);
+    const double vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        for (int k = 0; k < 3; k++) q[k] = a[k];
+    } else if (d3 >= 0.0 && d4 <= d3) {
+        for (int k = 0; k < 3; k++) q[k] = b[k];
+    } else if (d6 >= 0.0 && d5 <= d6) {
+        for (int k = 0; k < 3; k++) q[k] = c[k];
+    } else if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double v = d1 / (d1 - d3);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k];
+    } else if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + w * ac[k];
+    } else if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int k = 0; k < 3; k++) q[k] = b[k] + w * (c[k] - b[k]);
+    } else {
+        const double den = 1.0 / (va + vb + vc), v = vb * den, w = vc * den;
+        for (int k = 0; k < 3; k++) q[k] = a[k] + ab[k] * v + ac[k] * w;
+    }
+    const double e[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+    return dot3(e, e);
+}
+
+// Solid angle of a triangle seen from p (Van Oosterom and Strackee 1983).
+static double triangle_solid_angle(const double *p, const double *t) {
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = t[k] - p[k];
+        b[k] = t[3 + k] - p[k];
+        c[k] = t[6 + k] - p[k];
+    }
+    const double la = std::sqrt(sqn3(a)), lb = std::sqrt(sqn3(b)), lc = std::sqrt(sqn3(c));
+    const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                       a[2] * (b[0] * c[1] - b[1] * c[0]);
+    return 2.0 * std::atan2(det, la * lb * lc + dot3(a, b) * lc + dot3(a, c) * lb + dot3(b, c) * la);
+}
+
 double ShapeSDF::eval(const double *p) const {
     double dmesh;
-    if (kind == 1) {
+    if (kind == 2) {
+        // a closed mesh: distance to the nearest triangle, inside where the
+        // generalized winding number (sum of solid angles / 4 pi) exceeds 1/2
+        double d2 = std::numeric_limits<double>::max(), omega = 0.0;
+        for (size_t k = 0; k + 9 <= tri.size(); k += 9) {
+            d2 = std::min(d2, triangle_dist2(p, &tri[k]));
+            omega += triangle_solid_angle(p, &tri[k]);
+        }
+        dmesh = std::fabs(omega) > 2.0 * M_PI ? -std::sqrt(d2) : std::sqrt(d2);
+    } else if (kind == 1) {
         const double d[3] = {p[0] - centre[0], p[1] - centre[1], p[2] - centre[2]};
         dmesh = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) - radius;
     } else {
@@ -329,6 +392,23 @@ void VolumeMap::init(const ShapeSDF &s, double h, const unsigned *r) {
         c0[c] = 2.0 / cs[c];
     }
     cache.clear();
+    mesh_sdf.clear();
+    if (s.kind == 2) {
+        const uint64_t nc = (uint64_t)res[0] * res[1] * res[2];
+        mesh_sdf.resize(32 * nc);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long long cell = 0; cell < (long long)nc; cell++) {
+            const unsigned i = (unsigned)(cell % res[0]), j = (unsigned)((cell / res[0]) % res[1]),
+                           k = (unsigned)(cell / ((uint64_t)res[0] * res[1]));
+            const double smin[3] = {lo[0] + cs[0] * i, lo[1] + cs[1] * j, lo[2] + cs[2] * k};
+            for (int n = 0; n < 32; n++) {
+                double p[3], y[3];
+                serendipity_node(n, p);
+                for (int c = 0; c < 3; c++) y[c] = smin[c] + 0.5 * (p[c] + 1.0) * cs[c];
+                mesh_sdf[32 * cell + n] = sdf.eval(y);
+            }
+        }
+    }
 }
 
 bool VolumeMap::locate(const double *x, uint64_t &cell, double *xi) const {
@@ -349,6 +429,20 @@ bool VolumeMap::locate(const double *x, uint64_t &cell, double *xi) const {
 }
 
 double VolumeMap::node_sdf(const double *x) const { return sdf.eval(x); }
+
+// Field 0 of a mesh map interpolated at y (inside the domain): the
+// reference's volume integrand reads the discretised distance field
+// (SimulatorBase.cpp:2627-2658).
+double VolumeMap::mesh_field(const double *y) const {
+    uint64_t cell;
+    double xi[3];
+    locate(y, cell, xi);
+    double N[32];
+    serendipity(xi, N, nullptr);
+    double phi = 0.0;
+    for (int n = 0; n < 32; n++) phi += mesh_sdf[32 * cell + n] * N[n];
+    return phi;
+}
 
 // volume_func of SimulatorBase.cpp:2627-2658 with the analytic box distance
 // in place of the interpolated field 0 inside the integrand.
@@ -371,7 +465,7 @@ double VolumeMap::node_volume(const double *x) const {
                     for (int c = 0; c < 3; c++)
                         if (!(y[c] >= lo[c] && y[c] <= hi[c])) indom = false;
                     if (indom) {
-                        const double dist = sdf.eval(y);
+                        const double dist = sdf.kind == 2 ? mesh_field(y) : sdf.eval(y);
                         if (dist <= 0.0)
                             f = 1.0;
                         else if (dist < h)
@@ -1497,6 +1591,39 @@ int orc_set_volume_map_box(void *h, const double *lo, const double *hi, int inve
     s.invert = invert != 0;
     s.thickness = thickness;
     ((Sim *)h)->add_volume_map(s, res);
+    return SPH_OK;
+}
+int orc_add_volume_map_mesh(void *h, const double *vertices, size_t nv, const unsigned *faces, size_t nf,
+                            int invert, double thickness, const unsigned *res) {
+    orc::ShapeSDF s;
+    s.kind = 2;
+    for (int c = 0; c < 3; c++) {
+        s.lo[c] = std::numeric_limits<double>::max();
+        s.hi[c] = -std::numeric_limits<double>::max();
+    }
+    for (size_t v = 0; v < nv; v++)
+        for (int c = 0; c < 3; c++) {
+            s.lo[c] = std::min(s.lo[c], vertices[3 * v + c]);
+            s.hi[c] = std::max(s.hi[c], vertices[3 * v + c]);
+        }
+    for (size_t f = 0; f < 3 * nf; f++) {
+        if (faces[f] >= nv) return SPH_ERR_INVALID;
+        for (int c = 0; c < 3; c++) s.tri.push_back(vertices[3 * (size_t)faces[f] + c]);
+    }
+    s.invert = invert != 0;
+    s.thickness = thickness;
+    ((Sim *)h)->add_volume_map(s, res);
+    return SPH_OK;
+}
+int orc_shape_sdf_mesh(const double *vertices, size_t nv, const unsigned *faces, size_t nf, const double *x, size_t n,
+                       double *out) {
+    orc::ShapeSDF s;
+    s.kind = 2;
+    s.invert = false;
+    for (size_t f = 0; f < 3 * nf; f++)
+        for (int c = 0; c < 3; c++) s.tri.push_back(vertices[3 * (size_t)faces[f] + c]);
+    (void)nv;
+    for (size_t i = 0; i < n; i++) out[i] = s.eval(x + 3 * i);
     return SPH_OK;
 }
 int orc_add_volume_map_sphere(void *h, const double *centre, double radius, int invert, double thickness,

@@ -66,11 +66,12 @@ void gauss_legendre(int n, double *x, double *w);
 // when inverted, d_shape < 0 inside the shape. Kinds: 0 axis-aligned box
 // [lo, hi] (a wall when inverted), 1 sphere (centre, radius).
 struct ShapeSDF {
-    int kind = 0;
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int kind = 0;  // 0 box, 1 sphere, 2 closed triangle mesh
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // box; a mesh's bounding box
     double centre[3] = {0, 0, 0}, radius = 0;
     bool invert = true;
     double thickness = 0;
+    std::vector<double> tri;  // mesh: 9 coordinates per triangle
     double eval(const double *x) const;
     void aabb(double *blo, double *bhi) const;
 };
@@ -91,9 +92,11 @@ struct VolumeMap {
     Cubic cubic;
     double gx[16], gw[16];
     std::unordered_map<uint64_t, std::vector<double>> cache;  // 64 values per cell
+    std::vector<double> mesh_sdf;  // mesh: field 0 of every cell (32 per cell), built at init
     void init(const ShapeSDF &s, double supportRadius, const unsigned *res);
     bool locate(const double *x, uint64_t &cell, double *xi) const;
     double node_sdf(const double *x) const;
+    double mesh_field(const double *y) const;
     double node_volume(const double *x) const;
     const std::vector<double> &cell_values(uint64_t cell);
     void prefetch(const std::vector<uint64_t> &cells);

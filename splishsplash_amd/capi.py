@@ -67,6 +67,7 @@ _SIGS = {
     "sph_dfsph_destroy": (C.c_int, [_vp]),
     "sph_dfsph_set_fluid": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_reset": (C.c_int, [_vp]),
+    "sph_dfsph_add_volume_map_mesh": (C.c_int, [_vp, _dp, C.c_size_t, _up, C.c_size_t, C.c_int, C.c_double, _up]),
     "sph_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "sph_dfsph_write_particles": (C.c_int, [_vp, C.c_char_p, C.c_char_p]),
     "sph_dfsph_save_state": (C.c_int, [_vp, C.c_char_p]),
@@ -374,7 +375,7 @@ class Scene:
         _chk(lib().sph_scene_body(self.h, k, C.byref(kind), lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp),
                                   cen.ctypes.data_as(_dp), C.byref(rad), C.byref(inv), C.byref(th),
                                   res.ctypes.data_as(_up)))
-        return {"kind": "box" if kind.value == 0 else "sphere", "lo": lo, "hi": hi, "centre": cen,
+        return {"kind": ("box", "sphere", "mesh")[kind.value], "lo": lo, "hi": hi, "centre": cen,
                 "radius": rad.value, "invert": bool(inv.value), "thickness": th.value, "resolution": res}
 
     def create_dfsph(self):
@@ -481,6 +482,15 @@ class TimeStepDFSPH:
         r = np.ascontiguousarray(res, np.uint32)
         _chk(lib().sph_dfsph_add_volume_map_sphere(self.h, c.ctypes.data_as(_dp), float(radius), int(invert),
                                                    thickness, r.ctypes.data_as(_up)))
+
+    def add_volume_map_mesh(self, vertices, faces, res, invert=False, thickness=0.0):
+        """Static closed triangle mesh (world coordinates) as a Bender2019
+        volume map with res[0..2] cells over its box grown by 8h."""
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        r = np.ascontiguousarray(res, np.uint32)
+        _chk(lib().sph_dfsph_add_volume_map_mesh(self.h, v.ctypes.data_as(_dp), len(v), f.ctypes.data_as(_up), len(f),
+                                                 int(invert), thickness, r.ctypes.data_as(_up)))
 
     def init(self):
         _chk(lib().sph_dfsph_init(self.h))

@@ -1555,6 +1555,30 @@ void DFSPH::set_volume_map_box(const double lo[3], const double hi[3], int inver
     add_volume_map(sh, res);
 }
 
+void DFSPH::add_volume_map_mesh(const double *vertices, size_t nv, const unsigned *faces, size_t nf, int invert,
+                                double thickness, const unsigned res[3]) {
+    SPH_REQUIRE(vertices && faces && nv >= 4 && nf >= 4, SPH_ERR_INVALID, "volume map: a closed mesh needs faces");
+    Shape sh;
+    sh.kind = SHAPE_MESH;
+    for (int c = 0; c < 3; c++) {
+        sh.box_lo[c] = 1e300;
+        sh.box_hi[c] = -1e300;
+    }
+    for (size_t v = 0; v < nv; v++)
+        for (int c = 0; c < 3; c++) {
+            sh.box_lo[c] = std::min(sh.box_lo[c], vertices[3 * v + c]);
+            sh.box_hi[c] = std::max(sh.box_hi[c], vertices[3 * v + c]);
+        }
+    sh.tri.reserve(9 * nf);
+    for (size_t f = 0; f < 3 * nf; f++) {
+        SPH_REQUIRE(faces[f] < nv, SPH_ERR_INVALID, "volume map: face index out of range");
+        for (int c = 0; c < 3; c++) sh.tri.push_back(vertices[3 * (size_t)faces[f] + c]);
+    }
+    sh.invert = invert;
+    sh.thickness = thickness;
+    add_volume_map(sh, res);
+}
+
 void DFSPH::add_volume_map(const Shape &shape, const unsigned res[3]) {
     SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
     SPH_REQUIRE(vm_shapes_.size() < 64, SPH_ERR_INVALID, "volume map: at most 64 objects");

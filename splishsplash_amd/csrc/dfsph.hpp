@@ -89,6 +89,8 @@ public:
     void set_volume_map_box(const double lo[3], const double hi[3], int invert, double thickness,
                             const unsigned res[3]);
     void add_volume_map(const Shape &shape, const unsigned res[3]);
+    void add_volume_map_mesh(const double *vertices, size_t nv, const unsigned *faces, size_t nf, int invert,
+                             double thickness, const unsigned res[3]);
     void init();
     void init_if_needed() {
         if (!initialized_) init();

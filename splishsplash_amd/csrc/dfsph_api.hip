@@ -103,6 +103,15 @@ int sph_dfsph_add_volume_map_sphere(sph_dfsph_t s, const double centre[3], doubl
     });
 }
 
+int sph_dfsph_add_volume_map_mesh(sph_dfsph_t s, const double *vertices, size_t n_vertices, const unsigned *faces,
+                                  size_t n_faces, int invert, double thickness, const unsigned resolution[3]) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(resolution, SPH_ERR_INVALID, "null resolution");
+        s->s->add_volume_map_mesh(vertices, n_vertices, faces, n_faces, invert, thickness, resolution);
+    });
+}
+
 int sph_dfsph_init(sph_dfsph_t s) {
     return guarded([&] {
         DCHK(s);

@@ -204,8 +204,10 @@ std::string dir_of(const std::string &path) {
 
 // --------------------------------------------------------------- the scene
 struct SceneBody {
-    int kind = 0;  // 0 box, 1 sphere
+    int kind = 0;  // 0 box, 1 sphere, 2 closed triangle mesh
     double lo[3], hi[3], centre[3], radius = 0.0;
+    std::vector<double> verts;  // mesh (world coordinates)
+    std::vector<unsigned> faces;
     bool invert = false;
     double thickness = 0.0;
     unsigned res[3] = {20, 20, 20};
@@ -286,17 +288,31 @@ static void sample_block(double r, int mode, const double bmin[3], const double 
     }
 }
 
-// Vertices of an OBJ mesh ("v x y z" lines).
-static std::vector<double> obj_vertices(const std::string &path) {
+// Vertices ("v x y z") and faces ("f i j k ...", polygons fanned into
+// triangles, 1-based or negative relative indices) of an OBJ mesh.
+static std::vector<double> obj_vertices(const std::string &path, std::vector<unsigned> *faces = nullptr) {
     std::ifstream f(path);
     if (!f) throw Error(SPH_ERR_INVALID, "scene: cannot open geometry file " + path);
     std::vector<double> out;
     std::string line;
     while (std::getline(f, line)) {
-        if (line.size() < 2 || line[0] != 'v' || (line[1] != ' ' && line[1] != '\t')) continue;
+        if (line.size() < 2 || (line[1] != ' ' && line[1] != '\t')) continue;
         std::istringstream s(line.substr(2));
-        double a, b, c;
-        if (s >> a >> b >> c) out.insert(out.end(), {a, b, c});
+        if (line[0] == 'v') {
+            double a, b, c;
+            if (s >> a >> b >> c) out.insert(out.end(), {a, b, c});
+        } else if (line[0] == 'f' && faces) {
+            std::vector<long> idx;
+            std::string tok;
+            while (s >> tok) {
+                long k = std::strtol(tok.c_str(), nullptr, 10);  // "i", "i/t", "i/t/n", "i//n"
+                k = k > 0 ? k - 1 : (long)(out.size() / 3) + k;
+                if (k < 0 || k >= (long)(out.size() / 3)) throw Error(SPH_ERR_INVALID, "scene: bad face in " + path);
+                idx.push_back(k);
+            }
+            for (size_t t = 1; t + 1 < idx.size(); t++)
+                faces->insert(faces->end(), {(unsigned)idx[0], (unsigned)idx[t], (unsigned)idx[t + 1]});
+        }
     }
     if (out.empty()) throw Error(SPH_ERR_INVALID, "scene: no vertices in " + path);
     return out;
@@ -307,8 +323,8 @@ static std::vector<double> obj_vertices(const std::string &path) {
 // recognised as an axis-aligned box (every vertex on a corner of the
 // bounding box) or a sphere (every vertex at the same distance from the
 // centre), the shapes whose distance fields the volume maps build exactly.
-static SceneBody body_from_mesh(const std::vector<double> &V, const double sc[3], const double tr[3],
-                                const double axis[3], double angle, const std::string &name) {
+static SceneBody body_from_mesh(const std::vector<double> &V, const std::vector<unsigned> &faces, const double sc[3],
+                                const double tr[3], const double axis[3], double angle, const std::string &name) {
     // rotation matrix (Rodrigues)
     double n[3] = {axis[0], axis[1], axis[2]};
     const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -376,7 +392,13 @@ static SceneBody body_from_mesh(const std::vector<double> &V, const double sc[3]
         b.radius = dsum / (double)m;
         return b;
     }
-    unsupported("geometry \"" + name + "\" (only axis-aligned boxes and spheres have exact distance fields here)");
+    // any other closed mesh: its triangles in world coordinates
+    if (faces.size() < 12) unsupported("geometry \"" + name + "\" (not a closed triangle mesh)");
+    b.kind = 2;
+    b.verts = W;
+    b.faces = faces;
+    for (int c = 0; c < 3; c++) b.centre[c] = 0.5 * (b.lo[c] + b.hi[c]);
+    return b;
 }
 
 // Akinci boundary particles on the surface of a body: a regular grid of
@@ -527,7 +549,11 @@ static Scene *read_scene(const std::string &path) {
             vec3(b, "scale", sc);
             vec3(b, "rotationAxis", ax);
             const double angle = num(b, "rotationAngle", 0.0);
-            SceneBody body = body_from_mesh(obj_vertices(mesh), sc, tr, ax, angle, mesh);
+            std::vector<unsigned> faces;
+            const std::vector<double> verts = obj_vertices(mesh, &faces);
+            SceneBody body = body_from_mesh(verts, faces, sc, tr, ax, angle, mesh);
+            if (body.kind == 2 && P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012)
+                unsupported("Akinci2012 sampling of the general mesh \"" + mesh + "\" (boxes and spheres only)");
             body.invert = flag(b, "mapInvert", false);
             body.thickness = num(b, "mapThickness", 0.0);
             double res[3] = {20, 20, 20};
@@ -631,8 +657,11 @@ int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out) {
             for (const sph::SceneBody &b : S.bodies) {
                 if (b.kind == 0)
                     chk(sph_dfsph_set_volume_map_box(h, b.lo, b.hi, b.invert ? 1 : 0, b.thickness, b.res));
-                else
+                else if (b.kind == 1)
                     chk(sph_dfsph_add_volume_map_sphere(h, b.centre, b.radius, b.invert ? 1 : 0, b.thickness, b.res));
+                else
+                    chk(sph_dfsph_add_volume_map_mesh(h, b.verts.data(), b.verts.size() / 3, b.faces.data(),
+                                                      b.faces.size() / 3, b.invert ? 1 : 0, b.thickness, b.res));
             }
         }
         *out = h;

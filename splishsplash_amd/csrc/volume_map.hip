@@ -69,6 +69,83 @@ __device__ __forceinline__ double shape_sdf(const double *p, const VMapDesc &d) 
     return (d.invert ? -1.0 : 1.0) * (dm - d.thickness);
 }
 
+// Squared distance from p to triangle (a, b, c): the closest point by the
+// Voronoi regions of the vertices, edges and face (Ericson, Real-Time
+// Collision Detection, sec. 5.1.5).
+__device__ __forceinline__ double tri_dist2(const double *p, const double *t) {
+    const double a[3] = {t[0], t[1], t[2]}, b[3] = {t[3], t[4], t[5]}, c[3] = {t[6], t[7], t[8]};
+    double ab[3], ac[3], ap[3], q[3];
+    for (int k = 0; k < 3; k++) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    auto dot = [](const double *u, const double *v) { return (u[0] * v[0] + u[1] * v[1]) + u[2] * v[2]; };
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        for (int k = 0; k < 3; k++) q[k] = a[k];
+    } else {
+        double bp[3], cp[3];
+        for (int k = 0; k < 3; k++) {
+            bp[k] = p[k] - b[k];
+            cp[k] = p[k] - c[k];
+        }
+        const double d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cp), d6 = dot(ac, cp);
+        const double vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+        if (d3 >= 0.0 && d4 <= d3) {
+            for (int k = 0; k < 3; k++) q[k] = b[k];
+        } else if (d6 >= 0.0 && d5 <= d6) {
+            for (int k = 0; k < 3; k++) q[k] = c[k];
+        } else if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+            const double v = d1 / (d1 - d3);
+            for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k];
+        } else if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+            const double w = d2 / (d2 - d6);
+            for (int k = 0; k < 3; k++) q[k] = a[k] + w * ac[k];
+        } else if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+            const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+            for (int k = 0; k < 3; k++) q[k] = b[k] + w * (c[k] - b[k]);
+        } else {
+            const double den = 1.0 / (va + vb + vc), v = vb * den, w = vc * den;
+            for (int k = 0; k < 3; k++) q[k] = a[k] + ab[k] * v + ac[k] * w;
+        }
+    }
+    const double e[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+    return dot(e, e);
+}
+
+// Solid angle of triangle t seen from p (Van Oosterom and Strackee 1983),
+// signed by the triangle's orientation.
+__device__ __forceinline__ double tri_solid_angle(const double *p, const double *t) {
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = t[k] - p[k];
+        b[k] = t[3 + k] - p[k];
+        c[k] = t[6 + k] - p[k];
+    }
+    const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]),
+                 lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                       a[2] * (b[0] * c[1] - b[1] * c[0]);
+    const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2], ac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2],
+                 bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    return 2.0 * atan2(det, la * lb * lc + ab * lc + ac * lb + bc * la);
+}
+
+// Signed distance of a closed mesh: distance to the nearest triangle,
+// negative where the generalized winding number exceeds 1/2 (inside), then
+// the inversion and thickness of shape_sdf.
+__device__ __forceinline__ double mesh_sdf(const double *p, const VMapDesc &d) {
+    double d2 = DBL_MAX, omega = 0.0;
+    for (unsigned k = 0; k < d.ntri; k++) {
+        const double *t = d.tri + 9ull * k;
+        d2 = fmin(d2, tri_dist2(p, t));
+        omega += tri_solid_angle(p, t);
+    }
+    const double dm = (fabs(omega) > 2.0 * 3.14159265358979323846) ? -sqrt(d2) : sqrt(d2);
+    return (d.invert ? -1.0 : 1.0) * (dm - d.thickness);
+}
+
 struct NodeCounts {
     unsigned long long nc, nx, ny, nz;  // corners, x-, y-, z-edge nodes
 };
@@ -120,14 +197,28 @@ struct QuadArgs {
 // (GaussQuadrature.cpp:5930-5962) over [-h, h]^3 with the integrand of
 // SimulatorBase.cpp:2637-2650 (the analytic box distance replaces the
 // interpolated field 0 inside the integrand).
+__device__ double interp_sdf(const VMapDesc &d, const double *y);
+
+// Node values of the distance field of a mesh (its volume field follows in
+// k_vm_nodes once the distance field's cell records exist).
+__global__ __launch_bounds__(BLOCK) void k_vm_mesh_nodes(VMapDesc d, NodeCounts nc, unsigned long long total,
+                                                         float *sdf_node) {
+    const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    double x[3];
+    node_pos(g, d, nc, x);
+    sdf_node[g] = (float)mesh_sdf(x, d);
+}
+
+template <bool MESH>
 __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, unsigned long long total, QuadArgs q,
                                                     float *sdf_node, float *vol_node) {
     const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     double x[3];
     node_pos(g, d, nc, x);
-    const double dist_x = shape_sdf(x, d);
-    sdf_node[g] = (float)dist_x;
+    const double dist_x = MESH ? (double)sdf_node[g] : shape_sdf(x, d);
+    if (!MESH) sdf_node[g] = (float)dist_x;
     const double h = q.h;
     double res = 0.0;
     if (!(dist_x > 2.0 * h)) {
@@ -147,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, u
                         for (int c = 0; c < 3; c++)
                             if (!(y[c] >= d.lo[c] && y[c] <= dom_hi[c])) indom = false;
                         if (indom) {
-                            const double dist = shape_sdf(y, d);
+                            const double dist = MESH ? interp_sdf(d, y) : shape_sdf(y, d);
                             if (dist <= 0.0)
                                 f = 1.0;
                             else if (dist < h)
@@ -206,7 +297,10 @@ __global__ __launch_bounds__(BLOCK) void k_vm_cells(VMapDesc d, NodeCounts nc, u
 
 void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], double h) {
     SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
-    SPH_REQUIRE(shape.kind == SHAPE_BOX || shape.kind == SHAPE_SPHERE, SPH_ERR_INVALID, "volume map: unknown shape");
+    SPH_REQUIRE(shape.kind == SHAPE_BOX || shape.kind == SHAPE_SPHERE || shape.kind == SHAPE_MESH, SPH_ERR_INVALID,
+                "volume map: unknown shape");
+    SPH_REQUIRE(shape.kind != SHAPE_MESH || (!shape.tri.empty() && shape.tri.size() % 9 == 0), SPH_ERR_INVALID,
+                "volume map: a mesh needs triangles");
     const double pad = 8.0 * h + shape.thickness;
     d.kind = shape.kind;
     d.radius = shape.radius;
@@ -242,9 +336,25 @@ void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], 
     const double pi = 3.14159265358979323846;
     q.k = 8.0 / (pi * h * h * h);
     q.W0 = q.k;
-    k_vm_nodes<<<(unsigned)((total + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
-    SPH_HIP(hipGetLastError());
     const unsigned long long nt = (unsigned long long)ncells * 32ull;
+    const unsigned gn = (unsigned)((total + BLOCK - 1) / BLOCK);
+    if (shape.kind == SHAPE_MESH) {
+        // distance field first: the volume integrand interpolates it
+        d.ntri = (unsigned)(shape.tri.size() / 9);
+        tri.reserve(shape.tri.size());
+        SPH_HIP(hipMemcpy(tri.get(), shape.tri.data(), shape.tri.size() * sizeof(double), hipMemcpyHostToDevice));
+        d.tri = tri.get();
+        k_vm_mesh_nodes<<<gn, BLOCK, 0, s>>>(d, nc, total, sn.get());
+        SPH_HIP(hipGetLastError());
+        k_vm_cells<<<(unsigned)((nt + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, ncells, sn.get(), sn.get(),
+                                                                           sdf.get(), vol.get());
+        SPH_HIP(hipGetLastError());
+        d.sdf = sdf.get();
+        k_vm_nodes<true><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+    } else {
+        k_vm_nodes<false><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+    }
+    SPH_HIP(hipGetLastError());
     k_vm_cells<<<(unsigned)((nt + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, ncells, sn.get(), vn.get(), sdf.get(),
                                                                        vol.get());
     SPH_HIP(hipGetLastError());
@@ -294,6 +404,30 @@ __device__ __forceinline__ void shape32(const double *xi, double *N, double (*dN
         dN[n][o1] = (9.0 / 64.0) * fa * fb * p[o1] * f2;
         dN[n][o2] = (9.0 / 64.0) * fa * fb * f1 * p[o2];
     }
+}
+
+// Distance field interpolated at y (the field's value at a point of the
+// domain; DBL_MAX outside, which the integrand treats as far from the
+// surface).
+__device__ double interp_sdf(const VMapDesc &d, const double *y) {
+    unsigned mi[3];
+    double loc[3];
+    for (int c = 0; c < 3; c++) {
+        const double hi = d.lo[c] + d.cs[c] * d.res[c];
+        if (!(y[c] >= d.lo[c] && y[c] <= hi)) return DBL_MAX;
+        mi[c] = (unsigned)((y[c] - d.lo[c]) * d.inv[c]);
+        if (mi[c] >= d.res[c]) mi[c] = d.res[c] - 1;
+        const double smin = d.lo[c] + d.cs[c] * mi[c], smax = smin + d.cs[c];
+        const double denom = smax - smin;
+        loc[c] = (2.0 / denom) * y[c] - (smax + smin) / denom;
+    }
+    const unsigned long long cell = ((unsigned long long)mi[2] * d.res[1] + mi[1]) * d.res[0] + mi[0];
+    double N[32], dN[32][3];
+    shape32(loc, N, dN);
+    const float *rec = d.sdf + 32ull * cell;
+    double phi = 0.0;
+    for (int m = 0; m < 32; m++) phi += (double)rec[m] * N[m];
+    return phi;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,

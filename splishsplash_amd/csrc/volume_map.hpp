@@ -16,20 +16,28 @@
 // as 32 contiguous floats per field (128 B, two cache lines), so sampling a
 // particle reads one cell record instead of 32 scattered nodes.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace sph {
 
-enum { SHAPE_BOX = 0, SHAPE_SPHERE = 1 };
+enum { SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_MESH = 2 };
 
 // Analytic boundary shape: sign * (d_shape(x) - thickness), d_shape < 0
 // inside the shape, sign = -1 when inverted.
+//
+// A closed triangle mesh (SHAPE_MESH) has no closed-form distance: its
+// distance field is computed at the grid nodes (nearest triangle, sign from
+// the generalized winding number) and, as the reference does with its
+// Discregrid field, the volume integrand reads the interpolated field.
 struct Shape {
     int kind = SHAPE_BOX;
-    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // box
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // box; a mesh's bounding box
     double centre[3] = {0, 0, 0}, radius = 0;              // sphere
     int invert = 1;
     double thickness = 0;
+    std::vector<double> tri;  // mesh: 9 coordinates per triangle
 };
 
 struct VMapDesc {
@@ -42,6 +50,8 @@ struct VMapDesc {
     double centre[3], radius;
     int invert;
     double thickness;
+    const double *tri;  // mesh triangles (device), 9 doubles each
+    unsigned ntri;
     const float *sdf;  // ncells x 32
     const float *vol;  // ncells x 32
 };
@@ -50,6 +60,7 @@ class VolumeMap {
 public:
     VMapDesc d{};
     DBuf<float> sdf, vol;
+    DBuf<double> tri;
     size_t ncells = 0;
     bool built = false;
     // Builds both fields on the device (synchronous).

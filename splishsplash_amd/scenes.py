@@ -3,6 +3,8 @@
 All coordinates are rounded to float32 and returned as float64 arrays, so
 the FP32 device path and an FP64 checker start from identical states.
 """
+import os
+
 import numpy as np
 
 from .params import BOUNDARY_AKINCI2012, BOUNDARY_BENDER2019, VISCOSITY_WEILER2018, default_params
@@ -192,3 +194,24 @@ def ensemble_scene(seed, nmin=20_000, nmax=500_000, spacing=0.05, vm_cell=0.05, 
     sc["sphere_res"] = vm_cell
     sc["variant"] = {"seed": seed, "dims": dims, "box": hi, "corner": corner, "spheres": spheres}
     return sc
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def read_obj(path):
+    """Vertices (n, 3) float64 and triangles (m, 3) uint32 (0-based) of an
+    OBJ mesh; polygons are fanned into triangles."""
+    v, f = [], []
+    for line in open(path):
+        p = line.split()
+        if not p:
+            continue
+        if p[0] == "v":
+            v.append([float(t) for t in p[1:4]])
+        elif p[0] == "f":
+            idx = [int(t.split("/")[0]) for t in p[1:]]
+            idx = [i - 1 if i > 0 else len(v) + i for i in idx]
+            for k in range(1, len(idx) - 1):
+                f.append([idx[0], idx[k], idx[k + 1]])
+    return np.array(v, np.float64), np.array(f, np.uint32)

@@ -152,11 +152,12 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
     ({"Materials": [{"id": "Fluid", "surfaceTensionMethod": 1}]}, "surfaceTensionMethod"),
     ({"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]},
                       {"id": "Other", "start": [1, 0, 0], "end": [1.5, 0.5, 0.5]}]}, "fluid id"),
-    ({"RigidBodies": [{"geometryFile": "unit_box.obj", "rotationAngle": 0.3}]}, "axis-aligned"),
-    ({"RigidBodies": [{"geometryFile": "tetra.obj"}]}, "geometry"),
+    ({"RigidBodies": [{"geometryFile": "triangle.obj"}]}, "closed"),
+    ({"Configuration": {"boundaryHandlingMethod": 0}, "RigidBodies": [{"geometryFile": "tetra.obj"}]}, "Akinci2012"),
 ])
 def test_unsupported_content_is_rejected(tmp_path, change, key):
-    (tmp_path / "tetra.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\nf 1 2 4\nf 1 3 4\nf 2 3 4\n")
+    (tmp_path / "tetra.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 3 2\nf 1 2 4\nf 1 4 3\nf 2 3 4\n")
+    (tmp_path / "triangle.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
     scene = {"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]}
     for k, v in change.items():
         scene.setdefault(k, {}) if isinstance(v, dict) else None
@@ -167,6 +168,24 @@ def test_unsupported_content_is_rejected(tmp_path, change, key):
     with pytest.raises(capi.SPHError) as e:
         capi.Scene(write_scene(tmp_path, scene))
     assert key in str(e.value)
+
+
+def test_general_meshes_become_mesh_maps(tmp_path):
+    # a rotated box and a tetrahedron are closed meshes without a closed-form
+    # distance: they become mesh volume maps (Bender2019) with their
+    # triangles in world coordinates
+    (tmp_path / "tetra.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 3 2\nf 1 2 4\nf 1 4 3\nf 2 3 4\n")
+    scene = {"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}],
+             "RigidBodies": [{"geometryFile": "unit_box.obj", "rotationAxis": [0, 0, 1], "rotationAngle": 0.5,
+                              "translation": [1, 1, 1], "mapResolution": [12, 12, 12]},
+                             {"geometryFile": "tetra.obj", "scale": [0.3, 0.3, 0.3], "translation": [2, 0, 0]}]}
+    sc = capi.Scene(write_scene(tmp_path, scene))
+    b0, b1 = sc.body(0), sc.body(1)
+    assert b0["kind"] == "mesh" and b1["kind"] == "mesh"
+    half = 0.5 * (np.cos(0.5) + np.sin(0.5))  # half-width of the rotated unit square
+    np.testing.assert_allclose(b0["lo"], [1 - half, 1 - half, 0.5], atol=1e-12)
+    np.testing.assert_allclose(b1["hi"], [2.3, 0.3, 0.3], atol=1e-12)
+    assert list(b0["resolution"]) == [12, 12, 12]
 
 
 def test_malformed_files_are_errors(tmp_path):
