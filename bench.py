@@ -211,17 +211,19 @@ def ns_microbench(kind, h, n, reps, check):
            "ms_per_rep": 1e3 * wall, "kernels": ker}
     if check:
         from oracle import oracle
-        x32 = np.ascontiguousarray(x, np.float32)
+        # the time step decides pairs within FP32 rounding of h^2 in FP64:
+        # compared with the oracle's FP64 search on the same FP32 positions
+        x32 = np.ascontiguousarray(x, np.float32).astype(np.float64)
         if K < 100:
             off_g, idx_g = sim.neighbors(boundary=False)
-            off_o, idx_o = oracle.neighbors(x32, radius=np.float32(h))
+            off_o, idx_o = oracle.neighbors(x32, radius=4.0 * prm.particle_radius, dtype=np.float64)
             same = np.array_equal(off_g, off_o) and all(
                 np.array_equal(np.sort(idx_g[off_g[i]:off_g[i + 1]]), np.sort(idx_o[off_o[i]:off_o[i + 1]]))
                 for i in np.random.default_rng(0).integers(0, n, 20000))
             res["check"] = {"sets": "all counts, 20000 sampled sets", "equal": bool(same)}
         else:
             q = np.random.default_rng(0).integers(0, n, 2000)
-            off_o, _ = oracle.neighbors(x32[q], x32, radius=np.float32(h), brute=True)
+            off_o, _ = oracle.neighbors(x32[q], x32, radius=4.0 * prm.particle_radius, brute=True, dtype=np.float64)
             res["check"] = {"sets": "counts of 2000 sampled particles (brute force)",
                             "equal": bool(np.array_equal(np.diff(off_o) - 1, cnt[q]))}
     del sim

@@ -291,6 +291,39 @@ int sph_dfsph_reset(sph_dfsph_t s);
  * particles keep their state. */
 int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v,
                              size_t n);
+/* Emitter system (SPlisHSPlasH/EmitterSystem.h, Emitter.h). An emitter
+ * places a width x height grid of particles (type 0, box) or a disc of width
+ * particles across (type 1, circle) at translation + (t - next + h) v along
+ * column 0 of the row-major rotation, every 2r / velocity seconds between
+ * emit_start_time and emit_end_time, and animates the particles inside its
+ * region (Emitter.cpp:89-380). Emission happens at the end of each step,
+ * before the time advances (TimeStepDFSPH.cpp:242-248). New particles get
+ * the next ids and state 1 (animated) for one step. */
+typedef struct sph_emitter {
+    int      type;               /* 0 box, 1 circle (Emitter m_type)         */
+    unsigned width, height;      /* particles across; height: boxes only     */
+    double   translation[3];     /* m_x                                      */
+    double   rotation[9];        /* m_rotation, row-major                    */
+    double   velocity;           /* m_velocity (> 0)                         */
+    double   emit_start_time;    /* m_emitStartTime                          */
+    double   emit_end_time;      /* m_emitEndTime (DBL_MAX: none)            */
+} sph_emitter_t;
+/* EmitterSystem::addEmitter + Emitter::setEmitStartTime/EndTime
+ * (EmitterSystem.cpp:93-104, SimulatorBase.cpp:1319-1371). */
+int sph_dfsph_add_emitter(sph_dfsph_t s, const sph_emitter_t *e);
+/* EmitterSystem::enableReuseParticles / disableReuseParticles
+ * (EmitterSystem.cpp:106-117): particles leaving [box_min, box_max] are
+ * slowed (x 0.95) and placed again by the next emitter layers. */
+int sph_dfsph_set_emitter_reuse(sph_dfsph_t s, int enable,
+                                const double box_min[3],
+                                const double box_max[3]);
+/* nMaxEmitterParticles of Simulation::addFluidModel (Simulation.cpp:686,
+ * SimulatorBase.cpp:1267-1273, default 10000): emitters stop adding new
+ * particles once this many were added. */
+int sph_dfsph_set_max_emitter_particles(sph_dfsph_t s, size_t max_particles);
+/* m_numberOfEmittedParticles and m_numReusedParticles (EmitterSystem.h). */
+int sph_dfsph_emitter_counts(sph_dfsph_t s, size_t *emitted, size_t *reused);
+
 /* Steps n independent scenes together (no reference counterpart; the
  * ensemble that generates training data): every handle advances n_steps
  * steps, the handles are shared out over `threads` host threads (0: up to

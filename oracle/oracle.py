@@ -47,6 +47,11 @@ def lib():
             "orc_add_volume_map_sphere": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int, C.c_double, _up]),
             "orc_add_volume_map_mesh": (C.c_int, [C.c_void_p, _dp, C.c_size_t, _up, C.c_size_t, C.c_int, C.c_double, _up]),
             "orc_shape_sdf_mesh": (C.c_int, [_dp, C.c_size_t, _up, C.c_size_t, _dp, C.c_size_t, _dp]),
+            "orc_add_emitter": (C.c_int, [C.c_void_p, C.c_int, C.c_uint, C.c_uint, _dp, _dp, C.c_double,
+                                          C.c_double, C.c_double]),
+            "orc_set_emitter_reuse": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+            "orc_set_max_emitter_particles": (C.c_int, [C.c_void_p, C.c_size_t]),
+            "orc_emitter_counts": (C.c_int, [C.c_void_p, _szp, _szp]),
             "orc_init": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
@@ -266,14 +271,43 @@ class OracleDFSPH:
         _chk(lib().orc_add_volume_map_mesh(self.h, _ptr(v, _dp), len(v), _ptr(f, _up), len(f), int(invert), thickness,
                                            _ptr(r, _up)))
 
+    def add_emitter(self, width, height, pos, rotation, velocity, type=0, emit_start_time=0.0,
+                    emit_end_time=float(np.finfo(np.float64).max)):
+        """EmitterSystem::addEmitter plus the start and end times; rotation
+        is a 3x3 matrix whose column 0 is the emit direction."""
+        R = _d(rotation).reshape(3, 3)
+        _chk(lib().orc_add_emitter(self.h, int(type), int(width), int(height), _ptr(_d(pos), _dp), _ptr(R, _dp),
+                                   float(velocity), float(emit_start_time), float(emit_end_time)))
+
+    def enable_reuse_particles(self, box_min=(-1.0, -1.0, -1.0), box_max=(1.0, 1.0, 1.0)):
+        _chk(lib().orc_set_emitter_reuse(self.h, 1, _ptr(_d(box_min), _dp), _ptr(_d(box_max), _dp)))
+
+    def disable_reuse_particles(self):
+        _chk(lib().orc_set_emitter_reuse(self.h, 0, None, None))
+
+    def set_max_emitter_particles(self, m):
+        _chk(lib().orc_set_max_emitter_particles(self.h, int(m)))
+
+    def emitter_counts(self):
+        e, r = C.c_size_t(0), C.c_size_t(0)
+        _chk(lib().orc_emitter_counts(self.h, C.byref(e), C.byref(r)))
+        return e.value, r.value
+
+    def _refresh_n(self):
+        n = C.c_size_t(0)
+        _chk(lib().orc_num_particles(self.h, C.byref(n)))
+        self.n = n.value
+
     def init(self):
         _chk(lib().orc_init(self.h))
 
     def step(self, n=1):
         _chk(lib().orc_step(self.h, n))
+        self._refresh_n()
 
     def stage(self, name):
         _chk(lib().orc_stage(self.h, STAGES[name] if isinstance(name, str) else int(name)))
+        self._refresh_n()
 
     def get(self, name, comps=None):
         if comps is None:

@@ -629,6 +629,7 @@ void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
     pv.assign(n, 0.0);
     ap.assign(3 * n, 0.0);
     vdiff.assign(3 * n, 0.0);
+    state.assign(n, 0.0);
     vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
     vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
     initialized = false;
@@ -665,6 +666,7 @@ void Sim::init() {
             bV[i] = 1.0 / delta;
         }
     }
+    nCap = n + maxEmit;
     initialized = true;
 }
 
@@ -715,7 +717,9 @@ void Sim::volume_map(size_t o) {
         const bool chk = vm->interpolate(0, xi, &val, normal);
         if (chk) dist = val;
         bool animate = false;
-        if ((dist > 0.0) && (dist < supportRadius)) {
+        if (!active(i)) {
+            // TimeStep.cpp:268: only active particles see the boundary
+        } else if ((dist > 0.0) && (dist < supportRadius)) {
             double volume;
             vm->interpolate(1, xi, &volume, nullptr);
             if ((volume > 0.0) && (volume != std::numeric_limits<double>::max())) {
@@ -893,6 +897,7 @@ void Sim::divergence_setup() {
 void Sim::pressure_accel(const std::vector<double> &prho2, size_t i, double *ai) const {
     const double eps = 1.0e-5;
     ai[0] = ai[1] = ai[2] = 0.0;
+    if (!active(i)) return;  // TimeStepDFSPH.cpp:1309-1310
     const double p_i = prho2[i];
     const double *xi = &x[3 * i];
     for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
@@ -1365,7 +1370,9 @@ void Sim::cfl() {
 
 // TimeStepDFSPH.cpp:192-208 (uses h cached at the start of the step).
 void Sim::update_velocity() {
-    for (size_t i = 0; i < 3 * n; i++) v[i] += h_step * a[i];
+    for (size_t i = 0; i < n; i++)
+        if (active(i))  // TimeStepDFSPH.cpp:201
+            for (int c = 0; c < 3; c++) v[3 * i + c] += h_step * a[3 * i + c];
 }
 
 // pressureSolve set-up loop (TimeStepDFSPH.cpp:252-309).
@@ -1391,6 +1398,8 @@ double Sim::pressure_iteration() {
     std::vector<double> err(n), over(n);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
+        err[i] = over[i] = 0.0;
+        if (!active(i)) continue;  // TimeStepDFSPH.cpp:578-579
         double aij = aij_pj(i);
         aij *= h * h;
         const double s_i = 1.0 - rho_adv[i];
@@ -1436,10 +1445,155 @@ void Sim::pressure_solve() {
     pressure_finish();
 }
 
-// TimeStepDFSPH.cpp:220-248.
+// TimeStepDFSPH.cpp:220-248: positions of active particles, then the
+// emitters (Simulation::emitParticles, Simulation.cpp:665-674) at the time of
+// the step's start, then the time advances.
 void Sim::update_position() {
-    for (size_t i = 0; i < 3 * n; i++) x[i] += h_step * v[i];
+    for (size_t i = 0; i < n; i++)
+        if (active(i))  // TimeStepDFSPH.cpp:229
+            for (int c = 0; c < 3; c++) x[3 * i + c] += h_step * v[3 * i + c];
+    emitter_system_step();
     time += h_step;
+}
+
+// ------------------------------------------------------------------ emitters
+void Sim::add_emitter(const Emitter &e) {
+    emitters.push_back(e);
+    emitters.back().next = e.start;  // Emitter::setEmitStartTime (Emitter.h:59)
+    emitters.back().counter = 0;
+}
+
+// Emitter::getSize (Emitter.cpp:40-87): extent along the emit direction,
+// then the grid's height and width axes (a box), or the cylinder's height
+// and diameter.
+void Sim::emitter_size(const Emitter &e, double *size) const {
+    const double diam = 2.0 * r, margin = diam;
+    const bool akinci = P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012;
+    if (e.type == 0) {
+        const double m = akinci ? 2.0 : 2.5;
+        size[0] = 2.0 * hs;
+        size[1] = e.height * diam + m * margin;
+        size[2] = e.width * diam + m * margin;
+    } else {
+        const double rr = 0.5 * e.width * diam + margin;
+        const double s = akinci ? 1.0 : 1.125;  // 2 r or 2.25 r across
+        size[0] = akinci ? 2.0 * hs : 2.25 * hs;
+        size[1] = size[2] = 2.0 * s * rr;
+    }
+}
+
+// Grows (or shrinks) every per-particle array to m particles; new entries are
+// zero. The per-object volume-map slices are recomputed every step.
+void Sim::resize_particles(size_t m) {
+    for (std::vector<double> *f : {&x, &v, &a, &ap, &vdiff}) f->resize(3 * m, 0.0);
+    for (std::vector<double> *f : {&rho, &factor, &rho_adv, &p, &pv, &state}) f->resize(m, 0.0);
+    n = m;
+    vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
+    vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
+}
+
+// Emitter::emitParticles / emitParticlesCircle (Emitter.cpp:89-380): animate
+// the particles inside the emitter (velocity set to the emit velocity,
+// advanced by one step, state 1), then, once per diam / velocity seconds,
+// place a new layer: first into reused particles, then into new ones while
+// the fluid has room. Returns the number of new particles.
+size_t Sim::emit(Emitter &e, const std::vector<size_t> &reusedList, size_t &indexReuse) {
+    const double t = time, timeStepSize = dt;
+    const double dir[3] = {e.R[0], e.R[3], e.R[6]};
+    const double axisHeight[3] = {e.R[1], e.R[4], e.R[7]}, axisWidth[3] = {e.R[2], e.R[5], e.R[8]};
+    double emitVel[3];
+    for (int c = 0; c < 3; c++) emitVel[c] = e.velocity * dir[c];
+    const double diam = 2.0 * r;
+    if (t < e.start || t > e.end)  // clear the emitter shortly before it starts
+        for (int c = 0; c < 3; c++) emitVel[c] = dir[c] * r * 10.0 / 0.25;
+    if (t >= e.start - 0.25 && t <= e.end) {
+        double size[3];
+        emitter_size(e, size);
+        double pos[3];
+        for (int c = 0; c < 3; c++) pos[c] = e.x[c] + 0.5 * hs * dir[c];
+        const double r2 = (0.5 * size[1]) * (0.5 * size[1]);
+        for (size_t i = 0; i < n; i++) {
+            const double d[3] = {x[3 * i] - pos[0], x[3 * i + 1] - pos[1], x[3 * i + 2] - pos[2]};
+            double loc[3];  // R^T d
+            for (int c = 0; c < 3; c++) loc[c] = (e.R[c] * d[0] + e.R[3 + c] * d[1]) + e.R[6 + c] * d[2];
+            bool inside;
+            if (e.type == 1) {
+                const double d2 = loc[1] * loc[1] + loc[2] * loc[2];
+                inside = loc[0] > -0.5 * size[0] && loc[0] < 0.5 * size[0] && d2 < r2;
+            } else {
+                inside = std::fabs(loc[0]) < 0.5 * size[0] && std::fabs(loc[1]) < 0.5 * size[1] &&
+                         std::fabs(loc[2]) < 0.5 * size[2];
+            }
+            if (inside) {
+                for (int c = 0; c < 3; c++) {
+                    v[3 * i + c] = emitVel[c];
+                    x[3 * i + c] += timeStepSize * emitVel[c];
+                }
+                state[i] = 1.0;
+            }
+        }
+    }
+    if (t < e.next || t > e.end) return 0;
+    const unsigned rows = e.type == 1 ? e.width : e.height;
+    const double startX = -0.5 * (e.width - 1.0) * diam;
+    const double startZ = -0.5 * (rows - 1.0) * diam;
+    const double radius = 0.5 * e.width * diam, radius2 = radius * radius;
+    const double tt = t - e.next + timeStepSize;
+    double offset[3];
+    for (int c = 0; c < 3; c++) offset[c] = e.x[c] + tt * emitVel[c];
+    size_t numEmitted = 0;
+    if (n < nCap || !reusedList.empty()) {
+        const size_t n0 = n;
+        size_t indexNotReuse = n0;
+        for (unsigned i = 0; i < e.width; i++) {
+            for (unsigned j = 0; j < rows; j++) {
+                const double a = i * diam + startX, b = j * diam + startZ;
+                if (e.type == 1 && !(a * a + b * b <= radius2)) continue;
+                const bool fromReuse = indexReuse < reusedList.size();
+                const size_t index = fromReuse ? reusedList[indexReuse] : indexNotReuse;
+                if (index >= nCap) continue;
+                if (index >= n) resize_particles(index + 1);
+                for (int c = 0; c < 3; c++) {
+                    x[3 * index + c] = a * axisWidth[c] + b * axisHeight[c] + offset[c];
+                    v[3 * index + c] = emitVel[c];
+                }
+                state[index] = 1.0;
+                if (fromReuse) {
+                    indexReuse++;
+                } else {
+                    numEmitted++;
+                    indexNotReuse++;
+                }
+            }
+        }
+        // TimeStepDFSPH::emittedParticles: zero stiffness of the new particles
+        // (SimulationDataDFSPH.cpp:103-112); resize_particles zeroed them.
+    }
+    e.next += diam / e.velocity;
+    e.counter++;
+    return numEmitted;
+}
+
+// EmitterSystem::step (EmitterSystem.cpp:54-80) with reuseParticles (:34-52).
+void Sim::emitter_system_step() {
+    if (emitters.empty()) return;
+    for (size_t i = 0; i < n; i++)
+        if (state[i] == 1.0) state[i] = 0.0;
+    std::vector<size_t> reusedList;
+    if (reuse) {
+        for (size_t i = 0; i < n; i++) {
+            const double *xi = &x[3 * i];
+            bool out = false;
+            for (int c = 0; c < 3; c++) out = out || xi[c] < reuseMin[c] || xi[c] > reuseMax[c];
+            if (out) {
+                reusedList.push_back(i);
+                for (int c = 0; c < 3; c++) v[3 * i + c] *= 0.95;
+            }
+        }
+    }
+    size_t indexReuse = 0;
+    for (Emitter &e : emitters) emitted += emit(e, reusedList, indexReuse);
+    reused += indexReuse;
 }
 
 // TimeStepDFSPH::step (TimeStepDFSPH.cpp:117-249).
@@ -1532,6 +1686,7 @@ bool field_of(Sim *s, const char *name, FieldRef &f) {
     else if (n == "p_v / rho^2") f = {&s->pv, 1};
     else if (n == "pressure acceleration") f = {&s->ap, 3};
     else if (n == "velocity difference") f = {&s->vdiff, 3};
+    else if (n == "state") f = {&s->state, 1};
     else if (n == "boundary particle volume") f = {&s->bV, 1};
     else return false;
     return true;
@@ -1637,6 +1792,44 @@ int orc_add_volume_map_sphere(void *h, const double *centre, double radius, int 
     ((Sim *)h)->add_volume_map(s, res);
     return SPH_OK;
 }
+int orc_add_emitter(void *h, int type, unsigned width, unsigned height, const double *x, const double *R,
+                    double velocity, double start, double end) {
+    if (type != 0 && type != 1) return fail("emitter type must be 0 or 1");
+    if (!(velocity > 0.0)) return fail("emitter velocity must be positive");
+    orc::Emitter e;
+    e.type = type;
+    e.width = width;
+    e.height = height;
+    for (int c = 0; c < 3; c++) e.x[c] = x[c];
+    for (int c = 0; c < 9; c++) e.R[c] = R[c];
+    e.velocity = velocity;
+    e.start = start;
+    e.end = end;
+    ((Sim *)h)->add_emitter(e);
+    return SPH_OK;
+}
+int orc_set_emitter_reuse(void *h, int enable, const double *lo, const double *hi) {
+    Sim *s = (Sim *)h;
+    s->reuse = enable != 0;
+    if (lo && hi)
+        for (int c = 0; c < 3; c++) {
+            s->reuseMin[c] = lo[c];
+            s->reuseMax[c] = hi[c];
+        }
+    return SPH_OK;
+}
+int orc_set_max_emitter_particles(void *h, size_t m) {
+    Sim *s = (Sim *)h;
+    s->maxEmit = m;
+    if (s->initialized) s->nCap = s->n + m;
+    return SPH_OK;
+}
+int orc_emitter_counts(void *h, size_t *emitted, size_t *reused) {
+    Sim *s = (Sim *)h;
+    if (emitted) *emitted = s->emitted;
+    if (reused) *reused = s->reused;
+    return SPH_OK;
+}
 int orc_init(void *h) {
     ((Sim *)h)->init();
     return SPH_OK;
@@ -1656,7 +1849,8 @@ int orc_get_field(void *h, const char *name, double *out, size_t count) {
     if (n == "fluid neighbor count" || n == "boundary neighbor count") {
         const CSR &c = (n[0] == 'f') ? s->nf : s->nbd;
         if (count != s->n) return fail("size mismatch");
-        for (size_t i = 0; i < s->n; i++) out[i] = c.off.empty() ? 0.0 : (double)c.count(i);
+        // particles added after the last search have no list yet
+        for (size_t i = 0; i < s->n; i++) out[i] = i + 1 < c.off.size() ? (double)c.count(i) : 0.0;
         return SPH_OK;
     }
     double *vm = nullptr;

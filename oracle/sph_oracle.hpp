@@ -8,6 +8,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -113,6 +114,19 @@ CGResult conjugate_gradient(size_t n, MatVec &&A, Precond &&M, const std::vector
                             std::vector<double> &x, int maxIters, double tol);
 void inverse3(const double *m, double *inv);  // Eigen cofactor inverse
 
+// One emitter (Emitter.h:14-70): a width x height grid of particles (type 0)
+// or a disc of width particles across (type 1) leaving m_x along column 0 of
+// the rotation R (row-major); columns 2 and 1 span the grid.
+struct Emitter {
+    int type = 0;
+    unsigned width = 0, height = 0;
+    double x[3] = {0, 0, 0};
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double velocity = 0, start = 0, end = std::numeric_limits<double>::max();
+    double next = 0;  // m_nextEmitTime
+    unsigned counter = 0;
+};
+
 class Sim {
 public:
     explicit Sim(const sph_dfsph_params_t &p);
@@ -121,6 +135,7 @@ public:
     PreCubic K;
     size_t n = 0, nb = 0;
     std::vector<double> x, v, a, rho, factor, rho_adv, p, pv, ap, vdiff;
+    std::vector<double> state;          // ParticleState (FluidModel.h:75): 0 active, 1 animated by an emitter, 2 fixed
     std::vector<double> bx, bV;         // Akinci particles and volumes
     std::vector<std::unique_ptr<VolumeMap>> vms;  // boundary objects in insertion order
     std::vector<double> vmV, vmX;       // per object and fluid particle: V_b [o * n + i], x_b
@@ -131,6 +146,18 @@ public:
     double lastErr = 0, lastErrV = 0;
     double lastOver = 0, lastOverV = 0;  // excess: positive residual where p > 0 (not a reference quantity)
     bool initialized = false;
+    // EmitterSystem (EmitterSystem.h:16-60); numParticles = particles at
+    // init() + maxEmit (Simulation::addFluidModel, FluidModel::initModel).
+    std::vector<Emitter> emitters;
+    bool reuse = false;
+    double reuseMin[3] = {-1, -1, -1}, reuseMax[3] = {1, 1, 1};
+    size_t maxEmit = 10000, nCap = 0, emitted = 0, reused = 0;
+    void add_emitter(const Emitter &e);
+    void emitter_system_step();
+    size_t emit(Emitter &e, const std::vector<size_t> &reusedList, size_t &indexReuse);
+    void emitter_size(const Emitter &e, double *size) const;
+    void resize_particles(size_t m);
+    bool active(size_t i) const { return state.empty() || state[i] == 0.0; }
 
     void set_fluid(const double *x, const double *v, size_t n);
     void add_boundary(const double *x, size_t n);

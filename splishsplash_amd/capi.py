@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .params import DFSPHCounters, DFSPHParams, FIELD_COMPONENTS, STAGES, default_params
+from .params import DFSPHCounters, DFSPHParams, Emitter, FIELD_COMPONENTS, STAGES, default_params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsph_amd.so")
@@ -86,6 +86,10 @@ _SIGS = {
     "sph_scene_body": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, C.POINTER(C.c_int), _dp, _up]),
     "sph_scene_create_dfsph": (C.c_int, [_vp, C.POINTER(_vp)]),
     "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
+    "sph_dfsph_add_emitter": (C.c_int, [_vp, C.c_void_p]),
+    "sph_dfsph_set_emitter_reuse": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "sph_dfsph_set_max_emitter_particles": (C.c_int, [_vp, C.c_size_t]),
+    "sph_dfsph_emitter_counts": (C.c_int, [_vp, _szp, _szp]),
     "sph_dfsph_add_boundary_particles": (C.c_int, [_vp, _fp, C.c_size_t]),
     "sph_dfsph_set_volume_map_box": (C.c_int, [_vp, _dp, _dp, C.c_int, C.c_double, _up]),
     "sph_dfsph_add_volume_map_sphere": (C.c_int, [_vp, _dp, C.c_double, C.c_int, C.c_double, _up]),
@@ -518,11 +522,52 @@ class TimeStepDFSPH:
         _chk(lib().sph_dfsph_emit_particles(self.h, x.ctypes.data_as(_fp), v.ctypes.data_as(_fp), len(x)))
         self.n += len(x)
 
+    def add_emitter(self, width, height, pos, rotation, velocity, type=0, emit_start_time=0.0,
+                    emit_end_time=float(np.finfo(np.float64).max)):
+        """EmitterSystem::addEmitter (EmitterSystem.cpp:93-104) with the
+        emitter's start and end times; `rotation` is a 3x3 matrix whose
+        column 0 is the emit direction."""
+        e = Emitter()
+        e.type = int(type)
+        e.width = int(width)
+        e.height = int(height)
+        e.translation[:] = [float(c) for c in np.asarray(pos, np.float64).ravel()[:3]]
+        e.rotation[:] = [float(c) for c in np.asarray(rotation, np.float64).reshape(9)]
+        e.velocity = float(velocity)
+        e.emit_start_time = float(emit_start_time)
+        e.emit_end_time = float(emit_end_time)
+        _chk(lib().sph_dfsph_add_emitter(self.h, C.byref(e)))
+
+    def enable_reuse_particles(self, box_min=(-1.0, -1.0, -1.0), box_max=(1.0, 1.0, 1.0)):
+        """EmitterSystem::enableReuseParticles (EmitterSystem.cpp:106-111)."""
+        lo = np.ascontiguousarray(box_min, np.float64)
+        hi = np.ascontiguousarray(box_max, np.float64)
+        _chk(lib().sph_dfsph_set_emitter_reuse(self.h, 1, lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp)))
+
+    def disable_reuse_particles(self):
+        _chk(lib().sph_dfsph_set_emitter_reuse(self.h, 0, None, None))
+
+    def set_max_emitter_particles(self, m):
+        _chk(lib().sph_dfsph_set_max_emitter_particles(self.h, int(m)))
+
+    def emitter_counts(self):
+        """(sum of emitted particles, sum of reused particles)."""
+        e, r = C.c_size_t(0), C.c_size_t(0)
+        _chk(lib().sph_dfsph_emitter_counts(self.h, C.byref(e), C.byref(r)))
+        return e.value, r.value
+
+    def _refresh_n(self):
+        n = C.c_size_t(0)
+        _chk(lib().sph_dfsph_num_particles(self.h, C.byref(n)))
+        self.n = n.value
+
     def step(self, n=1):
         _chk(lib().sph_dfsph_step(self.h, n))
+        self._refresh_n()
 
     def stage(self, name):
         _chk(lib().sph_dfsph_stage(self.h, STAGES[name] if isinstance(name, str) else int(name)))
+        self._refresh_n()
 
     def get(self, name):
         if name == "boundary particle volume":
@@ -532,13 +577,13 @@ class TimeStepDFSPH:
                 comps = 3 if name.startswith("boundary x") else 1  # per volume-map object
             else:
                 comps = FIELD_COMPONENTS[name]
-            dt = np.uint32 if name in ("id", "fluid neighbor count", "boundary neighbor count") else np.float32
+            dt = np.uint32 if name in ("id", "state", "fluid neighbor count", "boundary neighbor count") else np.float32
             out = np.zeros((self.n, comps) if comps > 1 else self.n, dt)
         _chk(lib().sph_dfsph_get_field(self.h, name.encode(), out.ctypes.data, out.nbytes))
         return out
 
     def set(self, name, values):
-        a = np.ascontiguousarray(values, dtype=np.float32)
+        a = np.ascontiguousarray(values, dtype=np.uint32 if name == "state" else np.float32)
         _chk(lib().sph_dfsph_set_field(self.h, name.encode(), a.ctypes.data, a.nbytes))
 
     def neighbors(self, boundary=False):

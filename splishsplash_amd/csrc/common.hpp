@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -72,6 +73,17 @@ template <class T> struct DBuf {
         release();
         if (m == 0) return;
         SPH_HIP(hipMalloc(&p, m * sizeof(T)));
+        n = m;
+    }
+    // Grows to at least m elements keeping the first `keep` (device copy;
+    // the caller has finished the work that reads or writes them).
+    void grow_keep(size_t m, size_t keep) {
+        if (m <= n) return;
+        T *q = nullptr;
+        SPH_HIP(hipMalloc(&q, m * sizeof(T)));
+        if (p && keep) SPH_HIP(hipMemcpy(q, p, std::min(keep, n) * sizeof(T), hipMemcpyDeviceToDevice));
+        release();
+        p = q;
         n = m;
     }
     void swap(DBuf &o) {

@@ -58,6 +58,10 @@ __global__ void k_reset(Ctl *C, int which) {
 template <int LPP>
 __device__ __forceinline__ float lane_sum(float v);
 
+// ParticleState::Active (FluidModel.h:75) unless the state array says
+// otherwise; without states every particle is active.
+__device__ __forceinline__ bool is_active(const Dev &D, unsigned i) { return D.state == nullptr || D.state[i] == 0u; }
+
 template <bool DENS, bool FACT, bool DIV, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
     const unsigned t = gtid();
@@ -287,10 +291,13 @@ __global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
     const unsigned t = gtid();
     const unsigned i = t / LPP, h = t % LPP;
     const bool act = i < D.n;
+    // computePressureAccel leaves a_p = 0 for particles that are not active
+    // (TimeStepDFSPH.cpp:1303-1310)
+    const bool live = act && is_active(D, i);
     const float *__restrict__ P = PRESSURE ? D.p : D.pv;
     float ax = 0.f, ay = 0.f, az = 0.f, pi = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (act) {
+    if (live) {
         xi = D.x[i];
         pi = P[i];
         const unsigned cnt = D.cf[i];
@@ -357,7 +364,9 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
     if (check_done && (PRESSURE ? C->done : C->done_v)) return;
     const unsigned t = gtid();
     const unsigned i = t / LPP, hl = t % LPP;
-    const bool act = i < D.n;
+    // the pressure update skips particles that are not active
+    // (TimeStepDFSPH.cpp:578-579); the divergence update takes every particle
+    const bool act = i < D.n && (!PRESSURE || is_active(D, i));
     const double h = C->h;
     double err = 0.0, over = 0.0;
     float s = 0.f;
@@ -518,11 +527,12 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev D) {
     const unsigned t = gtid();
     const unsigned i = t / LPP, hl = t % LPP;
     const bool act = i < D.n;
+    const bool live = act && is_active(D, i);  // a_p = 0 otherwise (TimeStepDFSPH.cpp:1309)
     const float *__restrict__ P = PRESSURE ? D.p : D.pv;
     float3 a = make_float3(0.f, 0.f, 0.f);
     float pi = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (act) {
+    if (live) {
         xi = D.x[i];
         pi = P[i];
         if (LPP == 1) {
@@ -535,7 +545,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(Dev D) {
         a.x = lane_sum<LPP>(a.x);
         a.y = lane_sum<LPP>(a.y);
         a.z = lane_sum<LPP>(a.z);
-        if (act && hl == 0) accel_boundary(D, i, xi, pi, a);
+        if (live && hl == 0) accel_boundary(D, i, xi, pi, a);
     }
     if (!act || hl != 0) return;
     const float h = (float)D.ctl->h;
@@ -720,8 +730,10 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
             vm = (wx * wx + wy * wy) + wz * wz;
         }
         if (VUPD) {
+            // active particles only (TimeStepDFSPH.cpp:201)
             const float hs = (float)C->h_step;
-            D.v2[i] = make_float4(vi.x + hs * ai.x, vi.y + hs * ai.y, vi.z + hs * ai.z, 0.f);
+            D.v2[i] = is_active(D, i) ? make_float4(vi.x + hs * ai.x, vi.y + hs * ai.y, vi.z + hs * ai.z, 0.f)
+                                      : make_float4(vi.x, vi.y, vi.z, 0.f);
         }
         if (SCALEPV) D.pv[i] *= hf;
     }
@@ -819,6 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_final(Dev D) {
     Ctl *C = D.ctl;
     if (UPDX && blockIdx.x == 0 && threadIdx.x == 0) {
         // the time advances once per launch; every block reads h_step only
+        C->t_prev = C->time;
         C->time += C->h_step;
     }
     if (i >= D.n) return;
@@ -826,7 +839,7 @@ __global__ __launch_bounds__(BLOCK) void k_final(Dev D) {
         const double h = C->h;
         D.p[i] *= (float)(h * h);
     }
-    if (UPDX) {
+    if (UPDX && is_active(D, i)) {  // TimeStepDFSPH.cpp:229
         const float hs = (float)C->h_step;
         const float4 v = D.v[i];
         float4 x = D.x[i], c = D.xc[i];
@@ -1304,6 +1317,10 @@ template <bool PRESSURE>
 __global__ __launch_bounds__(BLOCK) void k_accel2(Dev D) {
     const unsigned i = gtid();
     if (i >= D.n) return;
+    if (!is_active(D, i)) {
+        D.ap[i] = D.ap2[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
     const float *__restrict__ P = PRESSURE ? D.p : D.pv;
     const float *__restrict__ Q = D.pred;
     const float4 xi = D.x[i];
@@ -1389,7 +1406,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
     const double h = C->h;
     double err = 0.0, over = 0.0;
     bool took = false;
-    if (i < D.n) {
+    if (i < D.n && (!PRESSURE || is_active(D, i))) {
         const float4 xi = D.x[i], ai = D.ap[i], bi = D.ap2[i];
         float s1 = 0.f, s2 = 0.f;
         const unsigned cnt = D.cf[i];
@@ -1599,6 +1616,7 @@ Dev DFSPH::dev() const {
     D.vdiff = vdiff_.get();
     D.xc = xc_.get();
     D.vmb = vmb_.get();
+    D.state = states_ ? pst_.get() : nullptr;
     D.nvm = nvm_;
     D.rho = rho_.get();
     D.alpha = alpha_.get();
@@ -1689,6 +1707,10 @@ void DFSPH::alloc_particles() {
     for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_, &p2_, &pv2_}) b->reserve(S);
     id_.reserve(S);
     id2_.reserve(S);
+    if (states_) {
+        pst_.reserve(S);
+        pst2_.reserve(S);
+    }
     cf_.reserve(S);
     cb_.reserve(S);
     partial_.reserve(12 * (size_t)nblocks(S) + 64);  // up to three values per block of up to 4 lanes per particle
@@ -1728,12 +1750,19 @@ void DFSPH::upload_particles(size_t n, const float *x, const float *v, const flo
     SPH_HIP(hipMemcpy(p_.get(), p ? p : zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
     SPH_HIP(hipMemcpy(pv_.get(), pv ? pv : zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
     SPH_HIP(hipMemcpy(id_.get(), ids.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    if (states_) {
+        const std::vector<unsigned> active(n, 0u);
+        SPH_HIP(hipMemcpy(pst_.get(), active.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
 }
 
 void DFSPH::init() {
     SPH_REQUIRE(n_ > 0, SPH_ERR_STATE, "set_fluid must be called first");
     const size_t n = n_;
-    stride_ = (unsigned)((n + 255) / 256 * 256);
+    // room for the emitters' particles up front (the reference's capacity
+    // numParticles = initial + maxEmitterParticles, FluidModel::initModel)
+    cap_emit_ = n + max_emit_;
+    stride_ = (unsigned)(((emitters_.empty() ? n : cap_emit_) + 255) / 256 * 256);
     const size_t S = stride_;
     const bool bender = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019;
     const size_t nobj = bender ? vm_shapes_.size() : 0;
@@ -1756,6 +1785,15 @@ void DFSPH::init() {
             lo[c] = std::min(lo[c], sp ? sh.centre[c] - sh.radius : sh.box_lo[c]);
             hi[c] = std::max(hi[c], sp ? sh.centre[c] + sh.radius : sh.box_hi[c]);
         }
+    for (const EmitterDesc &e : emitters_) {
+        double size[3];
+        emitter_size(e, size);
+        const double L = 0.5 * std::max(size[0], std::max(size[1], size[2])) + hs;
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], e.x[c] - L);
+            hi[c] = std::max(hi[c], e.x[c] + L);
+        }
+    }
     for (int c = 0; c < 3; c++) {
         lo[c] -= 2.0 * hs;
         hi[c] += 2.0 * hs;
@@ -1785,7 +1823,7 @@ void DFSPH::init() {
         for (;;) {
             bl.reserve((size_t)cap * bstride);
             k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 2);
-            build_lists_colmajor(stream, bx_.get(), (unsigned)nbp, bx_.get(), bstart_.get(), grid_, K.h2, true,
+            build_lists_colmajor(stream, bx_.get(), (unsigned)nbp, bx_.get(), bstart_.get(), grid_, hs, true,
                                  bl.get(), bstride, cap, bc.get(), &ctl_.get()->maxcnt_f, timer, "boundary_lists");
             read_ctl();
             if (hctl_->maxcnt_f <= cap) break;
@@ -1823,6 +1861,12 @@ void DFSPH::reset() {
     capped_ = cappedV_ = 0;
     learnedSolves_ = learnedSolvesV_ = 0;
     engaged_[0] = engaged_[1] = false;
+    // EmitterSystem::reset (EmitterSystem.cpp:82-91)
+    emitted_ = reused_ = 0;
+    for (EmitterDesc &e : emitters_) {
+        e.next = e.start;
+        e.counter = 0;
+    }
     if (!initialized_) return;
     const size_t n = n_;
     for (DBuf<float> *b : {&alpha_, &radv_, &p_, &pv_})
@@ -1833,75 +1877,16 @@ void DFSPH::reset() {
     SPH_HIP(hipStreamSynchronize(stream));
 }
 
-// Particles added by an emitter (Emitter.cpp:180-225 activates them and
-// calls emittedParticles, which zeroes their stiffness: TimeStepDFSPH.cpp:
-// 723-731, SimulationDataDFSPH.cpp:103-112). The existing particles keep
-// their state; the device arrays grow when needed (resize). The state is
-// rebuilt in id order with the new particles appended; the cell grid stays
-// that of init() (particles outside it fall into its border cells, which
-// the search handles exactly, only with more candidates).
-void DFSPH::emit_particles(const float *x, const float *v, size_t k) {
-    if (k == 0) return;
-    SPH_REQUIRE(x != nullptr, SPH_ERR_INVALID, "null positions");
-    SPH_REQUIRE(n_ + k < (size_t)0x7fffffff, SPH_ERR_INVALID, "too many particles");
-    if (!initialized_) {
-        hx_.insert(hx_.end(), x, x + 3 * k);
-        if (v)
-            hv_.insert(hv_.end(), v, v + 3 * k);
-        else
-            hv_.insert(hv_.end(), 3 * k, 0.f);
-        n_ += k;
-        return;
-    }
-    const size_t n = n_, m = n + k;
-    SPH_HIP(hipStreamSynchronize(stream));
-    std::vector<float4> dx(n), dv(n), dxc(n), dvd(n);
-    std::vector<float> dp(n), dpv(n);
-    std::vector<unsigned> id(n);
-    SPH_HIP(hipMemcpy(dx.data(), x_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(dv.data(), v_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(dxc.data(), xc_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(dvd.data(), vdiff_.get(), n * sizeof(float4), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(dp.data(), p_.get(), n * sizeof(float), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(dpv.data(), pv_.get(), n * sizeof(float), hipMemcpyDeviceToHost));
-    SPH_HIP(hipMemcpy(id.data(), id_.get(), n * sizeof(unsigned), hipMemcpyDeviceToHost));
-    std::vector<float> hx(3 * m), hv(3 * m, 0.f), hp(m, 0.f), hpv(m, 0.f);
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    std::vector<float4> hxc(m, z4), hvd(m, z4);
-    for (size_t d = 0; d < n; d++) {
-        const unsigned t = id[d];
-        SPH_REQUIRE(t < n, SPH_ERR_STATE, "particle ids are not a permutation");
-        hx[3 * t] = dx[d].x, hx[3 * t + 1] = dx[d].y, hx[3 * t + 2] = dx[d].z;
-        hv[3 * t] = dv[d].x, hv[3 * t + 1] = dv[d].y, hv[3 * t + 2] = dv[d].z;
-        hxc[t] = dxc[d];
-        hvd[t] = dvd[d];
-        hp[t] = dp[d];
-        hpv[t] = dpv[d];
-    }
-    std::copy(x, x + 3 * k, hx.begin() + 3 * n);
-    if (v) std::copy(v, v + 3 * k, hv.begin() + 3 * n);
-    n_ = m;
-    if (m > stride_) {
-        stride_ = (unsigned)((m + m / 4 + 255) / 256 * 256);  // headroom for further emission
-        alloc_particles();
-    }
-    upload_particles(m, hx.data(), hv.data(), hxc.data(), hvd.data(), hp.data(), hpv.data());
-    hx_.assign(hx.begin(), hx.end());
-    hv_.assign(hv.begin(), hv.end());
-    have_lists_ = false;
-    tuned_ = lpp_fixed_;
-}
-
 void DFSPH::build_lists() {
     const Dev D = dev();
     for (;;) {
         nf_.reserve((size_t)capF_ * stride_);
         if (D.bmode == 1) nb_l_.reserve((size_t)capB_ * stride_);
         k_reset<<<1, 1, 0, stream>>>(ctl_.get(), 2);
-        build_lists_colmajor(stream, x_.get(), (unsigned)n_, x_.get(), fsort_.start.get(), grid_, K.h2, true,
+        build_lists_colmajor(stream, x_.get(), (unsigned)n_, x_.get(), fsort_.start.get(), grid_, hs, true,
                              nf_.get(), stride_, capF_, cf_.get(), &ctl_.get()->maxcnt_f, timer, "fluid_lists");
         if (P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0)
-            build_lists_colmajor(stream, x_.get(), (unsigned)n_, bx_.get(), bstart_.get(), grid_, K.h2, false,
+            build_lists_colmajor(stream, x_.get(), (unsigned)n_, bx_.get(), bstart_.get(), grid_, hs, false,
                                  nb_l_.get(), stride_, capB_, cb_.get(), &ctl_.get()->maxcnt_b, timer,
                                  "boundary_lists");
         read_ctl();
@@ -1941,6 +1926,10 @@ void DFSPH::neighborhood_search() {
         pv_.swap(pv2_);
         permute_u1(stream, perm, n, id_.get(), id2_.get());
         id_.swap(id2_);
+        if (states_) {
+            permute_u1(stream, perm, n, pst_.get(), pst2_.get());
+            pst_.swap(pst2_);
+        }
     }
     build_lists();
 }
@@ -1950,12 +1939,13 @@ void DFSPH::volume_maps() {
     // objects in insertion order: a particle projected out of one object is
     // sampled against the next at its new position (TimeStep.cpp:177-222)
     if (nvm_ == 1) {
-        sample_volume_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(), hs, r, timer);
+        sample_volume_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(),
+                          states_ ? pst_.get() : nullptr, hs, r, timer);
         return;
     }
     for (unsigned o = 0; o < nvm_; o++)
         sample_volume_map(stream, vmaps[o]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(),
-                          vmraw_.get() + (size_t)o * stride_, hs, r, timer);
+                          vmraw_.get() + (size_t)o * stride_, states_ ? pst_.get() : nullptr, hs, r, timer);
     pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer);
 }
 
@@ -2464,6 +2454,7 @@ void DFSPH::step() {
     totIt_ += itLast_;
     pressure_finish(true);
     collect(true);
+    emitter_system_step();
     soff_++;
     steps_++;
 }
@@ -2525,7 +2516,10 @@ void DFSPH::stage(int s) {
             pressure_finish(false);
             finalize(true, false);
             break;
-        case SPH_STAGE_UPDATE_POSITION: finalize(false, true); break;
+        case SPH_STAGE_UPDATE_POSITION:
+            finalize(false, true);
+            emitter_system_step();
+            break;
         default: throw Error(SPH_ERR_INVALID, "unknown stage");
     }
     SPH_HIP(hipStreamSynchronize(stream));
@@ -2592,6 +2586,7 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
     else if (name == "p / rho^2") f1 = p_.get();
     else if (name == "p_v / rho^2") f1 = pv_.get();
     else if (name == "id") u1 = id_.get();
+    else if (name == "state") u1 = pst_.get();
     else if (name == "fluid neighbor count") u1 = cf_.get();
     else if (name == "boundary neighbor count") u1 = cb_.get();
     else throw Error(SPH_ERR_NOTFOUND, "unknown field '" + name + "'");
@@ -2618,7 +2613,8 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
         for (size_t s = 0; s < n; s++) out[id[s]] = t[s];
     } else {
         std::vector<unsigned> t(n);
-        if (name == "boundary neighbor count" && !(P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0))
+        if ((name == "boundary neighbor count" && !(P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb_ > 0)) ||
+            (name == "state" && !states_))
             std::fill(t.begin(), t.end(), 0u);
         else
             SPH_HIP(hipMemcpy(t.data(), u1, n * sizeof(unsigned), hipMemcpyDeviceToHost));
@@ -2643,9 +2639,22 @@ void DFSPH::set_field(const std::string &name, const void *host, size_t bytes) {
     else if (name == "advected density") f1 = radv_.get();
     else if (name == "p / rho^2") f1 = p_.get();
     else if (name == "p_v / rho^2") f1 = pv_.get();
-    else throw Error(SPH_ERR_NOTFOUND, "unknown or read-only field '" + name + "'");
+    else if (name != "state") throw Error(SPH_ERR_NOTFOUND, "unknown or read-only field '" + name + "'");
     SPH_REQUIRE(bytes == n * comps * 4, SPH_ERR_INVALID, "size mismatch for field '" + name + "'");
     const std::vector<unsigned> id = ids_host();
+    if (name == "state") {
+        // ParticleState: 0 active, 1 animated by an emitter, 2 fixed
+        const unsigned *in = static_cast<const unsigned *>(host);
+        std::vector<unsigned> t(n);
+        for (size_t s = 0; s < n; s++) {
+            t[s] = in[id[s]];
+            SPH_REQUIRE(t[s] <= 2u, SPH_ERR_INVALID, "particle state must be 0, 1 or 2");
+        }
+        enable_states();
+        SPH_HIP(hipStreamSynchronize(stream));
+        SPH_HIP(hipMemcpy(pst_.get(), t.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+        return;
+    }
     const float *in = static_cast<const float *>(host);
     if (f4) {
         std::vector<float4> t(n);

@@ -34,12 +34,14 @@ struct Ctl {
     double kref, kref_v;          // stiffness scales of the learned start
     unsigned nlearn, nlearn_v;    // particles that kept the learned start
     unsigned nsamp, nsamp_v;      // training samples written
+    double t_prev;                // time at the start of the last completed step
 };
 
 // Everything a sweep kernel needs, passed by value.
 struct Dev {
     unsigned n, stride;
     float4 *x, *v, *v2, *a, *ap, *vdiff, *xc, *vmb;
+    const unsigned *state;  // ParticleState per particle (0 active); null: all active
     float *rho, *alpha, *radv, *p, *pv;
     const unsigned *nf, *cf, *nb, *cb;
     const float4 *bx;  // Akinci particles (x, y, z, volume), cell-sorted
@@ -73,6 +75,19 @@ struct Dev {
     unsigned scap, scap_v, sstride, sstride_v, soff;
 };
 
+// One emitter (Emitter.h:14-70): a width x height grid (type 0) or a disc
+// of width particles across (type 1) leaving `x` along column 0 of the
+// row-major rotation R, at `velocity`, between the start and end times.
+struct EmitterDesc {
+    int type = 0;
+    unsigned width = 0, height = 0;
+    double x[3] = {0, 0, 0};
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double velocity = 0, start = 0, end = 0;
+    double next = 0;       // m_nextEmitTime
+    unsigned counter = 0;  // m_emitCounter
+};
+
 class DFSPH {
 public:
     explicit DFSPH(const sph_dfsph_params_t &p);
@@ -97,6 +112,13 @@ public:
     }
     void reset();
     void emit_particles(const float *x, const float *v, size_t n);
+    // Emitter system (EmitterSystem.h/.cpp)
+    void add_emitter(const EmitterDesc &e);
+    void set_emitter_reuse(bool enable, const double *lo, const double *hi);
+    void set_max_emitter_particles(size_t m);
+    size_t emitted() const { return emitted_; }
+    size_t reused() const { return reused_; }
+    const std::vector<EmitterDesc> &emitters() const { return emitters_; }
     void step();
     void stage(int s);
 
@@ -116,6 +138,19 @@ public:
     std::vector<std::unique_ptr<VolumeMap>> vmaps;  // boundary objects, in insertion order
 
 private:
+    // particle states and emitters
+    DBuf<unsigned> pst_, pst2_, mark_, slots_;  // particle states and their permutation target
+    DBuf<float4> emit_x_, emit_v_;
+    bool states_ = false;  // pst_ holds the particles' states (else all are active)
+    std::vector<EmitterDesc> emitters_;
+    bool reuse_ = false;
+    double reuse_lo_[3] = {-1, -1, -1}, reuse_hi_[3] = {1, 1, 1};
+    size_t max_emit_ = 10000, cap_emit_ = 0, emitted_ = 0, reused_ = 0;
+    void enable_states();
+    void emitter_system_step();
+    void emitter_size(const EmitterDesc &e, double *size) const;
+    void grow_particles(size_t m);
+
     size_t n_ = 0;
     unsigned stride_ = 0;
     bool initialized_ = false, have_lists_ = false;

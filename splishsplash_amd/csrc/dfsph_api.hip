@@ -134,6 +134,46 @@ int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v, size
     });
 }
 
+int sph_dfsph_add_emitter(sph_dfsph_t s, const sph_emitter_t *e) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(e != nullptr, SPH_ERR_INVALID, "null emitter");
+        sph::EmitterDesc d;
+        d.type = e->type;
+        d.width = e->width;
+        d.height = e->height;
+        for (int c = 0; c < 3; c++) d.x[c] = e->translation[c];
+        for (int c = 0; c < 9; c++) d.R[c] = e->rotation[c];
+        d.velocity = e->velocity;
+        d.start = e->emit_start_time;
+        d.end = e->emit_end_time;
+        s->s->add_emitter(d);
+    });
+}
+
+int sph_dfsph_set_emitter_reuse(sph_dfsph_t s, int enable, const double box_min[3], const double box_max[3]) {
+    return guarded([&] {
+        DCHK(s);
+        SPH_REQUIRE(!enable || (box_min && box_max), SPH_ERR_INVALID, "null reuse box");
+        s->s->set_emitter_reuse(enable != 0, box_min, box_max);
+    });
+}
+
+int sph_dfsph_set_max_emitter_particles(sph_dfsph_t s, size_t max_particles) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_max_emitter_particles(max_particles);
+    });
+}
+
+int sph_dfsph_emitter_counts(sph_dfsph_t s, size_t *emitted, size_t *reused) {
+    return guarded([&] {
+        DCHK(s);
+        if (emitted) *emitted = s->s->emitted();
+        if (reused) *reused = s->s->reused();
+    });
+}
+
 int sph_dfsph_step(sph_dfsph_t s, int n_steps) {
     return guarded([&] {
         DCHK(s);

@@ -1,0 +1,365 @@
+// emitter.hip -- particle states and the emitter system of the DFSPH handle
+// (SPlisHSPlasH/Emitter.cpp, SPlisHSPlasH/EmitterSystem.cpp).
+//
+// The reference keeps a fixed particle capacity (the initial particles plus
+// maxEmitterParticles) and activates slots as emitters place new layers. Here
+// the device arrays hold the active particles in cell order; a new layer is
+// appended after the last one (ids count on from the current number of
+// particles, as the reference's indices do) and the next step's sort places
+// it. Animation, state release and reuse marking are device kernels over the
+// cell-ordered arrays; the placement of a layer (at most width x height
+// particles) is decided on the host, which reads the step's time once.
+#include <cfloat>
+#include <cmath>
+
+#include "dfsph.hpp"
+
+namespace sph {
+
+namespace {
+
+__global__ __launch_bounds__(BLOCK) void k_fill_state(unsigned *st, unsigned n, unsigned v) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st[i] = v;
+}
+
+// EmitterSystem::step, first loop (EmitterSystem.cpp:60-71): particles the
+// emitters animated in the last step become active again.
+__global__ __launch_bounds__(BLOCK) void k_state_release(unsigned *st, unsigned n) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && st[i] == 1u) st[i] = 0u;
+}
+
+// EmitterSystem::reuseParticles (EmitterSystem.cpp:34-52): particles outside
+// the box are slowed down (x 0.95) and listed by id: mark[id] = cell-order
+// slot, or ~0 for a particle inside the box.
+__global__ __launch_bounds__(BLOCK) void k_reuse_mark(const float4 *__restrict__ x, float4 *__restrict__ v,
+                                                      const unsigned *__restrict__ id, unsigned *__restrict__ mark,
+                                                      unsigned n, double lx, double ly, double lz, double hx,
+                                                      double hy, double hz) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = x[i];
+    const double px = p.x, py = p.y, pz = p.z;
+    const bool out = px < lx || py < ly || pz < lz || px > hx || py > hy || pz > hz;
+    if (out) {
+        float4 w = v[i];
+        w.x *= 0.95f;
+        w.y *= 0.95f;
+        w.z *= 0.95f;
+        v[i] = w;
+    }
+    mark[id[i]] = out ? i : ~0u;
+}
+
+// The emitter's animation region, in the emitter frame around `pos`
+// (Emitter.h:35-52): a box of half extents `half`, or a cylinder along the
+// emit direction of half height half[0] and squared radius r2.
+struct AnimArgs {
+    double pos[3], R[9], half[3], r2;
+    int type;
+    float vx, vy, vz;  // emit velocity
+    float dx, dy, dz;  // displacement over one step: timeStepSize * velocity
+};
+
+// Emitter::emitParticles / emitParticlesCircle, animation loop
+// (Emitter.cpp:106-131, 254-281): particles inside the emitter move with the
+// emit velocity for one step and are marked as animated (state 1). The
+// position update carries the compensation term of the time integration.
+__global__ __launch_bounds__(BLOCK) void k_emit_animate(float4 *__restrict__ x, float4 *__restrict__ v,
+                                                        float4 *__restrict__ xc, unsigned *__restrict__ st, unsigned n,
+                                                        AnimArgs a) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = x[i];
+    const double d0 = (double)p.x - a.pos[0], d1 = (double)p.y - a.pos[1], d2 = (double)p.z - a.pos[2];
+    double loc[3];
+    for (int c = 0; c < 3; c++) loc[c] = (a.R[c] * d0 + a.R[3 + c] * d1) + a.R[6 + c] * d2;
+    bool inside;
+    if (a.type == 1) {
+        const double q = loc[1] * loc[1] + loc[2] * loc[2];
+        inside = loc[0] > -a.half[0] && loc[0] < a.half[0] && q < a.r2;
+    } else {
+        inside = fabs(loc[0]) < a.half[0] && fabs(loc[1]) < a.half[1] && fabs(loc[2]) < a.half[2];
+    }
+    if (!inside) return;
+    float4 w = v[i];
+    w.x = a.vx;
+    w.y = a.vy;
+    w.z = a.vz;
+    v[i] = w;
+    float4 c = xc[i];
+    const float yx = a.dx - c.x, yy = a.dy - c.y, yz = a.dz - c.z;
+    const float tx = p.x + yx, ty = p.y + yy, tz = p.z + yz;
+    c.x = (tx - p.x) - yx;
+    c.y = (ty - p.y) - yy;
+    c.z = (tz - p.z) - yz;
+    p.x = tx;
+    p.y = ty;
+    p.z = tz;
+    x[i] = p;
+    xc[i] = c;
+    st[i] = 1u;
+}
+
+// Places k particles: slot s = slots[q] gets position ex[q] and velocity
+// ev[q] with state `state`. A slot at or beyond n_old is a new particle: its
+// id is its slot and its stiffness and viscosity warm start are zero
+// (TimeStepDFSPH::emittedParticles, SimulationDataDFSPH.cpp:103-112). A
+// reused particle keeps its id and warm start (EmitterSystem.cpp:34-52).
+__global__ __launch_bounds__(BLOCK) void k_emit_scatter(const unsigned *__restrict__ slots,
+                                                        const float4 *__restrict__ ex, const float4 *__restrict__ ev,
+                                                        unsigned k, unsigned n_old, unsigned state, float4 *x,
+                                                        float4 *v, float4 *xc, float4 *vdiff, float *p, float *pv,
+                                                        unsigned *id, unsigned *st) {
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= k) return;
+    const unsigned s = slots[q];
+    x[s] = ex[q];
+    v[s] = ev[q];
+    xc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (st) st[s] = state;
+    if (s >= n_old) {
+        id[s] = s;
+        p[s] = 0.f;
+        pv[s] = 0.f;
+        vdiff[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- host
+void DFSPH::enable_states() {
+    if (states_) return;
+    states_ = true;
+    if (!initialized_) return;  // alloc_particles / upload_particles take it from here
+    pst_.reserve(stride_);
+    pst2_.reserve(stride_);
+    if (n_) {
+        k_fill_state<<<nblocks(n_), BLOCK, 0, stream>>>(pst_.get(), (unsigned)n_, 0u);
+        SPH_HIP(hipGetLastError());
+    }
+}
+
+void DFSPH::add_emitter(const EmitterDesc &e) {
+    SPH_REQUIRE(e.type == 0 || e.type == 1, SPH_ERR_INVALID, "emitter type must be 0 (box) or 1 (circle)");
+    SPH_REQUIRE(e.width > 0 && (e.type == 1 || e.height > 0), SPH_ERR_INVALID, "emitter width and height must be > 0");
+    SPH_REQUIRE(std::isfinite(e.velocity) && e.velocity > 0.0, SPH_ERR_INVALID, "emitter velocity must be > 0");
+    for (double c : e.R) SPH_REQUIRE(std::isfinite(c), SPH_ERR_INVALID, "emitter rotation must be finite");
+    for (double c : e.x) SPH_REQUIRE(std::isfinite(c), SPH_ERR_INVALID, "emitter position must be finite");
+    SPH_REQUIRE(!std::isnan(e.start) && !std::isnan(e.end), SPH_ERR_INVALID, "emitter times must be numbers");
+    EmitterDesc d = e;
+    d.next = e.start;  // Emitter::setEmitStartTime (Emitter.h:59)
+    d.counter = 0;
+    emitters_.push_back(d);
+    enable_states();
+}
+
+void DFSPH::set_emitter_reuse(bool enable, const double *lo, const double *hi) {
+    reuse_ = enable;
+    if (lo && hi)
+        for (int c = 0; c < 3; c++) {
+            reuse_lo_[c] = lo[c];
+            reuse_hi_[c] = hi[c];
+        }
+}
+
+void DFSPH::set_max_emitter_particles(size_t m) {
+    SPH_REQUIRE(m < (size_t)0x40000000, SPH_ERR_INVALID, "maxEmitterParticles too large");
+    if (initialized_) cap_emit_ = cap_emit_ - max_emit_ + m;
+    max_emit_ = m;
+}
+
+// Emitter::getSize (Emitter.cpp:40-87).
+void DFSPH::emitter_size(const EmitterDesc &e, double *size) const {
+    const double diam = 2.0 * r, margin = diam;
+    const bool akinci = P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012;
+    if (e.type == 0) {
+        const double m = akinci ? 2.0 : 2.5;
+        size[0] = 2.0 * hs;
+        size[1] = e.height * diam + m * margin;
+        size[2] = e.width * diam + m * margin;
+    } else {
+        const double rr = 0.5 * e.width * diam + margin;
+        size[0] = akinci ? 2.0 * hs : 2.25 * hs;
+        size[1] = size[2] = (akinci ? 2.0 : 2.25) * rr;
+    }
+}
+
+// Grows the particle arrays to hold at least m particles; the persistent
+// per-particle state of the current particles is kept, everything a step
+// recomputes is reallocated.
+void DFSPH::grow_particles(size_t m) {
+    if (m <= stride_) return;
+    SPH_HIP(hipStreamSynchronize(stream));
+    const size_t S = (m + m / 4 + 255) / 256 * 256;  // headroom for further emission
+    const size_t n = n_;
+    for (DBuf<float4> *b : {&x_, &v_, &xc_, &vdiff_}) b->grow_keep(S, n);
+    for (DBuf<float> *b : {&p_, &pv_}) b->grow_keep(S, n);
+    id_.grow_keep(S, n);
+    if (states_) {
+        pst_.grow_keep(S, n);
+        pst2_.reserve(S);
+    }
+    stride_ = (unsigned)S;
+    alloc_particles();
+    have_lists_ = false;
+}
+
+// Particles added by the caller (the activation half of Emitter.cpp:180-225
+// followed by TimeStepDFSPH::emittedParticles, TimeStepDFSPH.cpp:723-731):
+// appended in order with ids n .. n + k - 1, zero stiffness and state
+// active. The cell grid stays that of init(): particles outside it fall into
+// its border cells, which the search handles exactly, only with more
+// candidates.
+void DFSPH::emit_particles(const float *x, const float *v, size_t k) {
+    if (k == 0) return;
+    SPH_REQUIRE(x != nullptr, SPH_ERR_INVALID, "null positions");
+    SPH_REQUIRE(n_ + k < (size_t)0x7fffffff, SPH_ERR_INVALID, "too many particles");
+    if (!initialized_) {
+        hx_.insert(hx_.end(), x, x + 3 * k);
+        if (v)
+            hv_.insert(hv_.end(), v, v + 3 * k);
+        else
+            hv_.insert(hv_.end(), 3 * k, 0.f);
+        n_ += k;
+        return;
+    }
+    const size_t n = n_;
+    grow_particles(n + k);
+    std::vector<unsigned> slot(k);
+    std::vector<float4> hx(k), hv(k);
+    for (size_t q = 0; q < k; q++) {
+        slot[q] = (unsigned)(n + q);
+        hx[q] = make_float4(x[3 * q], x[3 * q + 1], x[3 * q + 2], 0.f);
+        hv[q] = v ? make_float4(v[3 * q], v[3 * q + 1], v[3 * q + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    slots_.reserve(k);
+    emit_x_.reserve(k);
+    emit_v_.reserve(k);
+    SPH_HIP(hipStreamSynchronize(stream));
+    SPH_HIP(hipMemcpy(slots_.get(), slot.data(), k * sizeof(unsigned), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(emit_x_.get(), hx.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+    SPH_HIP(hipMemcpy(emit_v_.get(), hv.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+    k_emit_scatter<<<nblocks(k), BLOCK, 0, stream>>>(slots_.get(), emit_x_.get(), emit_v_.get(), (unsigned)k,
+                                                     (unsigned)n, 0u, x_.get(), v_.get(), xc_.get(), vdiff_.get(),
+                                                     p_.get(), pv_.get(), id_.get(), states_ ? pst_.get() : nullptr);
+    SPH_HIP(hipGetLastError());
+    SPH_HIP(hipStreamSynchronize(stream));
+    n_ = n + k;
+    cap_emit_ += k;
+    have_lists_ = false;
+    tuned_ = lpp_fixed_;
+}
+
+// EmitterSystem::step (EmitterSystem.cpp:54-80) at the end of a step, before
+// the time advances (TimeStepDFSPH.cpp:242-248): release the particles
+// animated in the last step, list reusable particles, then each emitter in
+// turn animates the particles inside it and, when its next emit time has
+// come, places a layer (Emitter.cpp:89-380).
+void DFSPH::emitter_system_step() {
+    if (emitters_.empty() || !initialized_) return;
+    enable_states();
+    read_ctl();
+    const double t = hctl_->t_prev, timeStepSize = hctl_->h;
+    const double diam = 2.0 * r;
+    if (n_) {
+        k_state_release<<<nblocks(n_), BLOCK, 0, stream>>>(pst_.get(), (unsigned)n_);
+        SPH_HIP(hipGetLastError());
+    }
+    std::vector<unsigned> reuse;  // cell-order slots of reusable particles, by id
+    if (reuse_ && n_) {
+        mark_.reserve(stride_);
+        k_reuse_mark<<<nblocks(n_), BLOCK, 0, stream>>>(x_.get(), v_.get(), id_.get(), mark_.get(), (unsigned)n_,
+                                                        reuse_lo_[0], reuse_lo_[1], reuse_lo_[2], reuse_hi_[0],
+                                                        reuse_hi_[1], reuse_hi_[2]);
+        SPH_HIP(hipGetLastError());
+        std::vector<unsigned> mark(n_);
+        SPH_HIP(hipMemcpyAsync(mark.data(), mark_.get(), n_ * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+        SPH_HIP(hipStreamSynchronize(stream));
+        for (unsigned s : mark)
+            if (s != ~0u) reuse.push_back(s);
+    }
+    size_t indexReuse = 0;
+    for (EmitterDesc &e : emitters_) {
+        const double dir[3] = {e.R[0], e.R[3], e.R[6]};
+        const double axisHeight[3] = {e.R[1], e.R[4], e.R[7]}, axisWidth[3] = {e.R[2], e.R[5], e.R[8]};
+        double vel[3];
+        for (int c = 0; c < 3; c++) vel[c] = e.velocity * dir[c];
+        if (t < e.start || t > e.end)  // clear the emitter shortly before it starts
+            for (int c = 0; c < 3; c++) vel[c] = dir[c] * r * 10.0 / 0.25;
+        if (t >= e.start - 0.25 && t <= e.end && n_) {
+            double size[3];
+            emitter_size(e, size);
+            AnimArgs a{};
+            for (int c = 0; c < 3; c++) {
+                a.pos[c] = e.x[c] + 0.5 * hs * dir[c];
+                a.half[c] = 0.5 * size[c];
+            }
+            for (int c = 0; c < 9; c++) a.R[c] = e.R[c];
+            a.r2 = a.half[1] * a.half[1];
+            a.type = e.type;
+            a.vx = (float)vel[0], a.vy = (float)vel[1], a.vz = (float)vel[2];
+            a.dx = (float)(timeStepSize * vel[0]);
+            a.dy = (float)(timeStepSize * vel[1]);
+            a.dz = (float)(timeStepSize * vel[2]);
+            k_emit_animate<<<nblocks(n_), BLOCK, 0, stream>>>(x_.get(), v_.get(), xc_.get(), pst_.get(),
+                                                               (unsigned)n_, a);
+            SPH_HIP(hipGetLastError());
+        }
+        if (t < e.next || t > e.end) continue;
+        const unsigned rows = e.type == 1 ? e.width : e.height;
+        const double startX = -0.5 * (e.width - 1.0) * diam, startZ = -0.5 * (rows - 1.0) * diam;
+        const double radius = 0.5 * e.width * diam, radius2 = radius * radius;
+        const double tt = t - e.next + timeStepSize;
+        double offset[3];
+        for (int c = 0; c < 3; c++) offset[c] = e.x[c] + tt * vel[c];
+        const size_t n0 = n_;
+        std::vector<unsigned> slot;
+        std::vector<float4> hx, hv;
+        size_t fresh = 0;
+        if (n0 < cap_emit_ || !reuse.empty()) {
+            for (unsigned i = 0; i < e.width; i++)
+                for (unsigned j = 0; j < rows; j++) {
+                    const double a = i * diam + startX, b = j * diam + startZ;
+                    if (e.type == 1 && !(a * a + b * b <= radius2)) continue;
+                    const bool fromReuse = indexReuse < reuse.size();
+                    const size_t s = fromReuse ? reuse[indexReuse] : n0 + fresh;
+                    if (!fromReuse && s >= cap_emit_) continue;
+                    double p[3];
+                    for (int c = 0; c < 3; c++) p[c] = a * axisWidth[c] + b * axisHeight[c] + offset[c];
+                    slot.push_back((unsigned)s);
+                    hx.push_back(make_float4((float)p[0], (float)p[1], (float)p[2], 0.f));
+                    hv.push_back(make_float4((float)vel[0], (float)vel[1], (float)vel[2], 0.f));
+                    if (fromReuse)
+                        indexReuse++;
+                    else
+                        fresh++;
+                }
+        }
+        if (!slot.empty()) {
+            const size_t k = slot.size();
+            grow_particles(n0 + fresh);
+            slots_.reserve(k);
+            emit_x_.reserve(k);
+            emit_v_.reserve(k);
+            SPH_HIP(hipStreamSynchronize(stream));
+            SPH_HIP(hipMemcpy(slots_.get(), slot.data(), k * sizeof(unsigned), hipMemcpyHostToDevice));
+            SPH_HIP(hipMemcpy(emit_x_.get(), hx.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+            SPH_HIP(hipMemcpy(emit_v_.get(), hv.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+            k_emit_scatter<<<nblocks(k), BLOCK, 0, stream>>>(slots_.get(), emit_x_.get(), emit_v_.get(), (unsigned)k,
+                                                             (unsigned)n0, 1u, x_.get(), v_.get(), xc_.get(),
+                                                             vdiff_.get(), p_.get(), pv_.get(), id_.get(), pst_.get());
+            SPH_HIP(hipGetLastError());
+            n_ = n0 + fresh;
+            emitted_ += fresh;
+            if (fresh) have_lists_ = false;
+        }
+        e.next += diam / e.velocity;
+        e.counter++;
+    }
+    reused_ += indexReuse;
+}
+
+}  // namespace sph

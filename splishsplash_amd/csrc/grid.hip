@@ -246,12 +246,18 @@ __global__ __launch_bounds__(BLOCK) void k_permute(const unsigned *__restrict__ 
 // candidates stores its entries directly as it finds them. The order of a
 // list is row by row, ascending slot within a row.
 constexpr unsigned LIST_ROWS = 9;
+// Inclusion is decided exactly for the FP32 coordinates: a squared distance
+// within FP32 rounding of r^2 (between r2lo and r2hi) is recomputed in FP64
+// and compared with the FP64 r^2, so a pair at exactly the support radius
+// (the lattice of an emitter layer has them by construction) is classified
+// as in the FP64 search.
 template <bool SAME>
 __global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q, unsigned nq,
                                                     const float4 *__restrict__ src,
-                                                    const unsigned *__restrict__ start, GridDesc g, float r2,
-                                                    unsigned *__restrict__ nbr, unsigned stride, unsigned cap,
-                                                    unsigned *__restrict__ cnt, unsigned *__restrict__ maxcnt) {
+                                                    const unsigned *__restrict__ start, GridDesc g, float r2lo,
+                                                    float r2hi, double r2d, unsigned *__restrict__ nbr,
+                                                    unsigned stride, unsigned cap, unsigned *__restrict__ cnt,
+                                                    unsigned *__restrict__ maxcnt) {
     __shared__ unsigned rowbase[BLOCK / 64][LIST_ROWS][64];
     __shared__ unsigned long long masks[BLOCK / 64][LIST_ROWS][64];
     const unsigned lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -283,7 +289,13 @@ __global__ __launch_bounds__(BLOCK) void k_lists_cm(const float4 *__restrict__ q
             auto test = [&](unsigned j, const float4 &xj) {
                 const float dx = xi.x - xj.x, dy2 = xi.y - xj.y, dz2 = xi.z - xj.z;
                 const float d2 = (dx * dx + dy2 * dy2) + dz2 * dz2;
-                if (d2 < r2 && !(SAME && j == i)) {
+                bool in = d2 < r2hi && !(SAME && j == i);
+                if (in && !(d2 < r2lo)) {
+                    const double ex = (double)xi.x - (double)xj.x, ey = (double)xi.y - (double)xj.y,
+                                 ez = (double)xi.z - (double)xj.z;
+                    in = (ex * ex + ey * ey) + ez * ez < r2d;
+                }
+                if (in) {
                     if (wide) {
                         if (k < cap) nbr[(size_t)k * stride + i] = j;
                     } else {
@@ -448,15 +460,21 @@ void permute_u1(hipStream_t s, const unsigned *perm, unsigned n, const unsigned 
 }
 
 void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *start,
-                          const GridDesc &g, float r2, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
+                          const GridDesc &g, double radius, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
                           unsigned *cnt, unsigned *maxcnt, KernelTimer &T, const char *name) {
     if (!nq) return;
     SPH_REQUIRE(cap >= 1, SPH_ERR_INVALID, "neighbour capacity out of range");
     TimedScope ts(T, s, name);
+    const double r2d = radius * radius;
+    // FP32 evaluation error of d^2 stays below 4 ulp (exact differences of
+    // nearby coordinates, three rounded products, two rounded sums)
+    const float r2lo = (float)(r2d * (1.0 - 1e-6)), r2hi = (float)(r2d * (1.0 + 1e-6));
     if (same_set)
-        k_lists_cm<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<true><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2lo, r2hi, r2d, nbr, stride, cap, cnt,
+                                                       maxcnt);
     else
-        k_lists_cm<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2, nbr, stride, cap, cnt, maxcnt);
+        k_lists_cm<false><<<nblocks(nq), BLOCK, 0, s>>>(q, nq, src, start, g, r2lo, r2hi, r2d, nbr, stride, cap, cnt,
+                                                        maxcnt);
     SPH_HIP(hipGetLastError());
 }
 

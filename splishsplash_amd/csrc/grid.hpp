@@ -62,9 +62,10 @@ void fill_u32(hipStream_t s, unsigned *p, size_t n, unsigned v);
 // wavefront of consecutive queries reads 64 consecutive words per k), the
 // source slot as value; cnt[i] = number found (may exceed cap: then the
 // entries beyond cap are dropped and *maxcnt reports the true maximum).
-// same_set excludes slot == i.
+// same_set excludes slot == i. A pair is in when its distance is below
+// `radius`, decided in FP64 when the FP32 distance is within rounding of it.
 void build_lists_colmajor(hipStream_t s, const float4 *q, unsigned nq, const float4 *src, const unsigned *start,
-                          const GridDesc &g, float r2, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
+                          const GridDesc &g, double radius, bool same_set, unsigned *nbr, unsigned stride, unsigned cap,
                           unsigned *cnt, unsigned *maxcnt, KernelTimer &T, const char *name);
 
 // CSR variant for the drop-in search: queries in caller order, sources

@@ -432,10 +432,14 @@ __device__ double interp_sdf(const VMapDesc &d, const double *y) {
 
 __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,
                                                      float4 *__restrict__ v, float4 *__restrict__ xc,
-                                                     float4 *__restrict__ vmb, double supportRadius,
-                                                     double particleRadius) {
+                                                     float4 *__restrict__ vmb, const unsigned *__restrict__ state,
+                                                     double supportRadius, double particleRadius) {
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (state && state[i] != 0u) {  // only active particles see the boundary (TimeStep.cpp:268)
+        vmb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
     const float4 xf = x[i];
     const double xi[3] = {(double)xf.x, (double)xf.y, (double)xf.z};
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -517,10 +521,10 @@ __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, flo
 }
 
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
-                       double supportRadius, double particleRadius, KernelTimer &T) {
+                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T) {
     if (!n) return;
     TimedScope ts(T, s, "volume_map_sample");
-    k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, supportRadius, particleRadius);
+    k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, state, supportRadius, particleRadius);
     SPH_HIP(hipGetLastError());
 }
 

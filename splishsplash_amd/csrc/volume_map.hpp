@@ -74,8 +74,9 @@ public:
 // (TimeStep.cpp:223-346) for one static shape: writes vmb[i] = (x_b, V_b),
 // projects particles with sdf <= 0 back towards the surface and zeroes their
 // velocity; xc (position compensation) is cleared for moved particles.
+// Particles whose state (if given) is not 0 get an empty record.
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
-                       double supportRadius, double particleRadius, KernelTimer &T);
+                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T);
 
 // Packs, per particle, the records with V_b > 0 of `nobj` object-major
 // arrays (raw[o * stride + i]) into the leading slots of vmb (same layout),

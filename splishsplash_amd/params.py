@@ -85,6 +85,26 @@ class DFSPHCounters(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class Emitter(C.Structure):
+    """sph_emitter_t."""
+    _fields_ = [
+        ("type", C.c_int),
+        ("width", C.c_uint),
+        ("height", C.c_uint),
+        ("translation", C.c_double * 3),
+        ("rotation", C.c_double * 9),
+        ("velocity", C.c_double),
+        ("emit_start_time", C.c_double),
+        ("emit_end_time", C.c_double),
+    ]
+
+
+# ParticleState (FluidModel.h:75)
+STATE_ACTIVE = 0
+STATE_ANIMATED_BY_EMITTER = 1
+STATE_FIXED = 2
+
+
 def default_params(**kw):
     """Reference constructor defaults (Simulation.cpp:66-89, boundary
     handling Bender2019 at Simulation.cpp:88,
@@ -126,5 +146,5 @@ FIELD_COMPONENTS = {
     "factor": 1, "advected density": 1, "p / rho^2": 1, "p_v / rho^2": 1,
     "pressure acceleration": 3, "velocity difference": 3,
     "boundary volume": 1, "boundary x": 3,
-    "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1,
+    "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1, "state": 1,
 }

@@ -1,8 +1,12 @@
 """Neighbour search on the device against the oracle (bit-exact sets).
 
-The oracle evaluates the same FP32 inclusion test (strict d^2 < r^2 with
-((dx dx + dy dy) + dz dz), self excluded) on the same FP32 positions, by grid
-and by brute force (which agree, tests/test_oracle_ns.py)."""
+The drop-in search evaluates the FP32 inclusion test of CompactNSearch's
+default (float) build, strict d^2 < r^2 with ((dx dx + dy dy) + dz dz), self
+excluded; the oracle evaluates the same test on the same FP32 positions, by
+grid and by brute force (which agree, tests/test_oracle_ns.py). The time
+step's own lists decide a pair within FP32 rounding of r^2 in FP64, as the
+FP64 time step of the oracle does: they are compared with the oracle's FP64
+search on the same (FP32) positions."""
 import numpy as np
 import pytest
 
@@ -110,11 +114,14 @@ def test_dfsph_internal_lists_match_oracle():
     g.stage("neighborhood_search")
     x32 = np.ascontiguousarray(sc["x"], np.float32)
     off_g, idx_g = g.neighbors(boundary=False)
-    off_o, idx_o = oracle.neighbors(x32, radius=np.float32(0.1))
+    x64 = x32.astype(np.float64)
+    hs = 4.0 * sc["params"].particle_radius
+    off_o, idx_o = oracle.neighbors(x64, radius=hs, dtype=np.float64)
     assert_same_csr(off_g, idx_g, off_o, idx_o)
     fluid_counts = np.diff(off_o)
     off_g, idx_g = g.neighbors(boundary=True)
-    off_o, idx_o = oracle.neighbors(x32, np.ascontiguousarray(sc["boundary_x"], np.float32), radius=np.float32(0.1))
+    b64 = np.ascontiguousarray(sc["boundary_x"], np.float32).astype(np.float64)
+    off_o, idx_o = oracle.neighbors(x64, b64, radius=hs, dtype=np.float64)
     assert_same_csr(off_g, idx_g, off_o, idx_o)
     cnt = g.get("fluid neighbor count")
     np.testing.assert_array_equal(cnt, fluid_counts)
@@ -145,7 +152,7 @@ def test_dfsph_lists_beyond_the_lds_tile():
     g.stage("neighborhood_search")
     assert g.counters()["max_fluid_neighbors"] > 300
     off_g, idx_g = g.neighbors(boundary=False)
-    off_o, idx_o = oracle.neighbors(x, radius=np.float32(h))
+    off_o, idx_o = oracle.neighbors(x.astype(np.float64), radius=4.0 * prm.particle_radius, dtype=np.float64)
     assert_same_csr(off_g, idx_g, off_o, idx_o)
     g.stage("densities")
     o = oracle.OracleDFSPH(prm)
