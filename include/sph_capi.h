@@ -453,11 +453,16 @@ typedef struct sph_scene *sph_scene_t;
  * "RigidBodies" from OBJ meshes (Bender2019: one volume map each with
  * mapInvert / mapThickness / mapResolution, analytic for an axis-aligned box
  * or a sphere, a mesh map otherwise; Akinci2012: box and sphere surfaces
- * sampled at 2r). Other keys that
- * do not change the simulation are ignored; anything that would (dynamic
- * bodies, emitters, fluid models from files, other methods, several fluid
- * ids) fails with SPH_ERR_INVALID naming the key. Relative geometry paths
- * are resolved against the scene file's folder. */
+ * sampled at 2r), and "Emitters" (SimulatorBase::createEmitters,
+ * SimulatorBase.cpp:1286-1384: width, height, translation, rotationAxis,
+ * rotationAngle, velocity, type, emitStartTime, emitEndTime; the fluid
+ * material's maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max;
+ * with Bender2019 each emitter also gets its housing as a static mesh body
+ * after the scene's bodies). Other keys that do not change the simulation
+ * are ignored; anything that would (dynamic bodies, fluid models from
+ * files, other methods, several fluid ids) fails with SPH_ERR_INVALID
+ * naming the key. Relative geometry paths are resolved against the scene
+ * file's folder. */
 int sph_scene_load(const char *path, sph_scene_t *out);
 int sph_scene_destroy(sph_scene_t s);
 int sph_scene_params(sph_scene_t s, sph_dfsph_params_t *params);
@@ -472,7 +477,17 @@ int sph_scene_particles(sph_scene_t s, float *x, float *v, float *boundary_x);
 int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3],
                    double centre[3], double *radius, int *invert,
                    double *thickness, unsigned resolution[3]);
-/* A DFSPH handle with the scene's parameters, fluid and boundaries. */
+/* The world-space triangles of a mesh body (kind 2): counts first (null
+ * arrays), then 3 doubles per vertex and 3 indices per face. */
+int sph_scene_body_mesh(sph_scene_t s, int k, double *vertices,
+                        size_t *n_vertices, unsigned *faces, size_t *n_faces);
+/* Emitter k of the scene, and the fluid's emitter settings. */
+int sph_scene_emitter(sph_scene_t s, int k, sph_emitter_t *out);
+int sph_scene_emitter_settings(sph_scene_t s, int *n_emitters,
+                               size_t *max_emitter_particles, int *reuse,
+                               double box_min[3], double box_max[3]);
+/* A DFSPH handle with the scene's parameters, fluid, boundaries and
+ * emitters. */
 int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out);
 
 /* ------------------------------------------------------------------------ */

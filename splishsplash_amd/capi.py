@@ -84,6 +84,9 @@ _SIGS = {
     "sph_scene_counts": (C.c_int, [_vp, _szp, _szp, C.POINTER(C.c_int)]),
     "sph_scene_particles": (C.c_int, [_vp, _fp, _fp, _fp]),
     "sph_scene_body": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, C.POINTER(C.c_int), _dp, _up]),
+    "sph_scene_body_mesh": (C.c_int, [_vp, C.c_int, _dp, _szp, _up, _szp]),
+    "sph_scene_emitter": (C.c_int, [_vp, C.c_int, C.c_void_p]),
+    "sph_scene_emitter_settings": (C.c_int, [_vp, C.POINTER(C.c_int), _szp, C.POINTER(C.c_int), _dp, _dp]),
     "sph_scene_create_dfsph": (C.c_int, [_vp, C.POINTER(_vp)]),
     "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_emitter": (C.c_int, [_vp, C.c_void_p]),
@@ -346,7 +349,7 @@ class NeighborhoodSearch:
 
 class Scene:
     """A SPlisHSPlasH scene file read by the library (sph_scene_load):
-    parameters, sampled fluid blocks, static boxes and spheres."""
+    parameters, sampled fluid blocks, static bodies and emitters."""
 
     def __init__(self, path):
         h = _vp()
@@ -381,6 +384,31 @@ class Scene:
                                   res.ctypes.data_as(_up)))
         return {"kind": ("box", "sphere", "mesh")[kind.value], "lo": lo, "hi": hi, "centre": cen,
                 "radius": rad.value, "invert": bool(inv.value), "thickness": th.value, "resolution": res}
+
+    def body_mesh(self, k):
+        """(vertices (m, 3), faces (f, 3)) of mesh body k, world coordinates."""
+        nv, nf = C.c_size_t(), C.c_size_t()
+        _chk(lib().sph_scene_body_mesh(self.h, k, None, C.byref(nv), None, C.byref(nf)))
+        v = np.zeros((nv.value, 3))
+        f = np.zeros((nf.value, 3), np.uint32)
+        _chk(lib().sph_scene_body_mesh(self.h, k, v.ctypes.data_as(_dp), C.byref(nv), f.ctypes.data_as(_up),
+                                       C.byref(nf)))
+        return v, f
+
+    def emitters(self):
+        """The scene's emitters as dicts, and the fluid's emitter settings."""
+        n, me, ru = C.c_int(), C.c_size_t(), C.c_int()
+        lo, hi = np.zeros(3), np.zeros(3)
+        _chk(lib().sph_scene_emitter_settings(self.h, C.byref(n), C.byref(me), C.byref(ru), lo.ctypes.data_as(_dp),
+                                              hi.ctypes.data_as(_dp)))
+        out = []
+        for k in range(n.value):
+            e = Emitter()
+            _chk(lib().sph_scene_emitter(self.h, k, C.byref(e)))
+            out.append({"type": e.type, "width": e.width, "height": e.height, "translation": np.array(e.translation[:]),
+                        "rotation": np.array(e.rotation[:]).reshape(3, 3), "velocity": e.velocity,
+                        "emit_start_time": e.emit_start_time, "emit_end_time": e.emit_end_time})
+        return out, {"max_emitter_particles": me.value, "reuse": bool(ru.value), "box_min": lo, "box_max": hi}
 
     def create_dfsph(self):
         """A TimeStepDFSPH with the scene's parameters, fluid and boundaries."""

@@ -4,6 +4,7 @@
 // samples its fluid blocks as SimulatorBase::createFluidBlocks does
 // (Simulator/SimulatorBase.cpp:1418-1516), turns its static boxes and
 // spheres into the handle's boundaries and creates a DFSPH handle.
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -217,6 +218,12 @@ struct Scene {
     sph_dfsph_params_t params{};
     std::vector<float> x, v, bx;  // fluid positions, velocities, Akinci particles
     std::vector<SceneBody> bodies;
+    // emitters and the fluid's emitter settings (FluidParameterObject
+    // maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max)
+    std::vector<sph_emitter_t> emitters;
+    size_t max_emit = 10000;
+    bool reuse = false;
+    double reuse_lo[3] = {-1, -1, -1}, reuse_hi[3] = {1, 1, 1};
 };
 
 // SimulatorBase::createFluidBlocks (SimulatorBase.cpp:1418-1516) for one
@@ -318,6 +325,80 @@ static std::vector<double> obj_vertices(const std::string &path, std::vector<uns
     return out;
 }
 
+// Rotation about `axis` (normalised) by `angle`, row-major (Rodrigues; the
+// AngleAxis rotation of the reference's scene objects).
+static void axis_angle(const double axis[3], double angle, double R[9]) {
+    double n[3] = {axis[0], axis[1], axis[2]};
+    const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nl > 0.0)
+        for (double &c : n) c /= nl;
+    const double cs = std::cos(angle), sn = std::sin(angle), t = 1.0 - cs;
+    const double M[9] = {t * n[0] * n[0] + cs,        t * n[0] * n[1] - sn * n[2], t * n[0] * n[2] + sn * n[1],
+                         t * n[0] * n[1] + sn * n[2], t * n[1] * n[1] + cs,        t * n[1] * n[2] - sn * n[0],
+                         t * n[0] * n[2] - sn * n[1], t * n[1] * n[2] + sn * n[0], t * n[2] * n[2] + cs};
+    std::copy(M, M + 9, R);
+}
+
+// The housing the reference puts around an emitter (SimulatorBase.cpp:
+// 1325-1350: a static body from emitter_boundary/EmitterBox.obj or
+// EmitterCylinder.obj scaled by Emitter::getSize, turned and placed with the
+// emitter, mapInvert false, mapResolution 20^3): here a thin-walled cup of
+// its own design in unit coordinates, open along +x around the unit region
+// [-0.5, 0.5]^3 (box) or the unit cylinder along x (circle), walls 0.05
+// thick. Triangles are oriented with the normal out of the solid wall.
+static void housing_mesh(int type, std::vector<double> &V, std::vector<unsigned> &F) {
+    auto vtx = [&](double x, double y, double z) {
+        V.insert(V.end(), {x, y, z});
+        return (unsigned)(V.size() / 3 - 1);
+    };
+    auto tri = [&](unsigned a, unsigned b, unsigned c, const double hint[3]) {
+        const double *A = &V[3 * a], *B = &V[3 * b], *C = &V[3 * c];
+        const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        const double nx = u[1] * w[2] - u[2] * w[1], ny = u[2] * w[0] - u[0] * w[2], nz = u[0] * w[1] - u[1] * w[0];
+        if (nx * hint[0] + ny * hint[1] + nz * hint[2] >= 0.0)
+            F.insert(F.end(), {a, b, c});
+        else
+            F.insert(F.end(), {a, c, b});
+    };
+    auto quad = [&](unsigned a, unsigned b, unsigned c, unsigned d, const double hint[3]) {
+        tri(a, b, c, hint);
+        tri(a, c, d, hint);
+    };
+    const double in = 0.5, out = 0.55, back = -0.55, front = 0.5, ib = -0.5;
+    const int N = type == 1 ? 32 : 4;
+    // ring k of a wall: the square (box) or the N-gon (cylinder) of radius r at x
+    auto ring = [&](double x, double r) {
+        std::vector<unsigned> idx;
+        for (int k = 0; k < N; k++) {
+            if (type == 1) {
+                const double th = 2.0 * M_PI * k / N;
+                idx.push_back(vtx(x, r * std::cos(th), r * std::sin(th)));
+            } else {
+                static const double sq[4][2] = {{1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+                idx.push_back(vtx(x, r * sq[k][0], r * sq[k][1]));
+            }
+        }
+        return idx;
+    };
+    const std::vector<unsigned> ob = ring(back, out), of = ring(front, out), iff = ring(front, in), ibk = ring(ib, in);
+    const double px[3] = {1, 0, 0}, mx[3] = {-1, 0, 0};
+    for (int k = 0; k < N; k++) {
+        const int k1 = (k + 1) % N;
+        const double *A = &V[3 * of[k]], *B = &V[3 * of[k1]];
+        double radial[3] = {0.0, A[1] + B[1], A[2] + B[2]};  // outward from the axis at this segment
+        const double inward[3] = {0.0, -radial[1], -radial[2]};
+        quad(ob[k], of[k], of[k1], ob[k1], radial);     // outer side
+        quad(iff[k], ibk[k], ibk[k1], iff[k1], inward);  // inner side, facing the cavity
+        quad(of[k], iff[k], iff[k1], of[k1], px);        // rim
+    }
+    const unsigned cb = vtx(back, 0.0, 0.0), ci = vtx(ib, 0.0, 0.0);
+    for (int k = 0; k < N; k++) {
+        const int k1 = (k + 1) % N;
+        tri(cb, ob[k], ob[k1], mx);   // outer back cap
+        tri(ci, ibk[k], ibk[k1], px);  // cavity floor, facing the cavity
+    }
+}
+
 // A static rigid body of the scene (StaticBoundarySimulator.cpp:42-160:
 // mesh scaled, rotated about `rotationAxis` by `rotationAngle`, translated)
 // recognised as an axis-aligned box (every vertex on a corner of the
@@ -325,15 +406,8 @@ static std::vector<double> obj_vertices(const std::string &path, std::vector<uns
 // centre), the shapes whose distance fields the volume maps build exactly.
 static SceneBody body_from_mesh(const std::vector<double> &V, const std::vector<unsigned> &faces, const double sc[3],
                                 const double tr[3], const double axis[3], double angle, const std::string &name) {
-    // rotation matrix (Rodrigues)
-    double n[3] = {axis[0], axis[1], axis[2]};
-    const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (nl > 0.0)
-        for (double &c : n) c /= nl;
-    const double cs = std::cos(angle), sn = std::sin(angle), t = 1.0 - cs;
-    const double R[9] = {t * n[0] * n[0] + cs,        t * n[0] * n[1] - sn * n[2], t * n[0] * n[2] + sn * n[1],
-                         t * n[0] * n[1] + sn * n[2], t * n[1] * n[1] + cs,        t * n[1] * n[2] - sn * n[0],
-                         t * n[0] * n[2] - sn * n[1], t * n[1] * n[2] + sn * n[0], t * n[2] * n[2] + cs};
+    double R[9];
+    axis_angle(axis, angle, R);
     const size_t m = V.size() / 3;
     std::vector<double> W(V.size());
     SceneBody b;
@@ -475,7 +549,7 @@ static Scene *read_scene(const std::string &path) {
     }
     const double r = P.particle_radius;
 
-    for (const char *k : {"FluidModels", "Emitters", "AnimationFields"}) {
+    for (const char *k : {"FluidModels", "AnimationFields"}) {
         const JValue *v = root.find(k);
         if (v && v->kind == JValue::ARR && !v->arr.empty()) unsupported(std::string("\"") + k + "\"");
     }
@@ -530,6 +604,56 @@ static Scene *read_scene(const std::string &path) {
                 P.xsph = num(*x, "xsph", 0.0);
                 P.xsph_boundary = num(*x, "xsphBoundary", 0.0);
             }
+            const double me = num(m, "maxEmitterParticles", 10000.0);
+            if (!(me >= 0.0 && me < 1e9)) unsupported("\"maxEmitterParticles\" out of range");
+            S->max_emit = (size_t)me;
+            S->reuse = flag(m, "emitterReuseParticles", false);
+            vec3(m, "emitterBoxMin", S->reuse_lo);
+            vec3(m, "emitterBoxMax", S->reuse_hi);
+        }
+    }
+
+    // Emitters (SimulatorBase::createEmitters, SimulatorBase.cpp:1286-1384)
+    // with the housing each one gets as a static body
+    std::vector<SceneBody> housings;
+    if (const JValue *ems = root.find("Emitters")) {
+        for (const JValue &e : ems->arr) {
+            if (text(e, "id", "Fluid") != fluid_id)
+                unsupported("an emitter of another fluid id (\"" + text(e, "id", "Fluid") + "\")");
+            sph_emitter_t d{};
+            d.width = (unsigned)num(e, "width", 5);
+            d.height = (unsigned)num(e, "height", 5);
+            d.type = (int)num(e, "type", 0);
+            d.velocity = num(e, "velocity", 1.0);
+            d.emit_start_time = num(e, "emitStartTime", 0.0);
+            d.emit_end_time = num(e, "emitEndTime", DBL_MAX);
+            vec3(e, "translation", d.translation);
+            double ax[3] = {0, 0, 1};
+            vec3(e, "rotationAxis", ax);
+            const double angle = num(e, "rotationAngle", 0.0);
+            axis_angle(ax, angle, d.rotation);
+            S->emitters.push_back(d);
+            // housing: Emitter::getSize scale (Emitter.cpp:40-87)
+            const double diam = 2.0 * r, hs = 4.0 * r;
+            const bool akinci = P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012;
+            double sc[3];
+            if (d.type == 0) {
+                const double m = akinci ? 2.0 : 2.5;
+                sc[0] = 2.0 * hs;
+                sc[1] = d.height * diam + m * diam;
+                sc[2] = d.width * diam + m * diam;
+            } else {
+                const double rr = 0.5 * d.width * diam + diam;
+                sc[0] = akinci ? 2.0 * hs : 2.25 * hs;
+                sc[1] = sc[2] = (akinci ? 2.0 : 2.25) * rr;
+            }
+            std::vector<double> hv;
+            std::vector<unsigned> hf;
+            housing_mesh(d.type, hv, hf);
+            SceneBody hb = body_from_mesh(hv, hf, sc, d.translation, ax, angle, "emitter housing");
+            hb.invert = false;
+            hb.thickness = 0.0;
+            housings.push_back(hb);
         }
     }
 
@@ -563,6 +687,11 @@ static Scene *read_scene(const std::string &path) {
             S->bodies.push_back(body);
         }
     }
+    // emitter housings after the scene's bodies, as the reference appends
+    // them to the boundary models; Akinci2012 has no mesh sampling here, so
+    // its emitters run without a housing
+    if (P.boundary_handling_method == SPH_BOUNDARY_BENDER2019)
+        S->bodies.insert(S->bodies.end(), housings.begin(), housings.end());
     return S.release();
 }
 
@@ -617,6 +746,40 @@ int sph_scene_particles(sph_scene_t s, float *x, float *v, float *boundary_x) {
     });
 }
 
+int sph_scene_emitter(sph_scene_t s, int k, sph_emitter_t *out) {
+    return guarded([&] {
+        SPH_REQUIRE(s && out && k >= 0 && k < (int)s->s->emitters.size(), SPH_ERR_INVALID, "emitter index out of range");
+        *out = s->s->emitters[(size_t)k];
+    });
+}
+
+int sph_scene_emitter_settings(sph_scene_t s, int *n_emitters, size_t *max_emitter_particles, int *reuse,
+                               double box_min[3], double box_max[3]) {
+    return guarded([&] {
+        SPH_REQUIRE(s, SPH_ERR_INVALID, "null scene");
+        const sph::Scene &S = *s->s;
+        if (n_emitters) *n_emitters = (int)S.emitters.size();
+        if (max_emitter_particles) *max_emitter_particles = S.max_emit;
+        if (reuse) *reuse = S.reuse ? 1 : 0;
+        for (int c = 0; c < 3; c++) {
+            if (box_min) box_min[c] = S.reuse_lo[c];
+            if (box_max) box_max[c] = S.reuse_hi[c];
+        }
+    });
+}
+
+int sph_scene_body_mesh(sph_scene_t s, int k, double *vertices, size_t *n_vertices, unsigned *faces,
+                        size_t *n_faces) {
+    return guarded([&] {
+        SPH_REQUIRE(s && k >= 0 && k < (int)s->s->bodies.size(), SPH_ERR_INVALID, "body index out of range");
+        const sph::SceneBody &b = s->s->bodies[(size_t)k];
+        if (n_vertices) *n_vertices = b.verts.size() / 3;
+        if (n_faces) *n_faces = b.faces.size() / 3;
+        if (vertices) std::copy(b.verts.begin(), b.verts.end(), vertices);
+        if (faces) std::copy(b.faces.begin(), b.faces.end(), faces);
+    });
+}
+
 int sph_scene_body(sph_scene_t s, int k, int *kind, double lo[3], double hi[3], double centre[3], double *radius,
                    int *invert, double *thickness, unsigned resolution[3]) {
     return guarded([&] {
@@ -663,6 +826,11 @@ int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out) {
                     chk(sph_dfsph_add_volume_map_mesh(h, b.verts.data(), b.verts.size() / 3, b.faces.data(),
                                                       b.faces.size() / 3, b.invert ? 1 : 0, b.thickness, b.res));
             }
+        }
+        if (!S.emitters.empty()) {
+            chk(sph_dfsph_set_max_emitter_particles(h, S.max_emit));
+            chk(sph_dfsph_set_emitter_reuse(h, S.reuse ? 1 : 0, S.reuse_lo, S.reuse_hi));
+            for (const sph_emitter_t &e : S.emitters) chk(sph_dfsph_add_emitter(h, &e));
         }
         *out = h;
     });

@@ -91,3 +91,46 @@ def test_scene_with_a_mesh_obstacle_runs(tmp_path):
         assert c["avg_density_err"] <= eta or c["iterations"] == s.params.max_iterations
     x = sim.get("position")
     assert c["flags"] == 0 and np.isfinite(x).all()
+
+
+def test_emitter_scene_matches_oracle(tmp_path):
+    # the shape of the reference's Emitter.json (wall box, box emitter,
+    # circle emitter with its start time, housings as mesh maps, reuse box,
+    # maxEmitterParticles): the device and the oracle built from the scene's
+    # description step together from identical states
+    from test_scene import emitter_scene, write_scene
+    scene = emitter_scene(2)
+    scene["Emitters"][1]["emitStartTime"] = 0.05
+    scene["Configuration"]["cflMethod"] = 0
+    sc = capi.Scene(write_scene(tmp_path, scene))
+    g = sc.create_dfsph()
+    x, v, _ = sc.particles()
+    o = oracle.OracleDFSPH(sc.params)
+    o.set_fluid(x.astype(np.float64), v.astype(np.float64))
+    for k in range(sc.nbodies):
+        d = sc.body(k)
+        if d["kind"] == "box":
+            o.set_volume_map_box(d["lo"], d["hi"], d["resolution"], invert=d["invert"], thickness=d["thickness"])
+        else:
+            verts, faces = sc.body_mesh(k)
+            o.add_volume_map_mesh(verts, faces, d["resolution"], invert=d["invert"], thickness=d["thickness"])
+    ems, st = sc.emitters()
+    o.set_max_emitter_particles(st["max_emitter_particles"])
+    o.enable_reuse_particles(st["box_min"], st["box_max"])
+    for e in ems:
+        o.add_emitter(e["width"], e["height"], e["translation"], e["rotation"], e["velocity"], type=e["type"],
+                      emit_start_time=e["emit_start_time"], emit_end_time=e["emit_end_time"])
+    g.init()
+    o.init()
+    for k in range(80):
+        for nm in ("position", "velocity", "p / rho^2", "p_v / rho^2"):
+            o.set(nm, g.get(nm).astype(np.float64))
+        o.set("state", g.get("state").astype(np.float64))
+        g.step(1)
+        o.step(1)
+        assert g.n == o.n and g.emitter_counts() == o.emitter_counts(), k
+        np.testing.assert_array_equal(g.get("state"), o.get("state").astype(np.uint32))
+        for nm in ("position", "velocity"):
+            assert rel(g.get(nm), o.get(nm)) <= 1e-4, (k, nm)
+    e, _ = g.emitter_counts()
+    assert e >= 24 * 3 + 37  # the box emitter's layers and the circle emitter's first disc

@@ -143,7 +143,7 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
 
 @pytest.mark.parametrize("change, key", [
     ({"RigidBodies": [{"geometryFile": "unit_box.obj", "isDynamic": True}]}, "dynamic"),
-    ({"Emitters": [{"width": 5}]}, "Emitters"),
+    ({"Emitters": [{"id": "Other", "width": 5}]}, "fluid id"),
     ({"Configuration": {"simulationMethod": 1}}, "simulationMethod"),
     ({"Configuration": {"sim2D": True}}, "sim2D"),
     ({"Configuration": {"kernel": 0}}, "kernel"),
@@ -186,6 +186,66 @@ def test_general_meshes_become_mesh_maps(tmp_path):
     np.testing.assert_allclose(b0["lo"], [1 - half, 1 - half, 0.5], atol=1e-12)
     np.testing.assert_allclose(b1["hi"], [2.3, 0.3, 0.3], atol=1e-12)
     assert list(b0["resolution"]) == [12, 12, 12]
+
+
+def emitter_scene(bhm):
+    """The shape of the reference's data/Scenes/Emitter.json: a wall box, a
+    box emitter and a circle emitter turned by pi about y."""
+    return {
+        "Configuration": {"particleRadius": 0.025, "timeStepSize": 0.001, "boundaryHandlingMethod": bhm},
+        "Materials": [{"id": "Fluid", "maxEmitterParticles": 8000, "emitterReuseParticles": True,
+                       "emitterBoxMin": [-2.0, -2.5, -2.0], "emitterBoxMax": [1.0, 4, 2.0]}],
+        "RigidBodies": [{"geometryFile": "unit_box.obj", "translation": [0, 1, 0], "scale": [2.6, 2.1, 1.0],
+                         "isWall": True, "mapInvert": True, "mapResolution": [30, 25, 20]}],
+        "Emitters": [{"width": 8, "height": 3, "translation": [-1.2, 0.75, 0.0], "rotationAxis": [1, 0, 0],
+                      "rotationAngle": 0, "velocity": 2, "type": 0, "emitEndTime": 2},
+                     {"width": 7, "translation": [0.9, 0.3, 0.0], "rotationAxis": [0, 1, 0],
+                      "rotationAngle": math.pi, "velocity": 3, "type": 1, "emitStartTime": 2, "emitEndTime": 5}],
+        "FluidBlocks": [{"denseMode": 0, "start": [-1.25, 0.0, -0.5], "end": [0.5, 0.2, 0.5]}],
+    }
+
+
+def test_emitters_and_their_housings(tmp_path):
+    from oracle import oracle
+    sc = capi.Scene(write_scene(tmp_path, emitter_scene(2)))
+    ems, settings = sc.emitters()
+    assert settings["max_emitter_particles"] == 8000 and settings["reuse"]
+    np.testing.assert_allclose(settings["box_min"], [-2.0, -2.5, -2.0])
+    np.testing.assert_allclose(settings["box_max"], [1.0, 4.0, 2.0])
+    assert len(ems) == 2
+    a, b = ems
+    assert (a["type"], a["width"], a["height"], a["velocity"]) == (0, 8, 3, 2.0)
+    assert (a["emit_start_time"], a["emit_end_time"]) == (0.0, 2.0)
+    np.testing.assert_allclose(a["rotation"], np.eye(3), atol=1e-15)
+    assert (b["type"], b["width"], b["height"]) == (1, 7, 5)  # height: the default
+    assert b["emit_start_time"] == 2.0 and b["emit_end_time"] == 5.0
+    np.testing.assert_allclose(b["rotation"], np.diag([-1.0, 1.0, -1.0]), atol=1e-15)
+    np.testing.assert_allclose(b["translation"], [0.9, 0.3, 0.0])
+    # housings: one mesh body per emitter after the wall box
+    assert sc.nbodies == 3 and sc.body(0)["kind"] == "box"
+    r, h, d = 0.025, 0.1, 0.05
+    size_a = [2 * h, 3 * d + 2.5 * d, 8 * d + 2.5 * d]  # Emitter::getSize, box, not Akinci2012
+    for k, e, size in ((1, a, size_a), (2, b, [2.25 * h] + [2.25 * (0.5 * 7 * d + d)] * 2)):
+        body = sc.body(k)
+        assert body["kind"] == "mesh" and not body["invert"] and list(body["resolution"]) == [20, 20, 20]
+        v, f = sc.body_mesh(k)
+        R, t = e["rotation"], e["translation"]
+        dirx = R[:, 0]
+        # the cavity around the emitter is outside the wall; the back wall
+        # (unit x in [-0.55, -0.5]) is inside; far points are outside
+        sdf = oracle.mesh_sdf(v, f, np.array([t, t - 0.525 * size[0] * dirx, t + 10 * dirx, t - 10 * dirx]))
+        assert sdf[0] > 0 and sdf[1] < 0 and sdf[2] > 0 and sdf[3] > 0
+        # the open side: a point just past the rim on the axis is outside
+        assert oracle.mesh_sdf(v, f, (t + 0.6 * size[0] * dirx)[None])[0] > 0
+        # the cavity floor sits half a unit behind the emitter: the distance
+        # from the emitter position is 0.5 size[0] (nearer walls excepted)
+        lo = (R @ (np.array([-0.55, -0.55, -0.55]) * size)) + t
+        hi = (R @ (np.array([0.5, 0.55, 0.55]) * size)) + t
+        np.testing.assert_allclose(np.minimum(lo, hi), body["lo"], atol=1e-12)
+        np.testing.assert_allclose(np.maximum(lo, hi), body["hi"], atol=1e-12)
+    # Akinci2012: no housing (no mesh sampling), emitters still read
+    sc0 = capi.Scene(write_scene(tmp_path, emitter_scene(0)))
+    assert sc0.nbodies == 1 and len(sc0.emitters()[0]) == 2
 
 
 def test_malformed_files_are_errors(tmp_path):
