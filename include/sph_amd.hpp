@@ -189,6 +189,29 @@ public:
     void emittedParticles(const float *x, const float *v, std::size_t n) {
         check(sph_dfsph_emit_particles(h_, x, v, n), "emit_particles");
     }
+    // EmitterSystem on the device (EmitterSystem.cpp:93-117): addEmitter with
+    // the emitter's start and end times, enableReuseParticles /
+    // disableReuseParticles, and the fluid's maxEmitterParticles.
+    void addEmitter(unsigned width, unsigned height, const double pos[3], const double rotation[9], double velocity,
+                    int type, double emit_start_time = 0.0, double emit_end_time = 1.7976931348623157e308) {
+        sph_emitter_t e{};
+        e.type = type;
+        e.width = width;
+        e.height = height;
+        for (int c = 0; c < 3; c++) e.translation[c] = pos[c];
+        for (int c = 0; c < 9; c++) e.rotation[c] = rotation[c];
+        e.velocity = velocity;
+        e.emit_start_time = emit_start_time;
+        e.emit_end_time = emit_end_time;
+        check(sph_dfsph_add_emitter(h_, &e), "addEmitter");
+    }
+    void enableReuseParticles(const double box_min[3], const double box_max[3]) {
+        check(sph_dfsph_set_emitter_reuse(h_, 1, box_min, box_max), "enableReuseParticles");
+    }
+    void disableReuseParticles() { check(sph_dfsph_set_emitter_reuse(h_, 0, nullptr, nullptr), "disableReuseParticles"); }
+    void setMaxEmitterParticles(std::size_t n) {
+        check(sph_dfsph_set_max_emitter_particles(h_, n), "setMaxEmitterParticles");
+    }
     void step() { check(sph_dfsph_step(h_, 1), "step"); }
     void step(int n) { check(sph_dfsph_step(h_, n), "step"); }
     // sub-steps, TimeStepDFSPH.cpp:117-249
