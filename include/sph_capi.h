@@ -53,6 +53,7 @@ extern "C" {
  * NonPressureForceRegistration.cpp:52-58). */
 #define SPH_BOUNDARY_NONE       (-1)
 #define SPH_BOUNDARY_AKINCI2012  0
+#define SPH_BOUNDARY_KOSCHIER2017 1  /* density maps of the boundary objects */
 #define SPH_BOUNDARY_BENDER2019  2
 #define SPH_VISCOSITY_NONE       0
 #define SPH_VISCOSITY_STANDARD   1   /* Viscosity_Standard (explicit)        */
@@ -239,7 +240,15 @@ int sph_dfsph_set_fluid(sph_dfsph_t s, const float *x, const float *v,
 /* BoundaryModel_Akinci2012::initModel (BoundaryModel_Akinci2012.cpp:77-110)
  * for a static body: n boundary particles, 3n floats. */
 int sph_dfsph_add_boundary_particles(sph_dfsph_t s, const float *x, size_t n);
-/* Adds a BoundaryModel_Bender2019 boundary object: the volume map of an
+/* With boundary_handling_method SPH_BOUNDARY_KOSCHIER2017 the three calls
+ * below add BoundaryModel_Koschier2017 objects instead: field 1 is the
+ * density map of SimulatorBase::initDensityMap (SimulatorBase.cpp:
+ * 2322-2500: 0.8 times the integral over the support of (1 - 5 d / h) W
+ * where the distance d is at most h/5, 26-point Gauss rule per axis), and
+ * each step samples rho_b, grad rho_b and the boundary point per particle
+ * (TimeStep::computeDensityAndGradient, TimeStep.cpp:349-468).
+ *
+ * Adds a BoundaryModel_Bender2019 boundary object: the volume map of an
  * axis-aligned box wall (SimulatorBase::initVolumeMap,
  * SimulatorBase.cpp:2566-2704): the signed distance of the box surface
  * (inverted when `invert`, i.e. fluid inside), minus `thickness`, discretised
@@ -450,15 +459,15 @@ typedef struct sph_scene *sph_scene_t;
  * "Weiler et al. 2018" section, the "XSPH" section), "FluidBlocks" sampled
  * as SimulatorBase::createFluidBlocks (Simulator/SimulatorBase.cpp:
  * 1418-1516) with their initial velocities (:1079-1093), and static
- * "RigidBodies" from OBJ meshes (Bender2019: one volume map each with
- * mapInvert / mapThickness / mapResolution, analytic for an axis-aligned box
- * or a sphere, a mesh map otherwise; Akinci2012: box and sphere surfaces
- * sampled at 2r), and "Emitters" (SimulatorBase::createEmitters,
+ * "RigidBodies" from OBJ meshes (Bender2019 / Koschier2017: one volume or
+ * density map each with mapInvert / mapThickness / mapResolution, analytic
+ * for an axis-aligned box or a sphere, a mesh map otherwise; Akinci2012:
+ * box and sphere surfaces sampled at 2r), and "Emitters" (SimulatorBase::createEmitters,
  * SimulatorBase.cpp:1286-1384: width, height, translation, rotationAxis,
  * rotationAngle, velocity, type, emitStartTime, emitEndTime; the fluid
  * material's maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max;
- * with Bender2019 each emitter also gets its housing as a static mesh body
- * after the scene's bodies). Other keys that do not change the simulation
+ * with Bender2019 or Koschier2017 each emitter also gets its housing as a
+ * static mesh body after the scene's bodies). Other keys that do not change the simulation
  * are ignored; anything that would (dynamic bodies, fluid models from
  * files, other methods, several fluid ids) fails with SPH_ERR_INVALID
  * naming the key. Relative geometry paths are resolved against the scene

@@ -312,7 +312,8 @@ class OracleDFSPH:
     def get(self, name, comps=None):
         if comps is None:
             from splishsplash_amd.params import FIELD_COMPONENTS
-            comps = 3 if name.startswith("boundary x") else FIELD_COMPONENTS.get(name, 1)
+            vec = name.startswith("boundary x") or name.startswith("boundary density gradient")
+            comps = 3 if vec else FIELD_COMPONENTS.get(name, 1)
         count = self.n if name != "boundary particle volume" else None
         if count is None:
             raise ValueError("use get_boundary_volumes")

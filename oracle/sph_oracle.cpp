@@ -381,6 +381,7 @@ void VolumeMap::init(const ShapeSDF &s, double h, const unsigned *r) {
     supportRadius = h;
     cubic.set(h);
     gauss_legendre(16, gx, gw);  // p = 30 -> 16 points (GaussQuadrature.cpp:102-103)
+    gauss_legendre(26, gx26, gw26);  // p = 50 -> 26 points (GaussQuadrature.cpp:162-163)
     const double pad = 8.0 * h + s.thickness;
     double blo[3], bhi[3];
     s.aabb(blo, bhi);
@@ -480,6 +481,41 @@ double VolumeMap::node_volume(const double *x) const {
     return 0.8 * res;
 }
 
+// Density map node value (SimulatorBase.cpp:2430-2460): 0.8 times the
+// integral over the support of (1 - 5 d / h) W(e) where the distance d of
+// x + e is at most h/5, zero where d(x) > 1.2 h; 26-point product rule.
+double VolumeMap::node_density(const double *x) const {
+    const double h = supportRadius;
+    const double dist_x = sdf.eval(x);
+    if (dist_x > (1.0 + 1.0 / 5.0) * h) return 0.0;
+    double res = 0.0;
+    for (int i = 0; i < 26; i++) {
+        const double wi = gw26[i];
+        for (int j = 0; j < 26; j++) {
+            const double wij = wi * gw26[j];
+            for (int k = 0; k < 26; k++) {
+                const double wijk = wij * gw26[k];
+                const double d[3] = {h * gx26[i], h * gx26[j], h * gx26[k]};
+                double f = 0.0;
+                const double e2 = sqn3(d);
+                if (!(e2 > h * h)) {
+                    const double y[3] = {x[0] + d[0], x[1] + d[1], x[2] + d[2]};
+                    bool indom = true;
+                    for (int c = 0; c < 3; c++)
+                        if (!(y[c] >= lo[c] && y[c] <= hi[c])) indom = false;
+                    if (indom) {
+                        const double dist = sdf.kind == 2 ? mesh_field(y) : sdf.eval(y);
+                        if (!(dist > h / 5.0)) f = (1.0 - 5.0 * dist / h) * cubic.W(std::sqrt(e2));
+                    }
+                }
+                res += wijk * f;
+            }
+        }
+    }
+    res *= h * h * h;
+    return 0.8 * res;
+}
+
 const std::vector<double> &VolumeMap::cell_values(uint64_t cell) {
     auto it = cache.find(cell);
     if (it != cache.end()) return it->second;
@@ -492,7 +528,7 @@ const std::vector<double> &VolumeMap::cell_values(uint64_t cell) {
         serendipity_node(n, p);
         for (int c = 0; c < 3; c++) y[c] = smin[c] + 0.5 * (p[c] + 1.0) * cs[c];
         v[n] = node_sdf(y);
-        v[32 + n] = node_volume(y);
+        v[32 + n] = node_field1(y);
     }
     return cache.emplace(cell, std::move(v)).first->second;
 }
@@ -516,7 +552,7 @@ void VolumeMap::prefetch(const std::vector<uint64_t> &cells) {
             serendipity_node(n, p);
             for (int c = 0; c < 3; c++) y[c] = smin[c] + 0.5 * (p[c] + 1.0) * cs[c];
             v[n] = node_sdf(y);
-            v[32 + n] = node_volume(y);
+            v[32 + n] = node_field1(y);
         }
         vals[t] = std::move(v);
     }
@@ -632,6 +668,7 @@ void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
     state.assign(n, 0.0);
     vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
     vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
+    vmG.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
     initialized = false;
 }
 
@@ -644,9 +681,11 @@ void Sim::add_boundary(const double *xx, size_t nn) {
 
 void Sim::add_volume_map(const ShapeSDF &s, const unsigned *res) {
     vms.emplace_back(new VolumeMap());
+    vms.back()->density = P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017;
     vms.back()->init(s, hs, res);
     vmV.assign(vms.size() * n, 0.0);
     vmX.assign(3 * vms.size() * n, 0.0);
+    vmG.assign(3 * vms.size() * n, 0.0);
 }
 
 // Simulation::updateBoundaryVolume + BoundaryModel_Akinci2012::computeBoundaryVolume
@@ -688,8 +727,68 @@ void Sim::neighborhood_search() {
 // one boundary object after the other: a particle moved out of one object is
 // sampled against the next at its new position.
 void Sim::volume_maps() {
+    if (has_density_map())
+        for (size_t o = 0; o < vms.size(); o++) density_map(o);
     if (!has_volume_map()) return;
     for (size_t o = 0; o < vms.size(); o++) volume_map(o);
+}
+
+double Sim::map_volume(size_t o, size_t i) const {
+    const double Vj = vmV[o * n + i];
+    return has_density_map() ? Vj * (4.0 / 3.0 * M_PI) * hs * hs * hs : Vj;
+}
+
+// TimeStep::computeDensityAndGradient (TimeStep.cpp:349-468) for one static
+// density map: rho_b and grad rho_b = -grad field 1 where 0.1 r < d < h,
+// the boundary point x_i - (d + h/2) n; a particle at d <= 0.1 r is moved
+// out by at most 10 r dt / 1 s and its normal velocity set to 0.05 m/s.
+void Sim::density_map(size_t o) {
+    VolumeMap *vm = vms[o].get();
+    std::vector<uint64_t> cells;
+    cells.reserve(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t c;
+        double xi[3];
+        if (vm->locate(&x[3 * i], c, xi)) cells.push_back(c);
+    }
+    vm->prefetch(cells);
+    const double supportRadius = hs, particleRadius = r;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        double *xi = &x[3 * i];
+        double *xb = &vmX[3 * (o * n + i)];
+        double *gb = &vmG[3 * (o * n + i)];
+        double &rb = vmV[o * n + i];
+        rb = 0.0;
+        for (int c = 0; c < 3; c++) xb[c] = gb[c] = 0.0;
+        double normal[3] = {0, 0, 0};
+        double dist = std::numeric_limits<double>::max(), val;
+        if (vm->interpolate(0, xi, &val, normal)) dist = val;
+        if ((dist > 0.1 * particleRadius) && (dist < supportRadius)) {
+            double gradD[3], d;
+            vm->interpolate(1, xi, &d, gradD);
+            if ((d > 1e-6) && (d != std::numeric_limits<double>::max())) {
+                const double nl = std::sqrt(sqn3(normal));
+                if (nl > 1.0e-6) {
+                    rb = d;
+                    for (int c = 0; c < 3; c++) {
+                        gb[c] = -gradD[c];
+                        xb[c] = xi[c] - (dist + 0.5 * supportRadius) * (normal[c] / nl);
+                    }
+                }
+            }
+        } else if (dist <= 0.1 * particleRadius) {
+            const double nl = std::sqrt(sqn3(normal));
+            if (nl > 1.0e-5) {
+                for (int c = 0; c < 3; c++) normal[c] /= nl;
+                double d = -dist;
+                d = std::min(d, (0.25 / 0.005) * particleRadius * dt);
+                for (int c = 0; c < 3; c++) xi[c] += d * normal[c];
+                const double vn = 0.05 - dot3(&v[3 * i], normal);
+                for (int c = 0; c < 3; c++) v[3 * i + c] += vn * normal[c];
+            }
+        }
+    }
 }
 
 void Sim::volume_map(size_t o) {
@@ -785,6 +884,8 @@ void Sim::densities() {
                     density += Vj * K.W(d);
                 }
             }
+        } else if (has_density_map()) {
+            for (size_t o = 0; o < vms.size(); o++) density += vmV[o * n + i];  // TimeStep.cpp:153-158
         }
         rho[i] = density * density0;
     }
@@ -827,6 +928,10 @@ void Sim::dfsph_factor() {
                     for (int c = 0; c < 3; c++) gpi[c] -= -Vj * g[c];
                 }
             }
+        } else if (has_density_map()) {  // TimeStepDFSPH.cpp:1158-1163
+            for (size_t o = 0; o < vms.size(); o++)
+                if (vmV[o * n + i] != 0.0)
+                    for (int c = 0; c < 3; c++) gpi[c] -= vmG[3 * (o * n + i) + c];
         }
         sum_grad_p_k += sqn3(gpi);
         factor[i] = (sum_grad_p_k > eps) ? 1.0 / sum_grad_p_k : 0.0;
@@ -869,6 +974,9 @@ static inline double div_sum(const Sim &S, size_t i) {
                 delta += Vj * dot3(vi, g);
             }
         }
+    } else if (S.has_density_map()) {  // TimeStepDFSPH.cpp:1224-1231
+        for (size_t o = 0; o < S.vms.size(); o++)
+            if (S.vmV[o * S.n + i] != 0.0) delta -= dot3(vi, &S.vmG[3 * (o * S.n + i)]);
     }
     return delta;
 }
@@ -932,6 +1040,10 @@ void Sim::pressure_accel(const std::vector<double> &prho2, size_t i, double *ai)
                     for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * (-Vj * g[c]);
                 }
             }
+        } else if (has_density_map()) {  // TimeStepDFSPH.cpp:1346-1353
+            for (size_t o = 0; o < vms.size(); o++)
+                if (vmV[o * n + i] != 0.0)
+                    for (int c = 0; c < 3; c++) ai[c] += (1.0 * p_i) * vmG[3 * (o * n + i) + c];
         }
     }
 }
@@ -970,6 +1082,9 @@ double Sim::aij_pj(size_t i) const {
                 s += Vj * dot3(ai, g);
             }
         }
+    } else if (has_density_map()) {  // TimeStepDFSPH.cpp:1407-1412
+        for (size_t o = 0; o < vms.size(); o++)
+            if (vmV[o * n + i] != 0.0) s -= dot3(ai, &vmG[3 * (o * n + i)]);
     }
     return s;
 }
@@ -1094,10 +1209,10 @@ void Sim::weiler_matvec(const std::vector<double> &vec, std::vector<double> &res
                     const double s = d * mub * (density0 * bV[j] / density_i) * dot3(vi, xixj) / (sqn3(xixj) + 0.01 * h2);
                     for (int c = 0; c < 3; c++) ai[c] += s * g[c];
                 }
-            } else if (has_volume_map()) {
+            } else if (has_volume_map() || has_density_map()) {
                 for (size_t o = 0; o < vms.size(); o++) {
-                    const double Vj = vmV[o * n + i];
-                    if (!(Vj > 0.0)) continue;
+                    if (!(vmV[o * n + i] > 0.0)) continue;
+                    const double Vj = map_volume(o, i);  // Koschier2017: Viscosity_Weiler2018.cpp:154-200
                     const double *xj = &vmX[3 * (o * n + i)];
                     const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
                     double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
@@ -1160,10 +1275,10 @@ void Sim::weiler_diag(size_t i, double *m) const {
                 for (int r0 = 0; r0 < 3; r0++)
                     for (int c = 0; c < 3; c++) m[3 * r0 + c] += s * (g[r0] * xixj[c]);
             }
-        } else if (has_volume_map()) {
+        } else if (has_volume_map() || has_density_map()) {
             for (size_t o = 0; o < vms.size(); o++) {
-                const double Vj = vmV[o * n + i];
-                if (!(Vj > 0.0)) continue;
+                if (!(vmV[o * n + i] > 0.0)) continue;
+                const double Vj = map_volume(o, i);
                 const double *xj = &vmX[3 * (o * n + i)];
                 const double xixj[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
                 double normal[3] = {-xixj[0], -xixj[1], -xixj[2]};
@@ -1266,10 +1381,10 @@ void Sim::viscosity_standard() {
                 const double s = d * nub * (density0 * bV[j] / density_i) * dot3(vi, xixj) / (sqn3(xixj) + 0.01 * h2);
                 for (int c = 0; c < 3; c++) ai[c] += s * g[c];
             }
-        } else if (has_volume_map()) {
+        } else if (has_volume_map() || has_density_map()) {
             for (size_t o = 0; o < vms.size(); o++) {
-                const double Vj = vmV[o * n + i];
-                if (!(Vj > 0.0)) continue;
+                if (!(vmV[o * n + i] > 0.0)) continue;
+                const double Vj = map_volume(o, i);  // Koschier2017: Viscosity_Standard.cpp:295-340
                 const double *xj = &vmX[3 * (o * n + i)];
                 double normal[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
                 const double nl = std::sqrt(sqn3(normal));
@@ -1333,6 +1448,13 @@ void Sim::xsph() {
                     const double w = K.W(d);
                     const double s = -invH * cb * (density0 * Vj / density_i);
                     for (int c = 0; c < 3; c++) ai[c] += s * vi[c] * w;
+                }
+            } else if (has_density_map()) {  // XSPH.cpp:111-120
+                for (size_t o = 0; o < vms.size(); o++) {
+                    const double rb = vmV[o * n + i];
+                    if (rb == 0.0) continue;
+                    const double s = -invH * cb * (density0 / density_i) * rb;
+                    for (int c = 0; c < 3; c++) ai[c] += s * vi[c];
                 }
             }
         }
@@ -1490,6 +1612,7 @@ void Sim::resize_particles(size_t m) {
     n = m;
     vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
     vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
+    vmG.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
 }
 
 // Emitter::emitParticles / emitParticlesCircle (Emitter.cpp:89-380): animate
@@ -1697,16 +1820,20 @@ bool field_of(Sim *s, const char *name, FieldRef &f) {
 int vm_field(Sim *s, const char *name, size_t count, double *&out) {
     const std::string n(name);
     out = nullptr;
-    const bool isx = n.rfind("boundary x", 0) == 0;
-    if (!isx && n.rfind("boundary volume", 0) != 0) return SPH_OK;
-    const std::string rest = n.substr(isx ? 10 : 15);
+    // density maps: "boundary density" is the volume slot, "boundary
+    // density gradient" the gradient
+    const bool isg = n.rfind("boundary density gradient", 0) == 0;
+    const bool isd = !isg && n.rfind("boundary density", 0) == 0;
+    const bool isx = isg || n.rfind("boundary x", 0) == 0;
+    if (!isx && !isd && n.rfind("boundary volume", 0) != 0) return SPH_OK;
+    const std::string rest = n.substr(isg ? 25 : (isd ? 16 : (n.rfind("boundary x", 0) == 0 ? 10 : 15)));
     if (!rest.empty() && (rest[0] != ' ' || rest.size() < 2 || rest.find_first_not_of("0123456789", 1) != std::string::npos))
         return SPH_OK;  // not a volume-map field
     const size_t o = rest.empty() ? 0 : (size_t)std::atoi(rest.c_str() + 1);
     if (o >= std::max<size_t>(s->vms.size(), 1)) return fail("no such volume map");
     const size_t k = isx ? 3 : 1;
     if (count != k * s->n) return fail("size mismatch");
-    out = (isx ? s->vmX.data() : s->vmV.data()) + k * o * s->n;
+    out = (isg ? s->vmG.data() : (isx ? s->vmX.data() : s->vmV.data())) + k * o * s->n;
     return SPH_OK;
 }
 }  // namespace
@@ -2027,6 +2154,8 @@ int orc_shape_volume_map_eval(int kind, const double *a, const double *b, int in
                               const double *x, size_t n, double *val, double *grad, int *ok) {
     const orc::ShapeSDF s = make_shape(kind, a, b, invert, thickness);
     orc::VolumeMap vm;
+    vm.density = field == 2;  // field 2: a density map's field 1
+    if (field == 2) field = 1;
     vm.init(s, support_radius, res);
     if (interpolate) {
         std::vector<uint64_t> cells;
@@ -2041,7 +2170,7 @@ int orc_shape_volume_map_eval(int kind, const double *a, const double *b, int in
         if (interpolate) {
             ok[i] = vm.interpolate(field, x + 3 * i, val + i, grad ? grad + 3 * i : nullptr) ? 1 : 0;
         } else {
-            val[i] = field == 0 ? vm.node_sdf(x + 3 * i) : vm.node_volume(x + 3 * i);
+            val[i] = field == 0 ? vm.node_sdf(x + 3 * i) : vm.node_field1(x + 3 * i);
             ok[i] = 1;
         }
     }

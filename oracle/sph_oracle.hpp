@@ -92,6 +92,8 @@ struct VolumeMap {
     double supportRadius = 0;
     Cubic cubic;
     double gx[16], gw[16];
+    double gx26[26], gw26[26];
+    bool density = false;  // field 1 is a Koschier2017 density map
     std::unordered_map<uint64_t, std::vector<double>> cache;  // 64 values per cell
     std::vector<double> mesh_sdf;  // mesh: field 0 of every cell (32 per cell), built at init
     void init(const ShapeSDF &s, double supportRadius, const unsigned *res);
@@ -99,6 +101,8 @@ struct VolumeMap {
     double node_sdf(const double *x) const;
     double mesh_field(const double *y) const;
     double node_volume(const double *x) const;
+    double node_density(const double *x) const;
+    double node_field1(const double *x) const { return density ? node_density(x) : node_volume(x); }
     const std::vector<double> &cell_values(uint64_t cell);
     void prefetch(const std::vector<uint64_t> &cells);
     // Interpolates field 0 (sdf) or 1 (volume); false outside the domain.
@@ -139,6 +143,8 @@ public:
     std::vector<double> bx, bV;         // Akinci particles and volumes
     std::vector<std::unique_ptr<VolumeMap>> vms;  // boundary objects in insertion order
     std::vector<double> vmV, vmX;       // per object and fluid particle: V_b [o * n + i], x_b
+                                        // (Koschier2017: rho_b and x_b)
+    std::vector<double> vmG;            // Koschier2017: grad rho_b [3 * (o * n + i)]
     CSR nf, nbd;                        // fluid->fluid, fluid->boundary
     double time = 0, dt = 0, h_step = 0;
     int iterations = 0, iterationsV = 0, viscoIterations = 0, steps = 0;
@@ -169,6 +175,7 @@ public:
     void neighborhood_search();
     void volume_maps();
     void volume_map(size_t o);
+    void density_map(size_t o);
     void densities();
     void dfsph_factor();
     void divergence_setup();
@@ -194,6 +201,10 @@ public:
     void weiler_diag(size_t i, double *m) const;
     bool has_boundary_particles() const { return P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 && nb > 0; }
     bool has_volume_map() const { return P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 && !vms.empty(); }
+    bool has_density_map() const { return P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017 && !vms.empty(); }
+    // boundary volume standing for map record (o, i) in the viscosity
+    // samples: V_b, or rho_b 4/3 pi h^3 with density maps
+    double map_volume(size_t o, size_t i) const;
 };
 
 }  // namespace orc

@@ -601,8 +601,11 @@ class TimeStepDFSPH:
         if name == "boundary particle volume":
             out = np.zeros(self.nb, np.float32)
         else:
-            if name.startswith("boundary x ") or name.startswith("boundary volume "):
-                comps = 3 if name.startswith("boundary x") else 1  # per volume-map object
+            if (name.startswith("boundary x ") or name.startswith("boundary volume ") or
+                    name.startswith("boundary density")):
+                # per map object; density maps: "boundary density [o]" and
+                # "boundary density gradient [o]"
+                comps = 3 if (name.startswith("boundary x") or name.startswith("boundary density gradient")) else 1
             else:
                 comps = FIELD_COMPONENTS[name]
             dt = np.uint32 if name in ("id", "state", "fluid neighbor count", "boundary neighbor count") else np.float32

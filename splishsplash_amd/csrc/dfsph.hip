@@ -155,6 +155,22 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_init(Dev D) {
             }
             if (DIV) divb += xb.w * (((vi.x * gx + vi.y * gy) + vi.z * gz));
         }
+    } else if (D.bmode == 3) {
+        // density maps: rho_b adds to the density, grad rho_b to the factor's
+        // gradient sum and the density change (TimeStep.cpp:153-158,
+        // TimeStepDFSPH.cpp:1158-1163, 1224-1231)
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
+            const float4 gb = D.dmg[(size_t)o * D.stride + i];
+            if (DENS) dens += xb.w;
+            if (FACT) {
+                gpx -= gb.x;
+                gpy -= gb.y;
+                gpz -= gb.z;
+            }
+            if (DIV) divb -= ((vi.x * gb.x + vi.y * gb.y) + vi.z * gb.z);
+        }
     }
     if (DENS) D.rho[i] = dens * D.rho0;
     if (FACT || DIV) {
@@ -223,6 +239,14 @@ __device__ __forceinline__ void accel_boundary(const Dev &D, unsigned i, const f
                 ax += pi * (-xb.w * (g * r.x));
                 ay += pi * (-xb.w * (g * r.y));
                 az += pi * (-xb.w * (g * r.z));
+            }
+        } else if (D.bmode == 3) {  // p_i grad rho_b (TimeStepDFSPH.cpp:1346-1353)
+            for (unsigned o = 0; o < D.nvm; o++) {
+                if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+                const float4 gb = D.dmg[(size_t)o * D.stride + i];
+                ax += pi * gb.x;
+                ay += pi * gb.y;
+                az += pi * gb.z;
             }
         }
     }
@@ -342,6 +366,14 @@ __global__ __launch_bounds__(BLOCK) void k_accel(Dev D, int check_done) {
                 ay += pi * (-xb.w * (g * r.y));
                 az += pi * (-xb.w * (g * r.z));
             }
+        } else if (D.bmode == 3) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+                const float4 gb = D.dmg[(size_t)o * D.stride + i];
+                ax += pi * gb.x;
+                ay += pi * gb.y;
+                az += pi * gb.z;
+            }
         }
     }
     D.ap[i] = make_float4(ax, ay, az, 0.f);
@@ -414,6 +446,12 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi(Dev D, int check_done) {
                 float3 r;
                 const float g = kernel_G(xi, xb, D.K, r);
                 s += xb.w * (((ai.x * (g * r.x) + ai.y * (g * r.y)) + ai.z * (g * r.z)));
+            }
+        } else if (D.bmode == 3) {  // aij_pj -= a_i . grad rho_b (TimeStepDFSPH.cpp:1407-1412)
+            for (unsigned o = 0; o < D.nvm; o++) {
+                if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+                const float4 gb = D.dmg[(size_t)o * D.stride + i];
+                s -= (ai.x * gb.x + ai.y * gb.y) + ai.z * gb.z;
             }
         }
         const float alpha = D.alpha[i];
@@ -604,7 +642,9 @@ __device__ __forceinline__ void std_visc_boundary(const Dev &D, unsigned i, cons
             const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
             term(xb, D.sviscB * (D.rho0 * xb.w / rhoi), ai.x, ai.y, ai.z);
         }
-    } else if (D.bmode == 2) {
+    } else if (D.bmode == 2 || D.bmode == 3) {
+        // Koschier2017: the samples stand for rho_b times the support's
+        // volume 4/3 pi h^3 (Viscosity_Standard.cpp:295-340)
         for (unsigned o = 0; o < D.nvm; o++) {
             const float4 xb = D.vmb[(size_t)o * D.stride + i];
             if (!(xb.w > 0.f)) break;
@@ -614,7 +654,8 @@ __device__ __forceinline__ void std_visc_boundary(const Dev &D, unsigned i, cons
             nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
             float3 t1, t2;
             ortho(nrm, t1, t2);
-            const float dist = (1.0f - nl / h) * h, c = D.sviscB * (0.25f * xb.w);
+            const float vb = D.bmode == 3 ? xb.w * D.sphereVol : xb.w;
+            const float dist = (1.0f - nl / h) * h, c = D.sviscB * (0.25f * vb);
             float bx = 0.f, by = 0.f, bz = 0.f;  // a1 + a2 + a3 + a4
             for (int q = 0; q < 4; q++) {
                 const float3 t = q < 2 ? t1 : t2;
@@ -720,6 +761,15 @@ __global__ __launch_bounds__(BLOCK) void k_post(Dev D) {
                         if (!(xb.w > 0.f)) break;
                         bterm(xb);
                     }
+                } else if (D.bmode == 3) {  // -(1/h) c_b (rho0 / rho_i) v_i rho_b (XSPH.cpp:111-120)
+                    for (unsigned o = 0; o < D.nvm; o++) {
+                        const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                        if (!(xb.w > 0.f)) break;
+                        const float s = -invH * D.xsphB * (D.rho0 / rhoi) * xb.w;
+                        ai.x += s * vi.x;
+                        ai.y += s * vi.y;
+                        ai.z += s * vi.z;
+                    }
                 }
             }
         }
@@ -811,6 +861,12 @@ __global__ __launch_bounds__(BLOCK) void k_psetup(Dev D) {
             const float g = kernel_G(xi, xb, D.K, r);
             delta += xb.w * ((vi.x * (g * r.x) + vi.y * (g * r.y)) + vi.z * (g * r.z));
         }
+    } else if (D.bmode == 3) {  // TimeStepDFSPH.cpp:1224-1231
+        for (unsigned o = 0; o < D.nvm; o++) {
+            if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+            const float4 gb = D.dmg[(size_t)o * D.stride + i];
+            delta -= (vi.x * gb.x + vi.y * gb.y) + vi.z * gb.z;
+        }
     }
     const float radv = D.rho[i] / D.rho0 + (float)h * delta;
     D.radv[i] = radv;
@@ -890,7 +946,8 @@ __device__ __forceinline__ void weiler_boundary(const Dev &D, unsigned i, const 
             const float4 xb = D.bx[D.nb[(size_t)k * D.stride + i]];
             f(xb, dd * D.mub * (D.rho0 * xb.w / rhoi));
         }
-    } else if (D.bmode == 2) {
+    } else if (D.bmode == 2 || D.bmode == 3) {
+        // Koschier2017: rho_b times 4/3 pi h^3 (Viscosity_Weiler2018.cpp:154-200)
         for (unsigned o = 0; o < D.nvm; o++) {
             const float4 xb = D.vmb[(size_t)o * D.stride + i];
             if (!(xb.w > 0.f)) break;
@@ -900,7 +957,7 @@ __device__ __forceinline__ void weiler_boundary(const Dev &D, unsigned i, const 
                 nrm = make_float3(nrm.x / nl, nrm.y / nl, nrm.z / nl);
                 float3 t1, t2;
                 ortho(nrm, t1, t2);
-                const float dist = 0.5f * D.K.h, vol = 0.25f * xb.w;
+                const float dist = 0.5f * D.K.h, vol = 0.25f * (D.bmode == 3 ? xb.w * D.sphereVol : xb.w);
                 for (int q = 0; q < 4; q++) {
                     const float3 t = q < 2 ? t1 : t2;
                     const float sg = (q % 2 == 0) ? -1.f : 1.f;
@@ -1264,7 +1321,7 @@ __global__ __launch_bounds__(BLOCK) void k_features(Dev D) {
         const unsigned cb = D.cb[i];
         nnb += cb;
         for (unsigned k = 0; k < cb; k++) bvol += D.bx[D.nb[(size_t)k * D.stride + i]].w;
-    } else if (D.bmode == 2) {
+    } else if (D.bmode == 2 || D.bmode == 3) {
         for (unsigned o = 0; o < D.nvm; o++) {
             const float w = D.vmb[(size_t)o * D.stride + i].w;
             if (!(w > 0.f)) break;
@@ -1389,6 +1446,21 @@ __global__ __launch_bounds__(BLOCK) void k_accel2(Dev D) {
                 if (!(xb.w > 0.f)) break;
                 bterm(xb);
             }
+        } else if (D.bmode == 3) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+                const float4 gb = D.dmg[(size_t)o * D.stride + i];
+                if (bp) {
+                    ax += pi * gb.x;
+                    ay += pi * gb.y;
+                    az += pi * gb.z;
+                }
+                if (bq) {
+                    bxx += qi * gb.x;
+                    byy += qi * gb.y;
+                    bzz += qi * gb.z;
+                }
+            }
         }
     }
     D.ap[i] = make_float4(ax, ay, az, 0.f);
@@ -1454,6 +1526,13 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi2(Dev D) {
                 if (!(xb.w > 0.f)) break;
                 bterm(xb);
             }
+        } else if (D.bmode == 3) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+                const float4 gb = D.dmg[(size_t)o * D.stride + i];
+                s1 -= (ai.x * gb.x + ai.y * gb.y) + ai.z * gb.z;
+                s2 -= (bi.x * gb.x + bi.y * gb.y) + bi.z * gb.z;
+            }
         }
         const float sc = PRESSURE ? (float)(h * h) : (float)h;
         const float si = PRESSURE ? 1.0f - D.radv[i] : -D.radv[i];
@@ -1518,8 +1597,10 @@ DFSPH::DFSPH(const sph_dfsph_params_t &p) : P(p) {
                     P.viscosity_method == SPH_VISCOSITY_WEILER2018,
                 SPH_ERR_INVALID, "viscosity_method: only none (0), standard (1) and Weiler2018 (6) are implemented");
     SPH_REQUIRE(P.boundary_handling_method == SPH_BOUNDARY_NONE || P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012 ||
+                    P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017 ||
                     P.boundary_handling_method == SPH_BOUNDARY_BENDER2019,
-                SPH_ERR_INVALID, "boundary_handling_method: only none (-1), Akinci2012 (0) and Bender2019 (2)");
+                SPH_ERR_INVALID,
+                "boundary_handling_method: only none (-1), Akinci2012 (0), Koschier2017 (1) and Bender2019 (2)");
     r = P.particle_radius;
     hs = 4.0 * r;
     const double diam = 2.0 * r;
@@ -1629,7 +1710,12 @@ Dev DFSPH::dev() const {
     D.cb = cb_.get();
     D.bx = bx_.get();
     const int bm = P.boundary_handling_method;
-    D.bmode = (bm == SPH_BOUNDARY_AKINCI2012 && nb_ > 0) ? 1 : ((bm == SPH_BOUNDARY_BENDER2019 && nvm_ > 0) ? 2 : 0);
+    D.bmode = (bm == SPH_BOUNDARY_AKINCI2012 && nb_ > 0)       ? 1
+              : (bm == SPH_BOUNDARY_BENDER2019 && nvm_ > 0)   ? 2
+              : (bm == SPH_BOUNDARY_KOSCHIER2017 && nvm_ > 0) ? 3
+                                                              : 0;
+    D.dmg = dmg_.get();
+    D.sphereVol = (float)(4.0 / 3.0 * 3.14159265358979323846 * hs * hs * hs);
     D.V = (float)V;
     D.rho0 = (float)P.density0;
     D.eps = 1.0e-5f;
@@ -1701,9 +1787,13 @@ void DFSPH::upload_ctl_field(size_t off, const void *src, size_t bytes) {
 void DFSPH::alloc_particles() {
     const size_t S = stride_;
     for (DBuf<float4> *b : {&x_, &v_, &v2_, &a_, &ap_, &vdiff_, &xc_, &x2_, &vdiff2_, &xc2_}) b->reserve(S);
-    const size_t nobj = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 ? vm_shapes_.size() : 0;
+    const size_t nobj = maps() ? vm_shapes_.size() : 0;
     vmb_.reserve(std::max<size_t>(nobj, 1) * S);
     if (nobj > 1) vmraw_.reserve(nobj * S);
+    if (P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017) {
+        dmg_.reserve(std::max<size_t>(nobj, 1) * S);
+        if (nobj > 1) dmgraw_.reserve(nobj * S);
+    }
     for (DBuf<float> *b : {&rho_, &alpha_, &radv_, &p_, &pv_, &p2_, &pv2_}) b->reserve(S);
     id_.reserve(S);
     id2_.reserve(S);
@@ -1727,7 +1817,7 @@ void DFSPH::alloc_particles() {
 void DFSPH::upload_particles(size_t n, const float *x, const float *v, const float4 *xc, const float4 *vdiff,
                              const float *p, const float *pv) {
     const size_t S = stride_;
-    const size_t nobj = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 ? vm_shapes_.size() : 0;
+    const size_t nobj = maps() ? vm_shapes_.size() : 0;
     std::vector<float4> hx(n), hv(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<unsigned> ids(n);
     for (size_t i = 0; i < n; i++) {
@@ -1743,8 +1833,10 @@ void DFSPH::upload_particles(size_t n, const float *x, const float *v, const flo
         SPH_HIP(hipMemcpy(b->get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     SPH_HIP(hipMemcpy(vdiff_.get(), vdiff ? vdiff : zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     SPH_HIP(hipMemcpy(xc_.get(), xc ? xc : zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-    for (size_t o = 0; o < std::max<size_t>(nobj, 1); o++)
+    for (size_t o = 0; o < std::max<size_t>(nobj, 1); o++) {
         SPH_HIP(hipMemcpy(vmb_.get() + o * S, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        if (dmg_.get()) SPH_HIP(hipMemcpy(dmg_.get() + o * S, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
     for (DBuf<float> *b : {&rho_, &alpha_, &radv_})
         SPH_HIP(hipMemcpy(b->get(), zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
     SPH_HIP(hipMemcpy(p_.get(), p ? p : zf.data(), n * sizeof(float), hipMemcpyHostToDevice));
@@ -1764,8 +1856,7 @@ void DFSPH::init() {
     cap_emit_ = n + max_emit_;
     stride_ = (unsigned)(((emitters_.empty() ? n : cap_emit_) + 255) / 256 * 256);
     const size_t S = stride_;
-    const bool bender = P.boundary_handling_method == SPH_BOUNDARY_BENDER2019;
-    const size_t nobj = bender ? vm_shapes_.size() : 0;
+    const size_t nobj = maps() ? vm_shapes_.size() : 0;
     alloc_particles();
     upload_particles(n, hx_.data(), hv_.data(), nullptr, nullptr, nullptr, nullptr);
     // grid over fluid, boundary particles and the volume-map box
@@ -1837,7 +1928,8 @@ void DFSPH::init() {
     vmaps.clear();
     for (size_t o = 0; o < nobj; o++) {
         vmaps.emplace_back(new VolumeMap());
-        vmaps.back()->build(stream, vm_shapes_[o], vm_res_[o].data(), hs);
+        vmaps.back()->build(stream, vm_shapes_[o], vm_res_[o].data(), hs,
+                            P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017);
     }
     nvm_ = (unsigned)nobj;
     initialized_ = true;
@@ -1935,7 +2027,24 @@ void DFSPH::neighborhood_search() {
 }
 
 void DFSPH::volume_maps() {
-    if (P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 || nvm_ == 0) return;
+    if (!maps() || nvm_ == 0) return;
+    if (P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017) {
+        // TimeStep::computeDensityAndGradient (TimeStep.cpp:349-468), objects
+        // in insertion order as for the volume maps
+        const double *dt = &ctl_.get()->h;
+        if (nvm_ == 1) {
+            sample_density_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(),
+                               dmg_.get(), dt, hs, r, timer);
+            return;
+        }
+        for (unsigned o = 0; o < nvm_; o++)
+            sample_density_map(stream, vmaps[o]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(),
+                               vmraw_.get() + (size_t)o * stride_, dmgraw_.get() + (size_t)o * stride_, dt, hs, r,
+                               timer);
+        pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer, dmgraw_.get(),
+                         dmg_.get());
+        return;
+    }
     // objects in insertion order: a particle projected out of one object is
     // sampled against the next at its new position (TimeStep.cpp:177-222)
     if (nvm_ == 1) {
@@ -2565,10 +2674,17 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
     else if (name == "acceleration") f4 = a_.get(), comps = 3;
     else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
     else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
-    else if (name.rfind("boundary x", 0) == 0 || name.rfind("boundary volume", 0) == 0) {
-        // "boundary x" / "boundary volume" of volume-map object 0, or "... <o>"
-        const bool isx = name.rfind("boundary x", 0) == 0;
-        const std::string rest = name.substr(isx ? 10 : 15);
+    else if (name.rfind("boundary x", 0) == 0 || name.rfind("boundary volume", 0) == 0 ||
+             name.rfind("boundary density", 0) == 0) {
+        // "boundary x" / "boundary volume" of map object 0, or "... <o>";
+        // with Koschier2017 the volume is the boundary density and
+        // "boundary density gradient" is its gradient
+        const bool isg = name.rfind("boundary density gradient", 0) == 0;
+        SPH_REQUIRE(!isg || P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017, SPH_ERR_NOTFOUND,
+                    "boundary density gradients exist with Koschier2017 only");
+        const bool isx = isg || name.rfind("boundary x", 0) == 0;
+        const size_t plen = isg ? 25 : (name.rfind("boundary x", 0) == 0 ? 10 : (name.rfind("boundary volume", 0) == 0 ? 15 : 16));
+        const std::string rest = name.substr(plen);
         unsigned o = 0;
         if (!rest.empty()) {
             SPH_REQUIRE(rest[0] == ' ' && rest.size() > 1 && rest.find_first_not_of("0123456789", 1) == std::string::npos,
@@ -2576,7 +2692,8 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
             o = (unsigned)std::stoul(rest.substr(1));
         }
         SPH_REQUIRE(o < std::max(nvm_, 1u), SPH_ERR_NOTFOUND, "no volume map " + std::to_string(o));
-        f4 = (nvm_ > 1 ? vmraw_.get() : vmb_.get()) + (size_t)o * stride_;
+        f4 = (isg ? (nvm_ > 1 ? dmgraw_.get() : dmg_.get()) : (nvm_ > 1 ? vmraw_.get() : vmb_.get())) +
+             (size_t)o * stride_;
         comps = isx ? 3 : 1;
         vmpart = isx ? 1 : 2;
     }

@@ -45,9 +45,12 @@ struct Dev {
     float *rho, *alpha, *radv, *p, *pv;
     const unsigned *nf, *cf, *nb, *cb;
     const float4 *bx;  // Akinci particles (x, y, z, volume), cell-sorted
-    int bmode;         // 0 none, 1 Akinci2012, 2 Bender2019
-    unsigned nvm;      // volume-map slots: vmb[o * stride + i], entries with
-                       // V_b > 0 first (object order), then V_b = 0
+    int bmode;         // 0 none, 1 Akinci2012, 2 Bender2019, 3 Koschier2017
+    unsigned nvm;      // map slots: vmb[o * stride + i], entries with
+                       // w > 0 first (object order), then w = 0; w is V_b
+                       // (Bender2019) or rho_b (Koschier2017)
+    float4 *dmg;       // Koschier2017: grad rho_b per slot, same layout
+    float sphereVol;   // 4/3 pi h^3 (Koschier2017 viscosity samples)
     float V, rho0, eps, mass;
     float3 grav;
     KernelConst K;
@@ -163,7 +166,11 @@ private:
     unsigned nvm_ = 0;                         // volume maps built
 
     // device state
-    DBuf<float4> x_, v_, v2_, a_, ap_, vdiff_, xc_, vmb_, vmraw_, bx_;
+    DBuf<float4> x_, v_, v2_, a_, ap_, vdiff_, xc_, vmb_, vmraw_, bx_, dmg_, dmgraw_;
+    bool maps() const {  // boundary objects as maps (Bender2019 volume, Koschier2017 density)
+        return P.boundary_handling_method == SPH_BOUNDARY_BENDER2019 ||
+               P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017;
+    }
     DBuf<float4> x2_, vdiff2_, xc2_;  // permutation targets
     DBuf<float> rho_, alpha_, radv_, p_, pv_, p2_, pv2_;
     DBuf<unsigned> id_, id2_, nf_, cf_, nb_l_, cb_, bperm_, bstart_;

@@ -537,7 +537,8 @@ static Scene *read_scene(const std::string &path) {
     P.cfl_min_time_step_size = num(cfg, "cflMinTimeStepSize", P.cfl_min_time_step_size);
     P.cfl_max_time_step_size = num(cfg, "cflMaxTimeStepSize", P.cfl_max_time_step_size);
     P.boundary_handling_method = (int)num(cfg, "boundaryHandlingMethod", P.boundary_handling_method);
-    if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012 && P.boundary_handling_method != SPH_BOUNDARY_BENDER2019)
+    if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012 && P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 &&
+        P.boundary_handling_method != SPH_BOUNDARY_KOSCHIER2017)
         unsupported("\"boundaryHandlingMethod\" " + std::to_string(P.boundary_handling_method));
     if (const JValue *d = cfg.find("DFSPH")) {
         P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
@@ -690,7 +691,7 @@ static Scene *read_scene(const std::string &path) {
     // emitter housings after the scene's bodies, as the reference appends
     // them to the boundary models; Akinci2012 has no mesh sampling here, so
     // its emitters run without a housing
-    if (P.boundary_handling_method == SPH_BOUNDARY_BENDER2019)
+    if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012)
         S->bodies.insert(S->bodies.end(), housings.begin(), housings.end());
     return S.release();
 }

@@ -9,16 +9,16 @@
 
 namespace sph {
 
-struct Gauss16 {
-    double x[16], w[16];
+template <int N> struct GaussN {
+    double x[N], w[N];
 };
 
 // Gauss-Legendre rule on [-1, 1] (Newton on the Legendre recurrence, in
 // extended precision), abscissae ascending.
-static Gauss16 gauss16() {
-    Gauss16 g{};
+template <int N> static GaussN<N> gauss_rule() {
+    GaussN<N> g{};
     typedef long double LD;
-    const int n = 16, m = 8;
+    const int n = N, m = (N + 1) / 2;
     const LD pi = 3.141592653589793238462643383279502884L;
     for (int i = 1; i <= m; i++) {
         LD z = std::cos(pi * (i - 0.25L) / (n + 0.5L)), pp = 0;
@@ -189,7 +189,8 @@ __device__ __forceinline__ void node_pos(unsigned long long g, const VMapDesc &d
 }
 
 struct QuadArgs {
-    Gauss16 g;
+    GaussN<16> g;    // p = 30: volume maps
+    GaussN<26> g26;  // p = 50: density maps
     double h, k, W0;
 };
 
@@ -210,7 +211,13 @@ __global__ __launch_bounds__(BLOCK) void k_vm_mesh_nodes(VMapDesc d, NodeCounts 
     sdf_node[g] = (float)mesh_sdf(x, d);
 }
 
-template <bool MESH>
+// Field 1 at a node. Volume map (Bender2019): 0.8 int_{|e|<h} gamma(d(x+e))
+// de with gamma = 1 inside, W(d)/W(0) within h of the surface (p = 30,
+// SimulatorBase.cpp:2637-2650). Density map (Koschier2017): 0.8 int_{|e|<h}
+// (1 - 5 d(x+e) / h) W(e) de over d(x+e) <= h/5, zero where d(x) > 1.2 h
+// (p = 50, SimulatorBase.cpp:2430-2460). d is the analytic distance of a box
+// or sphere, or the interpolated distance field of a mesh.
+template <bool MESH, bool DENSITY>
 __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, unsigned long long total, QuadArgs q,
                                                     float *sdf_node, float *vol_node) {
     const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,29 +227,35 @@ __global__ __launch_bounds__(BLOCK) void k_vm_nodes(VMapDesc d, NodeCounts nc, u
     const double dist_x = MESH ? (double)sdf_node[g] : shape_sdf(x, d);
     if (!MESH) sdf_node[g] = (float)dist_x;
     const double h = q.h;
+    constexpr int NQ = DENSITY ? 26 : 16;
+    const double *gx = DENSITY ? q.g26.x : q.g.x, *gw = DENSITY ? q.g26.w : q.g.w;
     double res = 0.0;
-    if (!(dist_x > 2.0 * h)) {
+    if (!(dist_x > (DENSITY ? 1.2 * h : 2.0 * h))) {
         double dom_hi[3];
         for (int c = 0; c < 3; c++) dom_hi[c] = d.lo[c] + d.cs[c] * d.res[c];
-        for (int i = 0; i < 16; i++) {
-            const double wi = q.g.w[i];
-            for (int j = 0; j < 16; j++) {
-                const double wij = wi * q.g.w[j];
-                for (int k = 0; k < 16; k++) {
-                    const double wijk = wij * q.g.w[k];
-                    const double e[3] = {h * q.g.x[i], h * q.g.x[j], h * q.g.x[k]};
+        for (int i = 0; i < NQ; i++) {
+            const double wi = gw[i];
+            for (int j = 0; j < NQ; j++) {
+                const double wij = wi * gw[j];
+                for (int k = 0; k < NQ; k++) {
+                    const double wijk = wij * gw[k];
+                    const double e[3] = {h * gx[i], h * gx[j], h * gx[k]};
+                    const double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
                     double f = 0.0;
-                    if (!(((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]) > h * h)) {
+                    if (!(e2 > h * h)) {
                         const double y[3] = {x[0] + e[0], x[1] + e[1], x[2] + e[2]};
                         bool indom = true;
                         for (int c = 0; c < 3; c++)
                             if (!(y[c] >= d.lo[c] && y[c] <= dom_hi[c])) indom = false;
                         if (indom) {
                             const double dist = MESH ? interp_sdf(d, y) : shape_sdf(y, d);
-                            if (dist <= 0.0)
+                            if (DENSITY) {
+                                if (!(dist > h / 5.0)) f = (1.0 - 5.0 * dist / h) * cubic_W_d(sqrt(e2), h, q.k);
+                            } else if (dist <= 0.0) {
                                 f = 1.0;
-                            else if (dist < h)
+                            } else if (dist < h) {
                                 f = cubic_W_d(dist, h, q.k) / q.W0;
+                            }
                         }
                     }
                     res += wijk * f;
@@ -295,7 +308,7 @@ __global__ __launch_bounds__(BLOCK) void k_vm_cells(VMapDesc d, NodeCounts nc, u
     vol_cell[t] = vol_node[g];
 }
 
-void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], double h) {
+void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], double h, bool density) {
     SPH_REQUIRE(res[0] > 0 && res[1] > 0 && res[2] > 0, SPH_ERR_INVALID, "volume map: resolution must be > 0");
     SPH_REQUIRE(shape.kind == SHAPE_BOX || shape.kind == SHAPE_SPHERE || shape.kind == SHAPE_MESH, SPH_ERR_INVALID,
                 "volume map: unknown shape");
@@ -331,7 +344,9 @@ void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], 
     sdf.reserve(ncells * 32);
     vol.reserve(ncells * 32);
     QuadArgs q;
-    q.g = gauss16();
+    q.g = gauss_rule<16>();
+    q.g26 = gauss_rule<26>();
+    d.density = density ? 1 : 0;
     q.h = h;
     const double pi = 3.14159265358979323846;
     q.k = 8.0 / (pi * h * h * h);
@@ -350,9 +365,14 @@ void VolumeMap::build(hipStream_t s, const Shape &shape, const unsigned res[3], 
                                                                            sdf.get(), vol.get());
         SPH_HIP(hipGetLastError());
         d.sdf = sdf.get();
-        k_vm_nodes<true><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+        if (density)
+            k_vm_nodes<true, true><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+        else
+            k_vm_nodes<true, false><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+    } else if (density) {
+        k_vm_nodes<false, true><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
     } else {
-        k_vm_nodes<false><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
+        k_vm_nodes<false, false><<<gn, BLOCK, 0, s>>>(d, nc, total, q, sn.get(), vn.get());
     }
     SPH_HIP(hipGetLastError());
     k_vm_cells<<<(unsigned)((nt + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(d, nc, ncells, sn.get(), vn.get(), sdf.get(),
@@ -529,22 +549,123 @@ void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, 
 }
 
 __global__ __launch_bounds__(BLOCK) void k_vm_pack(unsigned n, unsigned stride, unsigned nobj,
-                                                   const float4 *__restrict__ raw, float4 *__restrict__ vmb) {
+                                                   const float4 *__restrict__ raw, float4 *__restrict__ vmb,
+                                                   const float4 *__restrict__ raw2, float4 *__restrict__ vmb2) {
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned k = 0;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     for (unsigned o = 0; o < nobj; o++) {
         const float4 e = raw[(size_t)o * stride + i];
-        if (e.w > 0.f) vmb[(size_t)(k++) * stride + i] = e;
+        if (e.w > 0.f) {
+            if (raw2) vmb2[(size_t)k * stride + i] = raw2[(size_t)o * stride + i];
+            vmb[(size_t)(k++) * stride + i] = e;
+        }
     }
-    for (; k < nobj; k++) vmb[(size_t)k * stride + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (; k < nobj; k++) {
+        vmb[(size_t)k * stride + i] = z;
+        if (raw2) vmb2[(size_t)k * stride + i] = z;
+    }
 }
 
 void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
-                      KernelTimer &T) {
+                      KernelTimer &T, const float4 *raw2, float4 *vmb2) {
     if (!n) return;
     TimedScope ts(T, s, "volume_map_pack");
-    k_vm_pack<<<nblocks(n), BLOCK, 0, s>>>(n, stride, nobj, raw, vmb);
+    k_vm_pack<<<nblocks(n), BLOCK, 0, s>>>(n, stride, nobj, raw, vmb, raw2, vmb2);
+    SPH_HIP(hipGetLastError());
+}
+
+// Field f (0 distance, 1 volume or density) and its gradient at world point
+// xi; false outside the map's domain.
+__device__ __forceinline__ bool interp_field(const VMapDesc &d, const double *xi, int f, double *val, double *grad) {
+    for (int c = 0; c < 3; c++) {
+        const double hi = d.lo[c] + d.cs[c] * d.res[c];
+        if (!(xi[c] >= d.lo[c] && xi[c] <= hi)) return false;
+    }
+    unsigned mi[3];
+    double loc[3], c0[3];
+    for (int c = 0; c < 3; c++) {
+        mi[c] = (unsigned)((xi[c] - d.lo[c]) * d.inv[c]);
+        if (mi[c] >= d.res[c]) mi[c] = d.res[c] - 1;
+        const double smin = d.lo[c] + d.cs[c] * mi[c], smax = smin + d.cs[c];
+        const double denom = smax - smin;
+        c0[c] = 2.0 / denom;
+        loc[c] = c0[c] * xi[c] - (smax + smin) / denom;
+    }
+    const unsigned long long cell = ((unsigned long long)mi[2] * d.res[1] + mi[1]) * d.res[0] + mi[0];
+    double N[32], dN[32][3];
+    shape32(loc, N, dN);
+    const float4 *rec = reinterpret_cast<const float4 *>((f == 0 ? d.sdf : d.vol) + 32ull * cell);
+    double phi = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const float4 c4 = rec[q];
+        const float cv[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int m = 4 * q + u;
+            phi += (double)cv[u] * N[m];
+            gx += (double)cv[u] * dN[m][0];
+            gy += (double)cv[u] * dN[m][1];
+            gz += (double)cv[u] * dN[m][2];
+        }
+    }
+    *val = phi;
+    grad[0] = gx * c0[0];
+    grad[1] = gy * c0[1];
+    grad[2] = gz * c0[2];
+    return true;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_dm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,
+                                                     float4 *__restrict__ v, float4 *__restrict__ xc,
+                                                     float4 *__restrict__ rec, float4 *__restrict__ grad, const double *dtp,
+                                                     double supportRadius, double particleRadius) {
+    const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    rec[i] = z;
+    grad[i] = z;
+    const float4 xf = x[i];
+    const double xi[3] = {(double)xf.x, (double)xf.y, (double)xf.z};
+    double dist = DBL_MAX, normal[3] = {0.0, 0.0, 0.0};
+    double val;
+    if (interp_field(d, xi, 0, &val, normal)) dist = val;
+    if (dist > 0.1 * particleRadius && dist < supportRadius) {
+        double gd[3], dv;
+        interp_field(d, xi, 1, &dv, gd);
+        if (dv > 1e-6 && dv != DBL_MAX) {
+            const double nl = sqrt((normal[0] * normal[0] + normal[1] * normal[1]) + normal[2] * normal[2]);
+            if (nl > 1.0e-6) {
+                const double dd = dist + 0.5 * supportRadius;
+                rec[i] = make_float4((float)(xi[0] - dd * normal[0] / nl), (float)(xi[1] - dd * normal[1] / nl),
+                                     (float)(xi[2] - dd * normal[2] / nl), (float)dv);
+                grad[i] = make_float4((float)-gd[0], (float)-gd[1], (float)-gd[2], 0.f);
+            }
+        }
+    } else if (dist <= 0.1 * particleRadius) {
+        const double nl = sqrt((normal[0] * normal[0] + normal[1] * normal[1]) + normal[2] * normal[2]);
+        if (nl > 1.0e-5) {
+            const double nx = normal[0] / nl, ny = normal[1] / nl, nz = normal[2] / nl;
+            const double dd = fmin(-dist, (0.25 / 0.005) * particleRadius * (*dtp));
+            x[i] = make_float4((float)(xi[0] + dd * nx), (float)(xi[1] + dd * ny), (float)(xi[2] + dd * nz), xf.w);
+            xc[i] = z;
+            float4 w = v[i];
+            const double vn = 0.05 - (((double)w.x * nx + (double)w.y * ny) + (double)w.z * nz);
+            w.x = (float)((double)w.x + vn * nx);
+            w.y = (float)((double)w.y + vn * ny);
+            w.z = (float)((double)w.z + vn * nz);
+            v[i] = w;
+        }
+    }
+}
+
+void sample_density_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *rec,
+                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T) {
+    if (!n) return;
+    TimedScope ts(T, s, "density_map_sample");
+    k_dm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, rec, grad, dt, supportRadius, particleRadius);
     SPH_HIP(hipGetLastError());
 }
 

@@ -53,7 +53,8 @@ struct VMapDesc {
     const double *tri;  // mesh triangles (device), 9 doubles each
     unsigned ntri;
     const float *sdf;  // ncells x 32
-    const float *vol;  // ncells x 32
+    const float *vol;  // ncells x 32: volume (Bender2019) or boundary density (Koschier2017)
+    int density;       // field 1 is a Koschier2017 density map
 };
 
 class VolumeMap {
@@ -63,8 +64,9 @@ public:
     DBuf<double> tri;
     size_t ncells = 0;
     bool built = false;
-    // Builds both fields on the device (synchronous).
-    void build(hipStream_t s, const Shape &shape, const unsigned res[3], double supportRadius);
+    // Builds both fields on the device (synchronous); field 1 is the volume
+    // (Bender2019) or, with `density`, the boundary density (Koschier2017).
+    void build(hipStream_t s, const Shape &shape, const unsigned res[3], double supportRadius, bool density = false);
     // Copies cell records (32 sdf values then 32 volume values, as floats)
     // of the given cells to host memory.
     void cell_values(const uint64_t *cells, size_t n, float *out) const;
@@ -78,10 +80,20 @@ public:
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
                        const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T);
 
+// TimeStep::computeDensityAndGradient (TimeStep.cpp:349-468) for one static
+// density map: rec[i] = (x_b, rho_b) with x_b = x_i - (dist + h/2) n,
+// grad[i] = (grad rho_b, 0) where 0.1 r < dist < h and rho_b > 1e-6, else
+// zero; particles with dist <= 0.1 r are moved towards the surface by at
+// most 10 r h_t / 1 s (h_t the time step) and get their normal velocity set
+// to 0.05 m/s. Every particle is sampled, whatever its state (the
+// reference's loop has no state check). dt points at the time step size.
+void sample_density_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *rec,
+                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T);
+
 // Packs, per particle, the records with V_b > 0 of `nobj` object-major
 // arrays (raw[o * stride + i]) into the leading slots of vmb (same layout),
 // keeping object order, and zeroes the remaining slots.
 void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
-                      KernelTimer &T);
+                      KernelTimer &T, const float4 *raw2 = nullptr, float4 *vmb2 = nullptr);
 
 }  // namespace sph

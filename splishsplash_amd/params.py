@@ -7,6 +7,7 @@ import ctypes as C
 
 BOUNDARY_NONE = -1
 BOUNDARY_AKINCI2012 = 0
+BOUNDARY_KOSCHIER2017 = 1
 BOUNDARY_BENDER2019 = 2
 VISCOSITY_NONE = 0
 VISCOSITY_STANDARD = 1
@@ -145,6 +146,6 @@ FIELD_COMPONENTS = {
     "position": 3, "velocity": 3, "acceleration": 3, "density": 1,
     "factor": 1, "advected density": 1, "p / rho^2": 1, "p_v / rho^2": 1,
     "pressure acceleration": 3, "velocity difference": 3,
-    "boundary volume": 1, "boundary x": 3,
+    "boundary volume": 1, "boundary x": 3, "boundary density": 1, "boundary density gradient": 3,
     "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1, "state": 1,
 }

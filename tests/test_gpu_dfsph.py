@@ -435,3 +435,46 @@ def test_volume_map_mesh_matches_oracle():
     g.set("p / rho^2", p)
     g.set("p_v / rho^2", pv)
     run_stage_sequence(g, o, STEP_SEQ, extra_state=tuple(names[2:]))
+
+
+@pytest.mark.parametrize("visc", ["standard", "weiler"])
+def test_density_map_stages_match_oracle(visc):
+    # Koschier2017: the box wall and a sphere obstacle as density maps; the
+    # sampling (boundary density, its gradient, the boundary point, particles
+    # moved out of the wall) and every stage with the density-map terms
+    # (density, factor, density change, pressure accelerations, a_ij p_j,
+    # XSPH, standard or Weiler2018 boundary viscosity) from identical states
+    from splishsplash_amd.params import BOUNDARY_KOSCHIER2017
+    x, params, vm = vm_scene()
+    params.boundary_handling_method = BOUNDARY_KOSCHIER2017
+    if visc == "standard":
+        params.viscosity_method = VISCOSITY_STANDARD
+        params.viscosity = 0.01
+        params.viscosity_boundary = 0.02
+    else:
+        params.viscosity_method = VISCOSITY_WEILER2018
+        params.viscosity = 0.05
+        params.viscosity_boundary = 0.05
+        params.visco_max_error = 1e-5
+        params.visco_max_iter = 200
+    spheres = [((0.45, 0.45, 0.45), 0.12)]
+    v, p, pv = perturbed_state(x, seed=6)
+    g, o = make_pair(params, x, v, vm=vm, spheres=spheres)
+    g.stage("neighborhood_search")
+    o.stage("neighborhood_search")
+    sync_state(g, o)
+    g.stage("volume_maps")
+    o.stage("volume_maps")
+    names = ["position", "velocity"]
+    for k in range(2):
+        names += [f"boundary density {k}", f"boundary density gradient {k}", f"boundary x {k}"]
+    for nm in names:
+        e = rel(g.get(nm), o.get(nm))
+        assert e <= TOL, f"volume_maps: {nm} rel err {e:.3e}"
+    assert (o.get("boundary density 0") > 0).sum() > 500 and (o.get("boundary density 1") > 0).sum() > 20
+    assert (g.get("position") != x).any(axis=1).sum() >= 3  # particles in the wall and the sphere moved out
+    g.set("p / rho^2", p)
+    g.set("p_v / rho^2", pv)
+    seq = STEP_SEQ[:6] + ["clear_accel", "viscosity", "xsph"] + STEP_SEQ[8:]
+    tol = {"viscosity": 1e-4} if visc == "weiler" else None
+    run_stage_sequence(g, o, seq, extra_state=tuple(names[2:]), tol=tol)

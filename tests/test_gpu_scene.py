@@ -44,9 +44,23 @@ def test_scene_file_runs_like_the_hand_built_handle():
     assert a.get("position")[:, 1].mean() < x[:, 1].mean()  # the column collapses
 
 
-def test_scene_file_step_matches_oracle():
+@pytest.mark.parametrize("bhm", [2, 1])
+def test_scene_file_step_matches_oracle(bhm, tmp_path):
+    # the packaged dam break as is (Bender2019 volume maps) and with
+    # Koschier2017 density maps for the same bodies
+    import json
+    import shutil
     path = os.path.join(DATA, "dam_break.json")
+    if bhm != 2:
+        scene = json.load(open(path))
+        scene["Configuration"]["boundaryHandlingMethod"] = bhm
+        for f in os.listdir(DATA):
+            if f.endswith(".obj"):
+                shutil.copy(os.path.join(DATA, f), tmp_path / f)
+        path = str(tmp_path / "scene.json")
+        json.dump(scene, open(path, "w"))
     sc = capi.Scene(path)
+    assert sc.params.boundary_handling_method == bhm
     g = sc.create_dfsph()
     g.step(40)
     o = oracle.OracleDFSPH(sc.params)

@@ -147,7 +147,7 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
     ({"Configuration": {"simulationMethod": 1}}, "simulationMethod"),
     ({"Configuration": {"sim2D": True}}, "sim2D"),
     ({"Configuration": {"kernel": 0}}, "kernel"),
-    ({"Configuration": {"boundaryHandlingMethod": 1}}, "boundaryHandlingMethod"),
+    ({"Configuration": {"boundaryHandlingMethod": 3}}, "boundaryHandlingMethod"),
     ({"Materials": [{"id": "Fluid", "viscosityMethod": 3}]}, "viscosityMethod"),
     ({"Materials": [{"id": "Fluid", "surfaceTensionMethod": 1}]}, "surfaceTensionMethod"),
     ({"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]},
