@@ -55,6 +55,8 @@ extern "C" {
 #define SPH_BOUNDARY_AKINCI2012  0
 #define SPH_BOUNDARY_KOSCHIER2017 1  /* density maps of the boundary objects */
 #define SPH_BOUNDARY_BENDER2019  2
+#define SPH_METHOD_PCISPH        1   /* Simulation simulationMethod values */
+#define SPH_METHOD_DFSPH         4
 #define SPH_VISCOSITY_NONE       0
 #define SPH_VISCOSITY_STANDARD   1   /* Viscosity_Standard (explicit)        */
 #define SPH_VISCOSITY_WEILER2018 6
@@ -300,6 +302,16 @@ int sph_dfsph_reset(sph_dfsph_t s);
  * particles keep their state. */
 int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v,
                              size_t n);
+/* Simulation::setSimulationMethod (Simulation.cpp:528-580) for this handle:
+ * SPH_METHOD_DFSPH (default) or SPH_METHOD_PCISPH, TimeStepPCISPH::step
+ * (PCISPH/TimeStepPCISPH.cpp:80-128): the predictor-corrector pressure loop
+ * with the scaling factor of SimulationDataPCISPH::init, using the
+ * parameters' min_iterations, max_iterations and max_error (PCISPH's own
+ * defaults are 3, 100 and 0.01 %). The neighbour search, boundaries,
+ * non-pressure forces, CFL, states and emitters are shared. Call before the
+ * first step or between steps. */
+int sph_dfsph_set_simulation_method(sph_dfsph_t s, int method);
+
 /* Emitter system (SPlisHSPlasH/EmitterSystem.h, Emitter.h). An emitter
  * places a width x height grid of particles (type 0, box) or a disc of width
  * particles across (type 1, circle) at translation + (t - next + h) v along
@@ -454,7 +466,8 @@ typedef struct sph_scene *sph_scene_t;
 /* SceneLoader::readScene (SPlisHSPlasH/Utilities/SceneLoader.cpp:13-150)
  * for what this time step runs. Read: "Configuration" (particleRadius,
  * timeStepSize, gravitation, cflMethod, cflFactor, cflMin/MaxTimeStepSize,
- * boundaryHandlingMethod and the "DFSPH" section), the fluid's "Materials"
+ * boundaryHandlingMethod, simulationMethod 4 (DFSPH) with the "DFSPH"
+ * section or 1 (PCISPH) with the "PCISPH" section), the fluid's "Materials"
  * entry (density0, viscosityMethod 0/1/6 with its "Standard viscosity" or
  * "Weiler et al. 2018" section, the "XSPH" section), "FluidBlocks" sampled
  * as SimulatorBase::createFluidBlocks (Simulator/SimulatorBase.cpp:

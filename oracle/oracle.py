@@ -52,6 +52,8 @@ def lib():
             "orc_set_emitter_reuse": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
             "orc_set_max_emitter_particles": (C.c_int, [C.c_void_p, C.c_size_t]),
             "orc_emitter_counts": (C.c_int, [C.c_void_p, _szp, _szp]),
+            "orc_set_simulation_method": (C.c_int, [C.c_void_p, C.c_int]),
+            "orc_pcisph_factor": (C.c_int, [C.c_void_p, _dp]),
             "orc_init": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
@@ -292,6 +294,14 @@ class OracleDFSPH:
         e, r = C.c_size_t(0), C.c_size_t(0)
         _chk(lib().orc_emitter_counts(self.h, C.byref(e), C.byref(r)))
         return e.value, r.value
+
+    def set_simulation_method(self, m):
+        _chk(lib().orc_set_simulation_method(self.h, int(m)))
+
+    def pcisph_factor(self):
+        f = C.c_double(0.0)
+        _chk(lib().orc_pcisph_factor(self.h, C.byref(f)))
+        return f.value
 
     def _refresh_n(self):
         n = C.c_size_t(0)

@@ -651,6 +651,21 @@ Sim::Sim(const sph_dfsph_params_t &p) : P(p) {
     mass = V * P.density0;
     K.set(hs);
     dt = P.time_step_size;
+    // SimulationDataPCISPH::init (SimulationDataPCISPH.cpp:40-108): lattice
+    // of spacing 2r around the origin, the precomputed kernel's gradient
+    double sg[3] = {0, 0, 0}, sg2 = 0.0;
+    for (double a0 = -hs; a0 <= hs; a0 += diam)
+        for (double a1 = -hs; a1 <= hs; a1 += diam)
+            for (double a2 = -hs; a2 <= hs; a2 += diam) {
+                const double d[3] = {-a0, -a1, -a2};
+                if (sqn3(d) < hs * hs) {
+                    double g[3];
+                    K.gradW(d, g);
+                    for (int c = 0; c < 3; c++) sg[c] += g[c];
+                    sg2 += sqn3(g);
+                }
+            }
+    pciFactor = 1.0 / (2.0 * V * V * (sqn3(sg) + sg2));
 }
 
 void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
@@ -666,6 +681,7 @@ void Sim::set_fluid(const double *xx, const double *vv, size_t nn) {
     ap.assign(3 * n, 0.0);
     vdiff.assign(3 * n, 0.0);
     state.assign(n, 0.0);
+    predX.assign(3 * n, 0.0);  // SimulationDataPCISPH::init
     vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
     vmX.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
     vmG.assign(3 * std::max<size_t>(vms.size(), 1) * n, 0.0);
@@ -742,20 +758,23 @@ double Sim::map_volume(size_t o, size_t i) const {
 // density map: rho_b and grad rho_b = -grad field 1 where 0.1 r < d < h,
 // the boundary point x_i - (d + h/2) n; a particle at d <= 0.1 r is moved
 // out by at most 10 r dt / 1 s and its normal velocity set to 0.05 m/s.
-void Sim::density_map(size_t o) {
+void Sim::density_map(size_t o, const double *xin) {
     VolumeMap *vm = vms[o].get();
+    const double *X = xin ? xin : x.data();
     std::vector<uint64_t> cells;
     cells.reserve(n);
     for (size_t i = 0; i < n; i++) {
         uint64_t c;
         double xi[3];
-        if (vm->locate(&x[3 * i], c, xi)) cells.push_back(c);
+        if (vm->locate(&X[3 * i], c, xi)) cells.push_back(c);
     }
     vm->prefetch(cells);
     const double supportRadius = hs, particleRadius = r;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
-        double *xi = &x[3 * i];
+        if (xin && !active(i)) continue;  // PCISPH's prediction visits active particles only
+        const double *xi = &X[3 * i];
+        double *xw = &x[3 * i];
         double *xb = &vmX[3 * (o * n + i)];
         double *gb = &vmG[3 * (o * n + i)];
         double &rb = vmV[o * n + i];
@@ -783,7 +802,7 @@ void Sim::density_map(size_t o) {
                 for (int c = 0; c < 3; c++) normal[c] /= nl;
                 double d = -dist;
                 d = std::min(d, (0.25 / 0.005) * particleRadius * dt);
-                for (int c = 0; c < 3; c++) xi[c] += d * normal[c];
+                for (int c = 0; c < 3; c++) xw[c] = xi[c] + d * normal[c];
                 const double vn = 0.05 - dot3(&v[3 * i], normal);
                 for (int c = 0; c < 3; c++) v[3 * i + c] += vn * normal[c];
             }
@@ -791,22 +810,25 @@ void Sim::density_map(size_t o) {
     }
 }
 
-void Sim::volume_map(size_t o) {
+void Sim::volume_map(size_t o, const double *xin) {
     VolumeMap *vm = vms[o].get();
     double *vmVo = &vmV[o * n];
     double *vmXo = &vmX[3 * o * n];
+    const double *X = xin ? xin : x.data();
     std::vector<uint64_t> cells;
     cells.reserve(n);
     for (size_t i = 0; i < n; i++) {
         uint64_t c;
         double xi[3];
-        if (vm->locate(&x[3 * i], c, xi)) cells.push_back(c);
+        if (vm->locate(&X[3 * i], c, xi)) cells.push_back(c);
     }
     vm->prefetch(cells);
     const double supportRadius = hs, particleRadius = r;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; i++) {
-        double *xi = &x[3 * i];
+        if (xin && !active(i)) continue;  // PCISPH's prediction visits active particles only
+        const double *xi = &X[3 * i];
+        double *xw = &x[3 * i];
         double *xb = &vmXo[3 * i];
         xb[0] = xb[1] = xb[2] = 0.0;
         vmVo[i] = 0.0;
@@ -847,7 +869,7 @@ void Sim::volume_map(size_t o) {
                     for (int c = 0; c < 3; c++) normal[c] /= nl;
                     double delta = 2.0 * particleRadius - dist;
                     delta = std::min(delta, 0.1 * particleRadius);
-                    for (int c = 0; c < 3; c++) xi[c] = xi[c] + delta * normal[c];
+                    for (int c = 0; c < 3; c++) xw[c] = xi[c] + delta * normal[c];
                     v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0;
                 }
             }
@@ -1607,7 +1629,7 @@ void Sim::emitter_size(const Emitter &e, double *size) const {
 // Grows (or shrinks) every per-particle array to m particles; new entries are
 // zero. The per-object volume-map slices are recomputed every step.
 void Sim::resize_particles(size_t m) {
-    for (std::vector<double> *f : {&x, &v, &a, &ap, &vdiff}) f->resize(3 * m, 0.0);
+    for (std::vector<double> *f : {&x, &v, &a, &ap, &vdiff, &predX}) f->resize(3 * m, 0.0);
     for (std::vector<double> *f : {&rho, &factor, &rho_adv, &p, &pv, &state}) f->resize(m, 0.0);
     n = m;
     vmV.assign(std::max<size_t>(vms.size(), 1) * n, 0.0);
@@ -1681,6 +1703,8 @@ size_t Sim::emit(Emitter &e, const std::vector<size_t> &reusedList, size_t &inde
                     v[3 * index + c] = emitVel[c];
                 }
                 state[index] = 1.0;
+                if (!fromReuse)  // SimulationDataPCISPH::emittedParticles
+                    for (int c = 0; c < 3; c++) predX[3 * index + c] = x[3 * index + c];
                 if (fromReuse) {
                     indexReuse++;
                 } else {
@@ -1719,8 +1743,141 @@ void Sim::emitter_system_step() {
     reused += indexReuse;
 }
 
+// TimeStepPCISPH::step (TimeStepPCISPH.cpp:80-128).
+void Sim::step_pcisph() {
+    neighborhood_search();
+    volume_maps();
+    densities();
+    clear_accelerations();
+    viscosity_standard();
+    viscosity_weiler();
+    xsph();
+    cfl();
+    // pressureSolve (:131-186)
+    const double h = dt, invH2 = 1.0 / (h * h), density0 = P.density0;
+    for (size_t i = 0; i < n; i++) {
+        if (active(i))
+            for (int c = 0; c < 3; c++) v[3 * i + c] += h * a[3 * i + c];
+        p[i] = 0.0;
+        for (int c = 0; c < 3; c++) ap[3 * i + c] = 0.0;
+    }
+    iterations = 0;
+    bool chk = false;
+    double avg = 0.0;
+    while ((!chk || (iterations < P.min_iterations)) && (iterations < P.max_iterations)) {
+        // prediction and the boundary at the predicted positions (:196-216)
+        for (size_t i = 0; i < n; i++) {
+            if (!active(i)) continue;
+            for (int c = 0; c < 3; c++) {
+                const double pv = v[3 * i + c] + h * (a[3 * i + c] + ap[3 * i + c]);
+                predX[3 * i + c] = x[3 * i + c] + h * pv;
+            }
+        }
+        if (has_volume_map())
+            for (size_t o = 0; o < vms.size(); o++) volume_map(o, predX.data());
+        if (has_density_map())
+            for (size_t o = 0; o < vms.size(); o++) density_map(o, predX.data());
+        // density at the predicted positions and the pressure (:219-275)
+        std::vector<double> err(n);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; i++) {
+            const double *xi = &predX[3 * i];
+            double dens = V * K.W0;
+            for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+                const size_t j = nf.idx[k];
+                const double d[3] = {xi[0] - predX[3 * j], xi[1] - predX[3 * j + 1], xi[2] - predX[3 * j + 2]};
+                dens += V * K.W(d);
+            }
+            if (has_boundary_particles()) {
+                for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                    const uint32_t j = nbd.idx[k];
+                    const double d[3] = {xi[0] - bx[3 * j], xi[1] - bx[3 * j + 1], xi[2] - bx[3 * j + 2]};
+                    dens += bV[j] * K.W(d);
+                }
+            } else if (has_density_map()) {
+                for (size_t o = 0; o < vms.size(); o++) dens += vmV[o * n + i];
+            } else if (has_volume_map()) {
+                for (size_t o = 0; o < vms.size(); o++) {
+                    const double Vj = vmV[o * n + i];
+                    if (Vj > 0.0) {
+                        const double *xj = &vmX[3 * (o * n + i)];
+                        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                        dens += Vj * K.W(d);
+                    }
+                }
+            }
+            rho_adv[i] = dens;
+            err[i] = density0 * (dens - 1.0);
+            p[i] = std::max(p[i] + invH2 * pciFactor * (dens - 1.0), 0.0);
+        }
+        double e = 0.0;
+        for (size_t i = 0; i < n; i++) e += err[i];
+        avg = n ? e / (double)n : 0.0;
+        // pressure accelerations at the current positions (:280-333)
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; i++) {
+            const double *xi = &x[3 * i];
+            double *ai = &ap[3 * i];
+            ai[0] = ai[1] = ai[2] = 0.0;
+            const double dpi = p[i];
+            for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+                const size_t j = nf.idx[k];
+                const double d[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) ai[c] -= V * (dpi + p[j]) * g[c];
+            }
+            if (has_boundary_particles()) {
+                for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                    const uint32_t j = nbd.idx[k];
+                    const double d[3] = {xi[0] - bx[3 * j], xi[1] - bx[3 * j + 1], xi[2] - bx[3 * j + 2]};
+                    double g[3];
+                    K.gradW(d, g);
+                    for (int c = 0; c < 3; c++) ai[c] -= bV[j] * dpi * g[c];
+                }
+            } else if (has_density_map()) {
+                for (size_t o = 0; o < vms.size(); o++)
+                    if (vmV[o * n + i] != 0.0)
+                        for (int c = 0; c < 3; c++) ai[c] -= -dpi * vmG[3 * (o * n + i) + c];
+            } else if (has_volume_map()) {
+                for (size_t o = 0; o < vms.size(); o++) {
+                    const double Vj = vmV[o * n + i];
+                    if (Vj > 0.0) {
+                        const double *xj = &vmX[3 * (o * n + i)];
+                        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                        double g[3];
+                        K.gradW(d, g);
+                        for (int c = 0; c < 3; c++) ai[c] -= Vj * dpi * g[c];
+                    }
+                }
+            }
+        }
+        const double eta = P.max_error * 0.01 * density0;
+        chk = (avg <= eta);
+        iterations++;
+    }
+    lastErr = avg;
+    totIt += iterations;
+    iterationsV = 0;
+    // integration with the step's starting h (:104-122), emitters, time
+    for (size_t i = 0; i < n; i++) {
+        if (!active(i)) continue;
+        for (int c = 0; c < 3; c++) {
+            v[3 * i + c] += h_step * ap[3 * i + c];
+            x[3 * i + c] += h_step * v[3 * i + c];
+        }
+    }
+    emitter_system_step();
+    time += h_step;
+}
+
 // TimeStepDFSPH::step (TimeStepDFSPH.cpp:117-249).
 void Sim::step() {
+    if (method == SPH_METHOD_PCISPH) {
+        step_pcisph();
+        steps++;
+        return;
+    }
     neighborhood_search();
     volume_maps();
     densities();
@@ -1810,6 +1967,7 @@ bool field_of(Sim *s, const char *name, FieldRef &f) {
     else if (n == "pressure acceleration") f = {&s->ap, 3};
     else if (n == "velocity difference") f = {&s->vdiff, 3};
     else if (n == "state") f = {&s->state, 1};
+    else if (n == "predicted position") f = {&s->predX, 3};
     else if (n == "boundary particle volume") f = {&s->bV, 1};
     else return false;
     return true;
@@ -1955,6 +2113,15 @@ int orc_emitter_counts(void *h, size_t *emitted, size_t *reused) {
     Sim *s = (Sim *)h;
     if (emitted) *emitted = s->emitted;
     if (reused) *reused = s->reused;
+    return SPH_OK;
+}
+int orc_pcisph_factor(void *h, double *out) {
+    *out = ((Sim *)h)->pciFactor;
+    return SPH_OK;
+}
+int orc_set_simulation_method(void *h, int m) {
+    if (m != SPH_METHOD_DFSPH && m != SPH_METHOD_PCISPH) return fail("simulation method: PCISPH (1) or DFSPH (4)");
+    ((Sim *)h)->method = m;
     return SPH_OK;
 }
 int orc_init(void *h) {

@@ -174,8 +174,13 @@ public:
 
     void neighborhood_search();
     void volume_maps();
-    void volume_map(size_t o);
-    void density_map(size_t o);
+    void volume_map(size_t o, const double *xin = nullptr);
+    void density_map(size_t o, const double *xin = nullptr);
+    // PCISPH (TimeStepPCISPH, SimulationDataPCISPH)
+    int method = SPH_METHOD_DFSPH;
+    std::vector<double> predX;
+    double pciFactor = 0.0;
+    void step_pcisph();
     void densities();
     void dfsph_factor();
     void divergence_setup();

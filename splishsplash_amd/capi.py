@@ -90,6 +90,7 @@ _SIGS = {
     "sph_scene_create_dfsph": (C.c_int, [_vp, C.POINTER(_vp)]),
     "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_emitter": (C.c_int, [_vp, C.c_void_p]),
+    "sph_dfsph_set_simulation_method": (C.c_int, [_vp, C.c_int]),
     "sph_dfsph_set_emitter_reuse": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "sph_dfsph_set_max_emitter_particles": (C.c_int, [_vp, C.c_size_t]),
     "sph_dfsph_emitter_counts": (C.c_int, [_vp, _szp, _szp]),
@@ -549,6 +550,11 @@ class TimeStepDFSPH:
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
         _chk(lib().sph_dfsph_emit_particles(self.h, x.ctypes.data_as(_fp), v.ctypes.data_as(_fp), len(x)))
         self.n += len(x)
+
+    def set_simulation_method(self, method):
+        """Simulation::setSimulationMethod for this handle: METHOD_DFSPH (4)
+        or METHOD_PCISPH (1)."""
+        _chk(lib().sph_dfsph_set_simulation_method(self.h, int(method)))
 
     def add_emitter(self, width, height, pos, rotation, velocity, type=0, emit_start_time=0.0,
                     emit_end_time=float(np.finfo(np.float64).max)):

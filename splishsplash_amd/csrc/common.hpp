@@ -95,6 +95,7 @@ template <class T> struct DBuf {
 
 // ------------------------------------------------------------- launching
 constexpr int BLOCK = 256;
+constexpr int RBLOCK = 1024;  // one-block reductions
 inline unsigned nblocks(size_t n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
 
 // Workgroups of one dispatch are spread round-robin over the 8 XCDs of the

@@ -22,7 +22,6 @@
 
 namespace sph {
 
-constexpr int RBLOCK = 1024;  // one-block reductions
 
 __device__ __forceinline__ unsigned gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
 
@@ -1607,6 +1606,7 @@ DFSPH::DFSPH(const sph_dfsph_params_t &p) : P(p) {
     V = 0.8 * diam * diam * diam;
     mass = V * P.density0;
     K = make_kernel_const(hs);
+    pci_factor_ = pcisph_factor(r, V);
     SPH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     SPH_HIP(hipHostMalloc((void **)&hctl_, sizeof(Ctl), hipHostMallocDefault));
     std::memset(hctl_, 0, sizeof(Ctl));
@@ -1801,6 +1801,10 @@ void DFSPH::alloc_particles() {
         pst_.reserve(S);
         pst2_.reserve(S);
     }
+    if (method_ == SPH_METHOD_PCISPH) {
+        predx_.reserve(S);
+        predx2_.reserve(S);
+    }
     cf_.reserve(S);
     cb_.reserve(S);
     partial_.reserve(12 * (size_t)nblocks(S) + 64);  // up to three values per block of up to 4 lanes per particle
@@ -1846,6 +1850,9 @@ void DFSPH::upload_particles(size_t n, const float *x, const float *v, const flo
         const std::vector<unsigned> active(n, 0u);
         SPH_HIP(hipMemcpy(pst_.get(), active.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
     }
+    // PCISPH's predicted positions start at zero (SimulationDataPCISPH::init)
+    if (method_ == SPH_METHOD_PCISPH)
+        SPH_HIP(hipMemcpy(predx_.get(), zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
 }
 
 void DFSPH::init() {
@@ -2022,40 +2029,49 @@ void DFSPH::neighborhood_search() {
             permute_u1(stream, perm, n, pst_.get(), pst2_.get());
             pst_.swap(pst2_);
         }
+        if (method_ == SPH_METHOD_PCISPH) {
+            permute_f4(stream, perm, n, predx_.get(), predx2_.get());
+            predx_.swap(predx2_);
+        }
     }
     build_lists();
 }
 
-void DFSPH::volume_maps() {
+// Boundary records of every map object. With `xin` the particles are
+// sampled at those positions (PCISPH's predicted positions), `skip` leaves
+// particles with a non-zero entry alone and `stop` makes the launches no-ops
+// once a solve has converged.
+void DFSPH::volume_maps(const float4 *xin, const unsigned *skip, const int *stop) {
     if (!maps() || nvm_ == 0) return;
+    const unsigned *st = (states_ && !skip) ? pst_.get() : nullptr;
     if (P.boundary_handling_method == SPH_BOUNDARY_KOSCHIER2017) {
         // TimeStep::computeDensityAndGradient (TimeStep.cpp:349-468), objects
         // in insertion order as for the volume maps
         const double *dt = &ctl_.get()->h;
         if (nvm_ == 1) {
             sample_density_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(),
-                               dmg_.get(), dt, hs, r, timer);
+                               dmg_.get(), dt, hs, r, timer, xin, skip, stop);
             return;
         }
         for (unsigned o = 0; o < nvm_; o++)
             sample_density_map(stream, vmaps[o]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(),
                                vmraw_.get() + (size_t)o * stride_, dmgraw_.get() + (size_t)o * stride_, dt, hs, r,
-                               timer);
+                               timer, xin, skip, stop);
         pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer, dmgraw_.get(),
-                         dmg_.get());
+                         dmg_.get(), stop);
         return;
     }
     // objects in insertion order: a particle projected out of one object is
     // sampled against the next at its new position (TimeStep.cpp:177-222)
     if (nvm_ == 1) {
-        sample_volume_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(),
-                          states_ ? pst_.get() : nullptr, hs, r, timer);
+        sample_volume_map(stream, vmaps[0]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(), vmb_.get(), st, hs, r,
+                          timer, xin, skip, stop);
         return;
     }
     for (unsigned o = 0; o < nvm_; o++)
         sample_volume_map(stream, vmaps[o]->d, (unsigned)n_, x_.get(), v_.get(), xc_.get(),
-                          vmraw_.get() + (size_t)o * stride_, states_ ? pst_.get() : nullptr, hs, r, timer);
-    pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer);
+                          vmraw_.get() + (size_t)o * stride_, st, hs, r, timer, xin, skip, stop);
+    pack_volume_maps(stream, (unsigned)n_, stride_, nvm_, vmraw_.get(), vmb_.get(), timer, nullptr, nullptr, stop);
 }
 
 #define LAUNCH(name, kern, ...)                                   \
@@ -2527,8 +2543,19 @@ void DFSPH::viscosity(bool apply_separately) {
     if (apply_separately) post_forces(false, true, 0, false, false, false);
 }
 
+// m / rho into the w of the fluid positions for the non-pressure forces.
+void DFSPH::mrho_copy() {
+    const Dev D = dev();
+    LAUNCH("mass_density_copy", k_mrho_to_w, D);
+    mrho_w_ = true;
+}
+
 void DFSPH::step() {
     if (!initialized_) init();
+    if (method_ == SPH_METHOD_PCISPH) {
+        step_pcisph();
+        return;
+    }
     neighborhood_search();
     volume_maps();
     const bool div = P.enable_divergence_solver != 0;
@@ -2545,11 +2572,7 @@ void DFSPH::step() {
     }
     const bool weiler = P.viscosity_method == SPH_VISCOSITY_WEILER2018;
     const bool stdv = P.viscosity_method == SPH_VISCOSITY_STANDARD;
-    if ((weiler || stdv || P.xsph != 0.0) && n_) {
-        const Dev D = dev();
-        LAUNCH("mass_density_copy", k_mrho_to_w, D);
-        mrho_w_ = true;
-    }
+    if ((weiler || stdv || P.xsph != 0.0) && n_) mrho_copy();
     if (weiler) viscosity(false);
     const int expl = ((P.xsph != 0.0 || P.xsph_boundary != 0.0) ? 1 : 0) | (stdv ? 2 : 0);
     post_forces(true, weiler, expl, P.cfl_method == 1 || P.cfl_method == 2, true, div);
@@ -2674,6 +2697,10 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
     else if (name == "acceleration") f4 = a_.get(), comps = 3;
     else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
     else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
+    else if (name == "predicted position") {
+        SPH_REQUIRE(method_ == SPH_METHOD_PCISPH, SPH_ERR_NOTFOUND, "predicted positions exist with PCISPH only");
+        f4 = predx_.get(), comps = 3;
+    }
     else if (name.rfind("boundary x", 0) == 0 || name.rfind("boundary volume", 0) == 0 ||
              name.rfind("boundary density", 0) == 0) {
         // "boundary x" / "boundary volume" of map object 0, or "... <o>";
@@ -2751,6 +2778,7 @@ void DFSPH::set_field(const std::string &name, const void *host, size_t bytes) {
     else if (name == "acceleration") f4 = a_.get(), comps = 3;
     else if (name == "pressure acceleration") f4 = ap_.get(), comps = 3;
     else if (name == "velocity difference") f4 = vdiff_.get(), comps = 3;
+    else if (name == "predicted position" && method_ == SPH_METHOD_PCISPH) f4 = predx_.get(), comps = 3;
     else if (name == "density") f1 = rho_.get();
     else if (name == "factor") f1 = alpha_.get();
     else if (name == "advected density") f1 = radv_.get();

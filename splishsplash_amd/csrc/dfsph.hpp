@@ -124,6 +124,10 @@ public:
     const std::vector<EmitterDesc> &emitters() const { return emitters_; }
     void step();
     void stage(int s);
+    // Simulation::setSimulationMethod for this handle: SPH_METHOD_DFSPH or
+    // SPH_METHOD_PCISPH (TimeStepPCISPH, pcisph.hip)
+    void set_method(int m);
+    int method() const { return method_; }
 
     void get_field(const std::string &name, void *host, size_t bytes);
     void set_field(const std::string &name, const void *host, size_t bytes);
@@ -208,7 +212,7 @@ private:
     void read_ctl();
     void neighborhood_search();
     void build_lists();
-    void volume_maps();
+    void volume_maps(const float4 *xin = nullptr, const unsigned *skip = nullptr, const int *stop = nullptr);
     void sweep_init(bool density, bool factor, bool div_setup);
     template <int LPP>
     void sweep_init_l(bool density, bool factor, bool div_setup);
@@ -227,6 +231,12 @@ private:
     // sweeps of a step write it themselves)
     int run_solver(bool pressure, int already = 0, bool copy_w = true);
     bool mrho_w_ = false;  // position w holds m / rho (within a step's non-pressure forces)
+    void mrho_copy();
+    // PCISPH
+    int method_ = SPH_METHOD_DFSPH;
+    double pci_factor_ = 0.0;
+    DBuf<float4> predx_, predx2_;  // predicted positions (persistent, sorted with the particles)
+    void step_pcisph();
     void stiff_to_w(bool pressure);
     template <bool PRESSURE>
     void iteration_any(bool force);
@@ -248,6 +258,10 @@ private:
 
 // Particle files (io.hip): the exporter's particle file, and the fluid part of
 // a state file, in the partio BGEO format.
+// PCISPH scaling factor of a particle radius r and volume V
+// (SimulationDataPCISPH.cpp:40-108).
+double pcisph_factor(double r, double V);
+
 void write_particles(DFSPH &s, const std::string &path, const char *fields);
 void write_state(DFSPH &s, const std::string &path);
 void read_state(DFSPH &s, const std::string &path);

@@ -134,6 +134,13 @@ int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v, size
     });
 }
 
+int sph_dfsph_set_simulation_method(sph_dfsph_t s, int method) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_method(method);
+    });
+}
+
 int sph_dfsph_add_emitter(sph_dfsph_t s, const sph_emitter_t *e) {
     return guarded([&] {
         DCHK(s);

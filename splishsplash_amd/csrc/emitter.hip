@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit_scatter(const unsigned *__restri
                                                         const float4 *__restrict__ ex, const float4 *__restrict__ ev,
                                                         unsigned k, unsigned n_old, unsigned state, float4 *x,
                                                         float4 *v, float4 *xc, float4 *vdiff, float *p, float *pv,
-                                                        unsigned *id, unsigned *st) {
+                                                        unsigned *id, unsigned *st, float4 *predx) {
     const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= k) return;
     const unsigned s = slots[q];
@@ -124,6 +124,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit_scatter(const unsigned *__restri
         p[s] = 0.f;
         pv[s] = 0.f;
         vdiff[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (predx) predx[s] = ex[q];  // SimulationDataPCISPH::emittedParticles
     }
 }
 
@@ -202,6 +203,10 @@ void DFSPH::grow_particles(size_t m) {
         pst_.grow_keep(S, n);
         pst2_.reserve(S);
     }
+    if (method_ == SPH_METHOD_PCISPH) {
+        predx_.grow_keep(S, n);
+        predx2_.reserve(S);
+    }
     stride_ = (unsigned)S;
     alloc_particles();
     have_lists_ = false;
@@ -244,7 +249,8 @@ void DFSPH::emit_particles(const float *x, const float *v, size_t k) {
     SPH_HIP(hipMemcpy(emit_v_.get(), hv.data(), k * sizeof(float4), hipMemcpyHostToDevice));
     k_emit_scatter<<<nblocks(k), BLOCK, 0, stream>>>(slots_.get(), emit_x_.get(), emit_v_.get(), (unsigned)k,
                                                      (unsigned)n, 0u, x_.get(), v_.get(), xc_.get(), vdiff_.get(),
-                                                     p_.get(), pv_.get(), id_.get(), states_ ? pst_.get() : nullptr);
+                                                     p_.get(), pv_.get(), id_.get(), states_ ? pst_.get() : nullptr,
+                                                     method_ == SPH_METHOD_PCISPH ? predx_.get() : nullptr);
     SPH_HIP(hipGetLastError());
     SPH_HIP(hipStreamSynchronize(stream));
     n_ = n + k;
@@ -350,7 +356,8 @@ void DFSPH::emitter_system_step() {
             SPH_HIP(hipMemcpy(emit_v_.get(), hv.data(), k * sizeof(float4), hipMemcpyHostToDevice));
             k_emit_scatter<<<nblocks(k), BLOCK, 0, stream>>>(slots_.get(), emit_x_.get(), emit_v_.get(), (unsigned)k,
                                                              (unsigned)n0, 1u, x_.get(), v_.get(), xc_.get(),
-                                                             vdiff_.get(), p_.get(), pv_.get(), id_.get(), pst_.get());
+                                                             vdiff_.get(), p_.get(), pv_.get(), id_.get(), pst_.get(),
+                                                             method_ == SPH_METHOD_PCISPH ? predx_.get() : nullptr);
             SPH_HIP(hipGetLastError());
             n_ = n0 + fresh;
             emitted_ += fresh;

@@ -1,0 +1,341 @@
+// pcisph.hip -- the PCISPH time step (SPlisHSPlasH/PCISPH/TimeStepPCISPH.cpp)
+// on the DFSPH handle's device state: the same neighbour lists, boundary
+// records, density sweep and non-pressure forces, with the predictor-
+// corrector pressure loop in place of the two DFSPH solves.
+//
+// One iteration is four sweeps: predict (v* = v + h (a + a_p), x* = x + h v*),
+// the boundary records at x* (volume or density maps), the density at the
+// predicted positions with the pressure update p += delta / h^2 (rho* - 1)
+// and the FP64 error sum, and the pressure accelerations at the current
+// positions. The convergence decision stays on the device: a one-block
+// reduction sets a pending flag that takes effect after the iteration's
+// acceleration sweep, so launches past convergence are no-ops.
+#include <algorithm>
+#include <cmath>
+
+#include "dfsph.hpp"
+
+namespace sph {
+
+namespace {
+
+__device__ __forceinline__ bool active_i(const Dev &D, unsigned i) { return D.state == nullptr || D.state[i] == 0u; }
+
+// pressureSolve initialisation (TimeStepPCISPH.cpp:144-160): v += h a for
+// active particles (h after the CFL update), p = 0, a_p = 0.
+__global__ __launch_bounds__(BLOCK) void k_pci_init(Dev D) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    if (active_i(D, i)) {
+        const float h = (float)D.ctl->h;
+        const float4 a = D.a[i];
+        float4 v = D.v[i];
+        v.x += h * a.x;
+        v.y += h * a.y;
+        v.z += h * a.z;
+        D.v[i] = v;
+    }
+    D.p[i] = 0.f;
+    D.ap[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// pressureSolveIteration, prediction (TimeStepPCISPH.cpp:196-216).
+__global__ __launch_bounds__(BLOCK) void k_pci_predict(Dev D, float4 *__restrict__ predx) {
+    if (D.ctl->done) return;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n || !active_i(D, i)) return;
+    const float h = (float)D.ctl->h;
+    const float4 a = D.a[i], ap = D.ap[i], v = D.v[i], x = D.x[i];
+    const float vx = v.x + h * (a.x + ap.x), vy = v.y + h * (a.y + ap.y), vz = v.z + h * (a.z + ap.z);
+    predx[i] = make_float4(x.x + h * vx, x.y + h * vy, x.z + h * vz, 0.f);
+}
+
+// Density at the predicted positions and the pressure update
+// (TimeStepPCISPH.cpp:219-275); per-block FP64 sums of rho0 (rho* - 1).
+__global__ __launch_bounds__(BLOCK) void k_pci_density(Dev D, const float4 *__restrict__ predx, float delta) {
+    __shared__ double lds[BLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->done) return;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    double err = 0.0;
+    if (i < D.n) {
+        const float4 xi = predx[i];
+        float dens = D.V * D.K.W0;
+        const unsigned cnt = D.cf[i];
+        const unsigned *__restrict__ L = D.nf + i;
+        const size_t S = D.stride;
+        unsigned k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            const float4 x0 = predx[L[k * S]], x1 = predx[L[(k + 1) * S]], x2 = predx[L[(k + 2) * S]],
+                         x3 = predx[L[(k + 3) * S]];
+            dens += D.V * kernel_W(xi, x0, D.K);
+            dens += D.V * kernel_W(xi, x1, D.K);
+            dens += D.V * kernel_W(xi, x2, D.K);
+            dens += D.V * kernel_W(xi, x3, D.K);
+        }
+        for (; k < cnt; k++) dens += D.V * kernel_W(xi, predx[L[k * S]], D.K);
+        if (D.bmode == 1) {
+            const unsigned cb = D.cb[i];
+            for (unsigned q = 0; q < cb; q++) {
+                const float4 xb = D.bx[D.nb[(size_t)q * D.stride + i]];
+                dens += xb.w * kernel_W(xi, xb, D.K);
+            }
+        } else if (D.bmode == 2) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float4 xb = D.vmb[(size_t)o * D.stride + i];
+                if (!(xb.w > 0.f)) break;
+                dens += xb.w * kernel_W(xi, xb, D.K);
+            }
+        } else if (D.bmode == 3) {
+            for (unsigned o = 0; o < D.nvm; o++) {
+                const float w = D.vmb[(size_t)o * D.stride + i].w;
+                if (!(w > 0.f)) break;
+                dens += w;
+            }
+        }
+        D.radv[i] = dens;
+        const double h = C->h;
+        const float invH2 = (float)(1.0 / (h * h));
+        D.p[i] = fmaxf(D.p[i] + invH2 * delta * (dens - 1.0f), 0.0f);
+        err = (double)D.rho0 * ((double)dens - 1.0);
+    }
+    const double bs = block_sum(err, lds);
+    if (threadIdx.x == 0) D.partial[blockIdx.x] = bs;
+}
+
+// Average density error and convergence (TimeStepPCISPH.cpp:162-186, 277):
+// the decision waits in `pending` until the iteration's last sweep is done.
+__global__ __launch_bounds__(RBLOCK) void k_pci_reduce(Dev D, unsigned nparts) {
+    __shared__ double lds[RBLOCK / 64];
+    Ctl *C = D.ctl;
+    if (C->done) return;
+    double t = 0.0;
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) t += D.partial[b];
+    t = block_sum(t, lds);
+    if (threadIdx.x == 0) {
+        const double avg = D.n ? t / (double)D.n : 0.0;
+        const bool finite = isfinite(avg);
+        if (!finite) C->flags |= 1;
+        C->avg_err = avg;
+        C->it += 1;
+        const double eta = D.maxError * 0.01 * (double)D.rho0;
+        C->done_v = ((avg <= eta && C->it >= D.minIt) || C->it >= D.maxIt || !finite) ? 1 : 0;  // pending
+    }
+}
+
+// Pressure accelerations at the current positions (TimeStepPCISPH.cpp:
+// 280-333): a_i = -sum V (p_i + p_j) grad W_ij and the boundary terms.
+__global__ __launch_bounds__(BLOCK) void k_pci_accel(Dev D) {
+    if (D.ctl->done) return;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.n) return;
+    const float4 xi = D.x[i];
+    const float pi = D.p[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const unsigned cnt = D.cf[i];
+    const unsigned *__restrict__ L = D.nf + i;
+    const size_t S = D.stride;
+    auto pair = [&](unsigned j) {
+        float3 r;
+        const float g = kernel_G(xi, D.x[j], D.K, r);
+        const float c = D.V * (pi + D.p[j]);
+        ax -= c * (g * r.x);
+        ay -= c * (g * r.y);
+        az -= c * (g * r.z);
+    };
+    unsigned k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const unsigned j0 = L[k * S], j1 = L[(k + 1) * S], j2 = L[(k + 2) * S], j3 = L[(k + 3) * S];
+        pair(j0);
+        pair(j1);
+        pair(j2);
+        pair(j3);
+    }
+    for (; k < cnt; k++) pair(L[k * S]);
+    auto bterm = [&](const float4 &xb) {
+        float3 r;
+        const float g = kernel_G(xi, xb, D.K, r);
+        const float c = xb.w * pi;
+        ax -= c * (g * r.x);
+        ay -= c * (g * r.y);
+        az -= c * (g * r.z);
+    };
+    if (D.bmode == 1) {
+        const unsigned cb = D.cb[i];
+        for (unsigned q = 0; q < cb; q++) bterm(D.bx[D.nb[(size_t)q * D.stride + i]]);
+    } else if (D.bmode == 2) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            const float4 xb = D.vmb[(size_t)o * D.stride + i];
+            if (!(xb.w > 0.f)) break;
+            bterm(xb);
+        }
+    } else if (D.bmode == 3) {
+        for (unsigned o = 0; o < D.nvm; o++) {
+            if (!(D.vmb[(size_t)o * D.stride + i].w > 0.f)) break;
+            const float4 gb = D.dmg[(size_t)o * D.stride + i];
+            ax += pi * gb.x;
+            ay += pi * gb.y;
+            az += pi * gb.z;
+        }
+    }
+    D.ap[i] = make_float4(ax, ay, az, 0.f);
+}
+
+// The iteration's decision takes effect.
+__global__ void k_pci_commit(Ctl *C) {
+    if (!C->done) C->done = C->done_v;
+}
+
+// Time integration (TimeStepPCISPH.cpp:104-122, h at the start of the step):
+// v += h a_p, x += h v for active particles (compensated), then the time.
+__global__ __launch_bounds__(BLOCK) void k_pci_final(Dev D) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    Ctl *C = D.ctl;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C->t_prev = C->time;
+        C->time += C->h_step;
+    }
+    if (i >= D.n || !active_i(D, i)) return;
+    const float hs = (float)C->h_step;
+    const float4 ap = D.ap[i];
+    float4 v = D.v[i];
+    v.x += hs * ap.x;
+    v.y += hs * ap.y;
+    v.z += hs * ap.z;
+    D.v[i] = v;
+    float4 x = D.x[i], c = D.xc[i];
+    const float yx = hs * v.x - c.x, yy = hs * v.y - c.y, yz = hs * v.z - c.z;
+    const float tx = x.x + yx, ty = x.y + yy, tz = x.z + yz;
+    c.x = (tx - x.x) - yx;
+    c.y = (ty - x.y) - yy;
+    c.z = (tz - x.z) - yz;
+    x.x = tx;
+    x.y = ty;
+    x.z = tz;
+    D.x[i] = x;
+    D.xc[i] = c;
+}
+
+__global__ void k_pci_reset(Ctl *C) {
+    C->it = 0;
+    C->done = 0;
+    C->done_v = 0;
+    C->avg_err = 0.0;
+}
+
+}  // namespace
+
+// SimulationDataPCISPH::init's scaling factor (SimulationDataPCISPH.cpp:
+// 40-108): delta = 1 / (beta (|sum grad W|^2 + sum |grad W|^2)), beta = 2 V^2,
+// over a cubic lattice of spacing 2r around the origin inside the support.
+// grad W is the precomputed kernel's (the table of the cubic's gradient over
+// r at 10,000 points, the mean of the two entries around r, as the sweeps
+// evaluate it), in FP64.
+double pcisph_factor(double r, double V) {
+    const double hs = 4.0 * r, diam = 2.0 * r, l = 48.0 / (M_PI * hs * hs * hs);
+    const unsigned RES = 10000u;
+    const double step = hs / (double)(RES - 1), invStep = 1.0 / step;
+    auto G = [&](unsigned i) {  // (dW/dr) / r at table point i
+        const double x = step * (double)i;
+        if (!(x > 1.0e-9)) return 0.0;
+        const double q = x / hs;
+        if (q > 1.0) return 0.0;
+        const double gl = q <= 0.5 ? l * q * (3.0 * q - 2.0) / hs : -l * (1.0 - q) * (1.0 - q) / hs;
+        return gl / x;
+    };
+    double sg[3] = {0, 0, 0}, sg2 = 0.0;
+    for (double a0 = -hs; a0 <= hs; a0 += diam)
+        for (double a1 = -hs; a1 <= hs; a1 += diam)
+            for (double a2 = -hs; a2 <= hs; a2 += diam) {
+                const double d[3] = {-a0, -a1, -a2};
+                const double d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+                if (!(d2 < hs * hs)) continue;
+                const unsigned pos = std::min<unsigned>((unsigned)(std::sqrt(d2) * invStep), RES - 2u);
+                const double g = 0.5 * (G(pos) + G(pos + 1));
+                for (int c = 0; c < 3; c++) sg[c] += g * d[c];
+                sg2 += g * g * d2;
+            }
+    const double beta = 2.0 * V * V;
+    return 1.0 / (beta * (((sg[0] * sg[0] + sg[1] * sg[1]) + sg[2] * sg[2]) + sg2));
+}
+
+void DFSPH::set_method(int m) {
+    SPH_REQUIRE(m == SPH_METHOD_DFSPH || m == SPH_METHOD_PCISPH, SPH_ERR_INVALID,
+                "simulation method: only PCISPH (1) and DFSPH (4) are implemented");
+    if (m == method_) return;
+    method_ = m;
+    if (m == SPH_METHOD_PCISPH && initialized_) {
+        predx_.reserve(stride_);
+        predx2_.reserve(stride_);
+        std::vector<float4> z(stride_, make_float4(0.f, 0.f, 0.f, 0.f));
+        SPH_HIP(hipStreamSynchronize(stream));
+        SPH_HIP(hipMemcpy(predx_.get(), z.data(), (size_t)stride_ * sizeof(float4), hipMemcpyHostToDevice));
+    }
+}
+
+// TimeStepPCISPH::step (TimeStepPCISPH.cpp:80-128).
+void DFSPH::step_pcisph() {
+    neighborhood_search();
+    volume_maps();
+    sweep_init(true, false, false);  // computeDensities
+    const bool weiler = P.viscosity_method == SPH_VISCOSITY_WEILER2018;
+    const bool stdv = P.viscosity_method == SPH_VISCOSITY_STANDARD;
+    if ((weiler || stdv || P.xsph != 0.0) && n_) mrho_copy();
+    if (weiler) viscosity(false);
+    const int expl = ((P.xsph != 0.0 || P.xsph_boundary != 0.0) ? 1 : 0) | (stdv ? 2 : 0);
+    // clearAccelerations, computeNonPressureForces, updateTimeStepSize
+    post_forces(true, weiler, expl, P.cfl_method == 1 || P.cfl_method == 2, false, false);
+    mrho_w_ = false;
+    // pressureSolve
+    const Dev D = dev();
+    const unsigned nb = nblocks(n_);
+    k_pci_reset<<<1, 1, 0, stream>>>(ctl_.get());
+    {
+        TimedScope ts(timer, stream, "pcisph_init");
+        k_pci_init<<<nb, BLOCK, 0, stream>>>(D);
+    }
+    SPH_HIP(hipGetLastError());
+    const float delta = (float)pci_factor_;
+    const bool mapsOn = maps() && nvm_ > 0;
+    const int *stop = &ctl_.get()->done;
+    int launched = 0, chunk = std::max(P.min_iterations, std::max(1, lastIt_));
+    for (;;) {
+        for (int c = 0; c < chunk && launched < P.max_iterations; c++, launched++) {
+            {
+                TimedScope ts(timer, stream, "pcisph_predict");
+                k_pci_predict<<<nb, BLOCK, 0, stream>>>(D, predx_.get());
+            }
+            if (mapsOn) volume_maps(predx_.get(), states_ ? pst_.get() : nullptr, stop);
+            {
+                TimedScope ts(timer, stream, "pcisph_density");
+                k_pci_density<<<nb, BLOCK, 0, stream>>>(D, predx_.get(), delta);
+            }
+            {
+                TimedScope ts(timer, stream, "pcisph_reduce");
+                k_pci_reduce<<<1, RBLOCK, 0, stream>>>(D, nb);
+            }
+            {
+                TimedScope ts(timer, stream, "pcisph_accel");
+                k_pci_accel<<<nb, BLOCK, 0, stream>>>(D);
+            }
+            k_pci_commit<<<1, 1, 0, stream>>>(ctl_.get());
+            SPH_HIP(hipGetLastError());
+        }
+        read_ctl();
+        if (hctl_->done || launched >= P.max_iterations) break;
+        chunk = 2;
+    }
+    itLast_ = hctl_->it;
+    lastIt_ = std::max(P.min_iterations, itLast_);
+    totIt_ += itLast_;
+    itVLast_ = 0;
+    {
+        TimedScope ts(timer, stream, "pcisph_final");
+        k_pci_final<<<nb, BLOCK, 0, stream>>>(D);
+    }
+    SPH_HIP(hipGetLastError());
+    emitter_system_step();
+    steps_++;
+}
+
+}  // namespace sph

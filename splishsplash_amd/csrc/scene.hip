@@ -218,6 +218,7 @@ struct Scene {
     sph_dfsph_params_t params{};
     std::vector<float> x, v, bx;  // fluid positions, velocities, Akinci particles
     std::vector<SceneBody> bodies;
+    int method = SPH_METHOD_DFSPH;  // simulationMethod
     // emitters and the fluid's emitter settings (FluidParameterObject
     // maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max)
     std::vector<sph_emitter_t> emitters;
@@ -525,7 +526,9 @@ static Scene *read_scene(const std::string &path) {
     // Configuration: Simulation, TimeManager, TimeStepDFSPH's "DFSPH" section
     const JValue *cfgp = root.find("Configuration");
     const JValue &cfg = cfgp ? *cfgp : empty_obj;
-    if (num(cfg, "simulationMethod", 4) != 4) unsupported("\"simulationMethod\" other than DFSPH (4)");
+    S->method = (int)num(cfg, "simulationMethod", 4);
+    if (S->method != SPH_METHOD_DFSPH && S->method != SPH_METHOD_PCISPH)
+        unsupported("\"simulationMethod\" other than PCISPH (1) and DFSPH (4)");
     if (flag(cfg, "sim2D", false)) unsupported("\"sim2D\"");
     for (const char *k : {"kernel", "gradKernel"})
         if (cfg.find(k) && num(cfg, k, 4) != 4) unsupported(std::string("\"") + k + "\" other than the precomputed cubic spline (4)");
@@ -540,7 +543,17 @@ static Scene *read_scene(const std::string &path) {
     if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012 && P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 &&
         P.boundary_handling_method != SPH_BOUNDARY_KOSCHIER2017)
         unsupported("\"boundaryHandlingMethod\" " + std::to_string(P.boundary_handling_method));
-    if (const JValue *d = cfg.find("DFSPH")) {
+    if (S->method == SPH_METHOD_PCISPH) {
+        // TimeStepPCISPH's parameters and defaults (TimeStepPCISPH.cpp:24-31)
+        P.min_iterations = 3;
+        P.max_iterations = 100;
+        P.max_error = 0.01;
+        if (const JValue *d = cfg.find("PCISPH")) {
+            P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
+            P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
+            P.max_error = num(*d, "maxError", P.max_error);
+        }
+    } else if (const JValue *d = cfg.find("DFSPH")) {
         P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
         P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
         P.max_error = num(*d, "maxError", P.max_error);
@@ -814,6 +827,7 @@ int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out) {
                 throw sph::Error(code, m);
             }
         };
+        chk(sph_dfsph_set_simulation_method(h, S.method));
         chk(sph_dfsph_set_fluid(h, S.x.data(), S.v.data(), S.x.size() / 3));
         if (S.params.boundary_handling_method == SPH_BOUNDARY_AKINCI2012) {
             if (!S.bx.empty()) chk(sph_dfsph_add_boundary_particles(h, S.bx.data(), S.bx.size() / 3));

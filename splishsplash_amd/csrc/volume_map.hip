@@ -453,14 +453,17 @@ __device__ double interp_sdf(const VMapDesc &d, const double *y) {
 __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,
                                                      float4 *__restrict__ v, float4 *__restrict__ xc,
                                                      float4 *__restrict__ vmb, const unsigned *__restrict__ state,
-                                                     double supportRadius, double particleRadius) {
+                                                     const float4 *__restrict__ xin, const unsigned *__restrict__ skip,
+                                                     const int *stop, double supportRadius, double particleRadius) {
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (stop && *stop) return;          // a converged solve's remaining launches
+    if (skip && skip[i] != 0u) return;  // not visited at all
     if (state && state[i] != 0u) {  // only active particles see the boundary (TimeStep.cpp:268)
         vmb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    const float4 xf = x[i];
+    const float4 xf = xin ? xin[i] : x[i];
     const double xi[3] = {(double)xf.x, (double)xf.y, (double)xf.z};
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     double dist = DBL_MAX, normal[3] = {0.0, 0.0, 0.0};
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, flo
             double delta = 2.0 * particleRadius - dist;
             delta = fmin(delta, 0.1 * particleRadius);
             x[i] = make_float4((float)(xi[0] + delta * normal[0] / nl), (float)(xi[1] + delta * normal[1] / nl),
-                               (float)(xi[2] + delta * normal[2] / nl), xf.w);
+                               (float)(xi[2] + delta * normal[2] / nl), xin ? x[i].w : xf.w);
             v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             xc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -541,18 +544,22 @@ __global__ __launch_bounds__(BLOCK) void k_vm_sample(VMapDesc d, unsigned n, flo
 }
 
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
-                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T) {
+                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T,
+                       const float4 *xin, const unsigned *skip, const int *stop) {
     if (!n) return;
     TimedScope ts(T, s, "volume_map_sample");
-    k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, state, supportRadius, particleRadius);
+    k_vm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, vmb, state, xin, skip, stop, supportRadius,
+                                             particleRadius);
     SPH_HIP(hipGetLastError());
 }
 
 __global__ __launch_bounds__(BLOCK) void k_vm_pack(unsigned n, unsigned stride, unsigned nobj,
                                                    const float4 *__restrict__ raw, float4 *__restrict__ vmb,
-                                                   const float4 *__restrict__ raw2, float4 *__restrict__ vmb2) {
+                                                   const float4 *__restrict__ raw2, float4 *__restrict__ vmb2,
+                                                   const int *stop) {
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (stop && *stop) return;
     unsigned k = 0;
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     for (unsigned o = 0; o < nobj; o++) {
@@ -569,10 +576,10 @@ __global__ __launch_bounds__(BLOCK) void k_vm_pack(unsigned n, unsigned stride, 
 }
 
 void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
-                      KernelTimer &T, const float4 *raw2, float4 *vmb2) {
+                      KernelTimer &T, const float4 *raw2, float4 *vmb2, const int *stop) {
     if (!n) return;
     TimedScope ts(T, s, "volume_map_pack");
-    k_vm_pack<<<nblocks(n), BLOCK, 0, s>>>(n, stride, nobj, raw, vmb, raw2, vmb2);
+    k_vm_pack<<<nblocks(n), BLOCK, 0, s>>>(n, stride, nobj, raw, vmb, raw2, vmb2, stop);
     SPH_HIP(hipGetLastError());
 }
 
@@ -621,13 +628,16 @@ __device__ __forceinline__ bool interp_field(const VMapDesc &d, const double *xi
 __global__ __launch_bounds__(BLOCK) void k_dm_sample(VMapDesc d, unsigned n, float4 *__restrict__ x,
                                                      float4 *__restrict__ v, float4 *__restrict__ xc,
                                                      float4 *__restrict__ rec, float4 *__restrict__ grad, const double *dtp,
-                                                     double supportRadius, double particleRadius) {
+                                                     const float4 *__restrict__ xin, const unsigned *__restrict__ skip,
+                                                     const int *stop, double supportRadius, double particleRadius) {
     const unsigned i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (stop && *stop) return;
+    if (skip && skip[i] != 0u) return;  // not visited at all
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     rec[i] = z;
     grad[i] = z;
-    const float4 xf = x[i];
+    const float4 xf = xin ? xin[i] : x[i];
     const double xi[3] = {(double)xf.x, (double)xf.y, (double)xf.z};
     double dist = DBL_MAX, normal[3] = {0.0, 0.0, 0.0};
     double val;
@@ -649,7 +659,7 @@ __global__ __launch_bounds__(BLOCK) void k_dm_sample(VMapDesc d, unsigned n, flo
         if (nl > 1.0e-5) {
             const double nx = normal[0] / nl, ny = normal[1] / nl, nz = normal[2] / nl;
             const double dd = fmin(-dist, (0.25 / 0.005) * particleRadius * (*dtp));
-            x[i] = make_float4((float)(xi[0] + dd * nx), (float)(xi[1] + dd * ny), (float)(xi[2] + dd * nz), xf.w);
+            x[i] = make_float4((float)(xi[0] + dd * nx), (float)(xi[1] + dd * ny), (float)(xi[2] + dd * nz), x[i].w);
             xc[i] = z;
             float4 w = v[i];
             const double vn = 0.05 - (((double)w.x * nx + (double)w.y * ny) + (double)w.z * nz);
@@ -662,10 +672,12 @@ __global__ __launch_bounds__(BLOCK) void k_dm_sample(VMapDesc d, unsigned n, flo
 }
 
 void sample_density_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *rec,
-                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T) {
+                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T,
+                        const float4 *xin, const unsigned *skip, const int *stop) {
     if (!n) return;
     TimedScope ts(T, s, "density_map_sample");
-    k_dm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, rec, grad, dt, supportRadius, particleRadius);
+    k_dm_sample<<<nblocks(n), BLOCK, 0, s>>>(d, n, x, v, xc, rec, grad, dt, xin, skip, stop, supportRadius,
+                                             particleRadius);
     SPH_HIP(hipGetLastError());
 }
 

@@ -77,8 +77,13 @@ public:
 // projects particles with sdf <= 0 back towards the surface and zeroes their
 // velocity; xc (position compensation) is cleared for moved particles.
 // Particles whose state (if given) is not 0 get an empty record.
+// With `xin` the particle positions are read from xin (a moved particle is
+// still written to x: x = xin + delta n); with `skip`, particles whose entry
+// is not 0 are left untouched (record included); with `stop`, nothing is
+// done while *stop is set (launches of a solve that has converged).
 void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *vmb,
-                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T);
+                       const unsigned *state, double supportRadius, double particleRadius, KernelTimer &T,
+                       const float4 *xin = nullptr, const unsigned *skip = nullptr, const int *stop = nullptr);
 
 // TimeStep::computeDensityAndGradient (TimeStep.cpp:349-468) for one static
 // density map: rec[i] = (x_b, rho_b) with x_b = x_i - (dist + h/2) n,
@@ -88,12 +93,14 @@ void sample_volume_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, 
 // to 0.05 m/s. Every particle is sampled, whatever its state (the
 // reference's loop has no state check). dt points at the time step size.
 void sample_density_map(hipStream_t s, const VMapDesc &d, unsigned n, float4 *x, float4 *v, float4 *xc, float4 *rec,
-                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T);
+                        float4 *grad, const double *dt, double supportRadius, double particleRadius, KernelTimer &T,
+                        const float4 *xin = nullptr, const unsigned *skip = nullptr, const int *stop = nullptr);
 
 // Packs, per particle, the records with V_b > 0 of `nobj` object-major
 // arrays (raw[o * stride + i]) into the leading slots of vmb (same layout),
 // keeping object order, and zeroes the remaining slots.
 void pack_volume_maps(hipStream_t s, unsigned n, unsigned stride, unsigned nobj, const float4 *raw, float4 *vmb,
-                      KernelTimer &T, const float4 *raw2 = nullptr, float4 *vmb2 = nullptr);
+                      KernelTimer &T, const float4 *raw2 = nullptr, float4 *vmb2 = nullptr,
+                      const int *stop = nullptr);
 
 }  // namespace sph

@@ -100,6 +100,10 @@ class Emitter(C.Structure):
     ]
 
 
+# Simulation simulationMethod values
+METHOD_PCISPH = 1
+METHOD_DFSPH = 4
+
 # ParticleState (FluidModel.h:75)
 STATE_ACTIVE = 0
 STATE_ANIMATED_BY_EMITTER = 1
@@ -147,5 +151,5 @@ FIELD_COMPONENTS = {
     "factor": 1, "advected density": 1, "p / rho^2": 1, "p_v / rho^2": 1,
     "pressure acceleration": 3, "velocity difference": 3,
     "boundary volume": 1, "boundary x": 3, "boundary density": 1, "boundary density gradient": 3,
-    "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1, "state": 1,
+    "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1, "state": 1, "predicted position": 3,
 }

@@ -1,0 +1,113 @@
+"""PCISPH (TimeStepPCISPH) on the device against the FP64 oracle, whose
+PCISPH step is pinned on the CPU (tests/test_oracle_pcisph.py): whole steps
+from identical states with Akinci2012 particles, Bender2019 volume maps
+(box wall and sphere, sampled again at the predicted positions every
+iteration) and Koschier2017 density maps, with XSPH and standard viscosity;
+the iteration counts are equal, positions agree to 1e-4 and velocities to
+1e-3 normwise. The looser velocity bound is PCISPH's own sensitivity: its
+pressure comes from rho* - 1 at predicted positions x + h v*, and FP32
+positions (rounded at 6e-8 relative, as in the reference's default float
+build) move rho* by ~1e-6, against rho* - 1 ~ 1e-3. An emitter run checks
+the states and the predicted positions of animated particles."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from splishsplash_amd import capi, scenes
+from splishsplash_amd.params import (BOUNDARY_BENDER2019, BOUNDARY_KOSCHIER2017, METHOD_PCISPH, VISCOSITY_STANDARD,
+                                     default_params)
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def build(params, x, v=None, boundary_x=None, vm=None, spheres=()):
+    g = capi.TimeStepDFSPH(params)
+    o = oracle.OracleDFSPH(params)
+    for s in (g, o):
+        s.set_simulation_method(METHOD_PCISPH)
+        s.set_fluid(x, v)
+        if boundary_x is not None:
+            s.add_boundary_particles(boundary_x)
+        if vm is not None:
+            s.set_volume_map_box(*vm)
+    for c, r in spheres:
+        g.add_volume_map_sphere(c, r, 0.05)
+        o.add_volume_map_sphere(c, r, 0.05)
+    g.init()
+    o.init()
+    return g, o
+
+
+def stepwise(g, o, steps, fields=("position", "velocity"), pred=False):
+    for k in range(steps):
+        for nm in fields:
+            o.set(nm, g.get(nm).astype(np.float64))
+        if pred:
+            o.set("predicted position", g.get("predicted position").astype(np.float64))
+            o.set("state", g.get("state").astype(np.float64))
+        c = g.counters()
+        o.set_time(c["time"], c["time_step_size"])
+        g.step(1)
+        o.step(1)
+        cg, co = g.counters(), o.counters()
+        assert cg["iterations"] == co["iterations"], (k, cg["iterations"], co["iterations"])
+        assert cg["avg_density_err"] == pytest.approx(co["avg_density_err"], rel=1e-3, abs=1e-4)
+        assert g.n == o.n
+        for nm, tol in (("position", 1e-4), ("velocity", 1e-3)):
+            e = rel(g.get(nm), o.get(nm))
+            assert e <= tol, (k, nm, e)
+    return g.counters()
+
+
+def test_pcisph_akinci_dam_break():
+    sc = scenes.dam_break_small(per_side=16)
+    prm = sc["params"]
+    prm.xsph = 0.01
+    g, o = build(prm, sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    g.step(30)  # let the column start to collapse
+    c = stepwise(g, o, 5)
+    assert c["iterations"] >= prm.min_iterations
+
+
+@pytest.mark.parametrize("bhm", [BOUNDARY_BENDER2019, BOUNDARY_KOSCHIER2017])
+def test_pcisph_maps(bhm):
+    x = scenes.lattice_block((14, 14, 14), 0.05, (0.06, 0.06, 0.06), jitter=0.002, seed=5)
+    prm = default_params(boundary_handling_method=bhm, xsph=0.01, xsph_boundary=0.02, cfl_factor=0.5,
+                         time_step_size=0.002, viscosity_method=VISCOSITY_STANDARD, viscosity=0.01,
+                         viscosity_boundary=0.01, max_error=0.05)
+    lo, hi = (0.0, 0.0, 0.0), (1.2, 1.0, 1.0)
+    vm = (lo, hi, scenes.volume_map_resolution(lo, hi, 0.1, 0.05))
+    g, o = build(prm, x, vm=vm, spheres=[((0.9, 0.2, 0.5), 0.15)])
+    g.step(25)
+    stepwise(g, o, 5)
+
+
+def test_pcisph_with_an_emitter():
+    x = scenes.lattice_block((6, 6, 6), 0.05, (-0.15, 0.0, -0.15), jitter=0.002, seed=3)
+    prm = default_params(particle_radius=0.025, boundary_handling_method=-1, cfl_method=0, time_step_size=0.001)
+    g, o = build(prm, x)
+    R = np.array([[math.cos(-1.2), -math.sin(-1.2), 0.0], [math.sin(-1.2), math.cos(-1.2), 0.0], [0.0, 0.0, 1.0]])
+    for s in (g, o):
+        s.add_emitter(5, 4, [0.0, 0.45, 0.0], R, 2.0)
+    stepwise(g, o, 60, pred=True)
+    assert g.emitter_counts()[0] == 3 * 20
+    np.testing.assert_array_equal(g.get("state"), o.get("state").astype(np.uint32))
+
+
+def test_method_switch_and_errors():
+    sc = scenes.dam_break_small(per_side=8)
+    g = capi.TimeStepDFSPH(sc["params"])
+    g.set_fluid(sc["x"], sc["v"])
+    g.add_boundary_particles(sc["boundary_x"])
+    g.step(3)
+    g.set_simulation_method(METHOD_PCISPH)  # between steps
+    g.step(3)
+    assert np.isfinite(g.get("position")).all() and g.counters()["iterations"] >= 2
+    with pytest.raises(capi.SPHError):
+        g.set_simulation_method(0)

@@ -144,7 +144,7 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
 @pytest.mark.parametrize("change, key", [
     ({"RigidBodies": [{"geometryFile": "unit_box.obj", "isDynamic": True}]}, "dynamic"),
     ({"Emitters": [{"id": "Other", "width": 5}]}, "fluid id"),
-    ({"Configuration": {"simulationMethod": 1}}, "simulationMethod"),
+    ({"Configuration": {"simulationMethod": 3}}, "simulationMethod"),
     ({"Configuration": {"sim2D": True}}, "sim2D"),
     ({"Configuration": {"kernel": 0}}, "kernel"),
     ({"Configuration": {"boundaryHandlingMethod": 3}}, "boundaryHandlingMethod"),
@@ -246,6 +246,14 @@ def test_emitters_and_their_housings(tmp_path):
     # Akinci2012: no housing (no mesh sampling), emitters still read
     sc0 = capi.Scene(write_scene(tmp_path, emitter_scene(0)))
     assert sc0.nbodies == 1 and len(sc0.emitters()[0]) == 2
+
+
+def test_pcisph_scene_parameters(tmp_path):
+    scene = {"Configuration": {"simulationMethod": 1, "PCISPH": {"maxIterations": 50, "maxError": 0.05}},
+             "FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]}
+    sc = capi.Scene(write_scene(tmp_path, scene))
+    p = sc.params
+    assert (p.min_iterations, p.max_iterations, p.max_error) == (3, 50, 0.05)  # TimeStepPCISPH defaults + section
 
 
 def test_malformed_files_are_errors(tmp_path):
