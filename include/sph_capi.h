@@ -56,6 +56,7 @@ extern "C" {
 #define SPH_BOUNDARY_KOSCHIER2017 1  /* density maps of the boundary objects */
 #define SPH_BOUNDARY_BENDER2019  2
 #define SPH_METHOD_PCISPH        1   /* Simulation simulationMethod values */
+#define SPH_METHOD_IISPH         3
 #define SPH_METHOD_DFSPH         4
 #define SPH_VISCOSITY_NONE       0
 #define SPH_VISCOSITY_STANDARD   1   /* Viscosity_Standard (explicit)        */
@@ -303,13 +304,15 @@ int sph_dfsph_reset(sph_dfsph_t s);
 int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v,
                              size_t n);
 /* Simulation::setSimulationMethod (Simulation.cpp:528-580) for this handle:
- * SPH_METHOD_DFSPH (default) or SPH_METHOD_PCISPH, TimeStepPCISPH::step
+ * SPH_METHOD_DFSPH (default); SPH_METHOD_PCISPH, TimeStepPCISPH::step
  * (PCISPH/TimeStepPCISPH.cpp:80-128): the predictor-corrector pressure loop
- * with the scaling factor of SimulationDataPCISPH::init, using the
- * parameters' min_iterations, max_iterations and max_error (PCISPH's own
- * defaults are 3, 100 and 0.01 %). The neighbour search, boundaries,
- * non-pressure forces, CFL, states and emitters are shared. Call before the
- * first step or between steps. */
+ * with the scaling factor of SimulationDataPCISPH::init; or SPH_METHOD_IISPH,
+ * TimeStepIISPH::step (IISPH/TimeStepIISPH.cpp:91-554): predictAdvection
+ * and the relaxed Jacobi solve with omega 0.5. Both use the parameters'
+ * min_iterations, max_iterations and max_error (their own defaults: 3 / 2,
+ * 100 and 0.01 %). The neighbour search, boundaries, non-pressure forces,
+ * CFL, states and emitters are shared. Call before the first step or
+ * between steps. */
 int sph_dfsph_set_simulation_method(sph_dfsph_t s, int method);
 
 /* Emitter system (SPlisHSPlasH/EmitterSystem.h, Emitter.h). An emitter

@@ -1871,10 +1871,202 @@ void Sim::step_pcisph() {
     time += h_step;
 }
 
+// TimeStepIISPH::step (TimeStepIISPH.cpp:91-128) with predictAdvection
+// (:137-287), pressureSolve (:289-339), pressureSolveIteration (:341-462),
+// integration and computePressureAccels (:464-554).
+void Sim::step_iisph() {
+    neighborhood_search();
+    volume_maps();
+    densities();
+    clear_accelerations();
+    viscosity_standard();
+    viscosity_weiler();
+    xsph();
+    cfl();
+    const double h = dt, h2 = h * h, density0 = P.density0, omega = 0.5;
+    dii.assign(3 * n, 0.0);
+    dijpj.assign(3 * n, 0.0);
+    lastP.assign(n, 0.0);
+    auto boundary_terms = [&](size_t i, auto &&fluid_like, auto &&dmap) {
+        const double *xi = &x[3 * i];
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double d[3] = {xi[0] - bx[3 * j], xi[1] - bx[3 * j + 1], xi[2] - bx[3 * j + 2]};
+                double g[3];
+                K.gradW(d, g);
+                fluid_like(bV[j], g);
+            }
+        } else if (has_density_map()) {
+            for (size_t o = 0; o < vms.size(); o++)
+                if (vmV[o * n + i] != 0.0) dmap(&vmG[3 * (o * n + i)]);
+        } else if (has_volume_map()) {
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (Vj > 0.0) {
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double g[3];
+                    K.gradW(d, g);
+                    fluid_like(Vj, g);
+                }
+            }
+        }
+    };
+    // predict v_adv, d_ii
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        if (active(i))
+            for (int c = 0; c < 3; c++) v[3 * i + c] += h * a[3 * i + c];
+        const double density = rho[i] / density0, density2 = density * density;
+        const double *xi = &x[3 * i];
+        double *d = &dii[3 * i];
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+            double g[3];
+            K.gradW(r, g);
+            for (int c = 0; c < 3; c++) d[c] -= V / density2 * g[c];
+        }
+        boundary_terms(
+            (size_t)i, [&](double Vb, const double *g) { for (int c = 0; c < 3; c++) d[c] -= Vb / density2 * g[c]; },
+            [&](const double *gr) { for (int c = 0; c < 3; c++) d[c] += 1.0 / density2 * gr[c]; });
+    }
+    // rho_adv, last pressure, a_ii
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double density = rho[i] / density0, density2 = density * density, dpi = V / density2;
+        const double *xi = &x[3 * i], *vi = &v[3 * i], *d = &dii[3 * i];
+        double adv = density, aii = 0.0;
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+            double g[3];
+            K.gradW(r, g);
+            const double dv[3] = {vi[0] - v[3 * j], vi[1] - v[3 * j + 1], vi[2] - v[3 * j + 2]};
+            adv += h * V * dot3(dv, g);
+            const double u[3] = {d[0] - dpi * g[0], d[1] - dpi * g[1], d[2] - dpi * g[2]};
+            aii += V * dot3(u, g);
+        }
+        boundary_terms(
+            (size_t)i,
+            [&](double Vb, const double *g) {
+                adv += h * Vb * dot3(vi, g);
+                const double u[3] = {d[0] - dpi * g[0], d[1] - dpi * g[1], d[2] - dpi * g[2]};
+                aii += Vb * dot3(u, g);
+            },
+            [&](const double *gr) {
+                adv -= h * dot3(vi, gr);
+                const double dji[3] = {-gr[0] / density2, -gr[1] / density2, -gr[2] / density2};
+                const double u[3] = {d[0] - dji[0], d[1] - dji[1], d[2] - dji[2]};
+                aii -= dot3(u, gr);
+            });
+        rho_adv[i] = adv;
+        lastP[i] = 0.5 * p[i];
+        factor[i] = aii;  // a_ii
+    }
+    iterations = 0;
+    bool chk = false;
+    double avg = 0.0;
+    std::vector<double> pnew(n), err(n);
+    while ((!chk || (iterations < P.min_iterations)) && (iterations < P.max_iterations)) {
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; i++) {
+            const double *xi = &x[3 * i];
+            double *s0 = &dijpj[3 * i];
+            s0[0] = s0[1] = s0[2] = 0.0;
+            for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+                const size_t j = nf.idx[k];
+                const double dj = rho[j] / density0;
+                const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+                double g[3];
+                K.gradW(r, g);
+                for (int c = 0; c < 3; c++) s0[c] -= V / (dj * dj) * lastP[j] * g[c];
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; i++) {
+            const double density = rho[i] / density0, dpi = V / (density * density);
+            const double *xi = &x[3 * i], *si = &dijpj[3 * i];
+            double sum = 0.0;
+            for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+                const size_t j = nf.idx[k];
+                const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+                double g[3];
+                K.gradW(r, g);
+                double u[3];
+                for (int c = 0; c < 3; c++)
+                    u[c] = si[c] - dii[3 * j + c] * lastP[j] - (dijpj[3 * j + c] - dpi * g[c] * lastP[i]);
+                sum += V * dot3(u, g);
+            }
+            boundary_terms(
+                (size_t)i, [&](double Vb, const double *g) { sum += Vb * dot3(si, g); },
+                [&](const double *gr) { sum -= dot3(si, gr); });
+            const double b = 1.0 - rho_adv[i];
+            const double denom = factor[i] * h2;
+            double pi = 0.0;
+            if (std::fabs(denom) > 1.0e-9) pi = std::max((1.0 - omega) * lastP[i] + omega / denom * (b - h2 * sum), 0.0);
+            pnew[i] = pi;
+            err[i] = 0.0;
+            if (pi != 0.0) {
+                const double nd = density0 * ((factor[i] * pi + sum) * h2 - b) + density0;
+                err[i] = nd - density0;
+            }
+        }
+        double e = 0.0;
+        for (size_t i = 0; i < n; i++) {
+            e += err[i];
+            p[i] = pnew[i];
+            lastP[i] = pnew[i];
+        }
+        avg = n ? e / (double)n : 0.0;
+        const double eta = P.max_error * 0.01 * density0;
+        chk = (avg <= eta);
+        iterations++;
+    }
+    lastErr = avg;
+    totIt += iterations;
+    iterationsV = 0;
+    // pressure accelerations and the integration with the current h
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double density = rho[i] / density0, dpi = p[i] / (density * density);
+        const double *xi = &x[3 * i];
+        double *ai = &ap[3 * i];
+        ai[0] = ai[1] = ai[2] = 0.0;
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double dj = rho[j] / density0, dpj = p[j] / (dj * dj);
+            const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+            double g[3];
+            K.gradW(r, g);
+            for (int c = 0; c < 3; c++) ai[c] -= V * (dpi + dpj) * g[c];
+        }
+        boundary_terms(
+            (size_t)i, [&](double Vb, const double *g) { for (int c = 0; c < 3; c++) ai[c] -= Vb * dpi * g[c]; },
+            [&](const double *gr) { for (int c = 0; c < 3; c++) ai[c] -= -dpi * gr[c]; });
+    }
+    const double hi = dt;
+    for (size_t i = 0; i < n; i++) {
+        if (!active(i)) continue;
+        for (int c = 0; c < 3; c++) {
+            v[3 * i + c] += ap[3 * i + c] * hi;
+            x[3 * i + c] += v[3 * i + c] * hi;
+        }
+    }
+    emitter_system_step();
+    time += h_step;
+}
+
 // TimeStepDFSPH::step (TimeStepDFSPH.cpp:117-249).
 void Sim::step() {
     if (method == SPH_METHOD_PCISPH) {
         step_pcisph();
+        steps++;
+        return;
+    }
+    if (method == SPH_METHOD_IISPH) {
+        step_iisph();
         steps++;
         return;
     }
@@ -2120,7 +2312,8 @@ int orc_pcisph_factor(void *h, double *out) {
     return SPH_OK;
 }
 int orc_set_simulation_method(void *h, int m) {
-    if (m != SPH_METHOD_DFSPH && m != SPH_METHOD_PCISPH) return fail("simulation method: PCISPH (1) or DFSPH (4)");
+    if (m != SPH_METHOD_DFSPH && m != SPH_METHOD_PCISPH && m != SPH_METHOD_IISPH)
+        return fail("simulation method: PCISPH (1), IISPH (3) or DFSPH (4)");
     ((Sim *)h)->method = m;
     return SPH_OK;
 }

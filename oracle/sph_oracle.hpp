@@ -181,6 +181,9 @@ public:
     std::vector<double> predX;
     double pciFactor = 0.0;
     void step_pcisph();
+    // IISPH (TimeStepIISPH, SimulationDataIISPH)
+    std::vector<double> dii, dijpj, lastP;
+    void step_iisph();
     void densities();
     void dfsph_factor();
     void divergence_setup();

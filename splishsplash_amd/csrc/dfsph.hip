@@ -2556,6 +2556,10 @@ void DFSPH::step() {
         step_pcisph();
         return;
     }
+    if (method_ == SPH_METHOD_IISPH) {
+        step_iisph();
+        return;
+    }
     neighborhood_search();
     volume_maps();
     const bool div = P.enable_divergence_solver != 0;

@@ -124,8 +124,9 @@ public:
     const std::vector<EmitterDesc> &emitters() const { return emitters_; }
     void step();
     void stage(int s);
-    // Simulation::setSimulationMethod for this handle: SPH_METHOD_DFSPH or
-    // SPH_METHOD_PCISPH (TimeStepPCISPH, pcisph.hip)
+    // Simulation::setSimulationMethod for this handle: SPH_METHOD_DFSPH,
+    // SPH_METHOD_PCISPH (TimeStepPCISPH, pcisph.hip) or SPH_METHOD_IISPH
+    // (TimeStepIISPH, iisph.hip)
     void set_method(int m);
     int method() const { return method_; }
 
@@ -237,6 +238,10 @@ private:
     double pci_factor_ = 0.0;
     DBuf<float4> predx_, predx2_;  // predicted positions (persistent, sorted with the particles)
     void step_pcisph();
+    // IISPH (iisph.hip): d_ii, sum_j d_ij p_j, last pressure, last pressure / rho^2
+    DBuf<float4> iid_, iidp_;
+    DBuf<float> ilp_, iq_;
+    void step_iisph();
     void stiff_to_w(bool pressure);
     template <bool PRESSURE>
     void iteration_any(bool force);

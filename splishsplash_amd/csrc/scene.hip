@@ -527,8 +527,8 @@ static Scene *read_scene(const std::string &path) {
     const JValue *cfgp = root.find("Configuration");
     const JValue &cfg = cfgp ? *cfgp : empty_obj;
     S->method = (int)num(cfg, "simulationMethod", 4);
-    if (S->method != SPH_METHOD_DFSPH && S->method != SPH_METHOD_PCISPH)
-        unsupported("\"simulationMethod\" other than PCISPH (1) and DFSPH (4)");
+    if (S->method != SPH_METHOD_DFSPH && S->method != SPH_METHOD_PCISPH && S->method != SPH_METHOD_IISPH)
+        unsupported("\"simulationMethod\" other than PCISPH (1), IISPH (3) and DFSPH (4)");
     if (flag(cfg, "sim2D", false)) unsupported("\"sim2D\"");
     for (const char *k : {"kernel", "gradKernel"})
         if (cfg.find(k) && num(cfg, k, 4) != 4) unsupported(std::string("\"") + k + "\" other than the precomputed cubic spline (4)");
@@ -543,12 +543,14 @@ static Scene *read_scene(const std::string &path) {
     if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012 && P.boundary_handling_method != SPH_BOUNDARY_BENDER2019 &&
         P.boundary_handling_method != SPH_BOUNDARY_KOSCHIER2017)
         unsupported("\"boundaryHandlingMethod\" " + std::to_string(P.boundary_handling_method));
-    if (S->method == SPH_METHOD_PCISPH) {
-        // TimeStepPCISPH's parameters and defaults (TimeStepPCISPH.cpp:24-31)
-        P.min_iterations = 3;
+    if (S->method == SPH_METHOD_PCISPH || S->method == SPH_METHOD_IISPH) {
+        // TimeStepPCISPH's / TimeStepIISPH's parameters and defaults
+        // (TimeStepPCISPH.cpp:24-31, TimeStepIISPH.cpp:26-33)
+        const bool pci = S->method == SPH_METHOD_PCISPH;
+        P.min_iterations = pci ? 3 : 2;
         P.max_iterations = 100;
         P.max_error = 0.01;
-        if (const JValue *d = cfg.find("PCISPH")) {
+        if (const JValue *d = cfg.find(pci ? "PCISPH" : "IISPH")) {
             P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
             P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
             P.max_error = num(*d, "maxError", P.max_error);

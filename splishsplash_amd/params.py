@@ -102,6 +102,7 @@ class Emitter(C.Structure):
 
 # Simulation simulationMethod values
 METHOD_PCISPH = 1
+METHOD_IISPH = 3
 METHOD_DFSPH = 4
 
 # ParticleState (FluidModel.h:75)

@@ -1,5 +1,6 @@
-"""PCISPH (TimeStepPCISPH) on the device against the FP64 oracle, whose
-PCISPH step is pinned on the CPU (tests/test_oracle_pcisph.py): whole steps
+"""PCISPH (TimeStepPCISPH) and IISPH (TimeStepIISPH) on the device against
+the FP64 oracle, whose steps are pinned on the CPU
+(tests/test_oracle_pcisph.py, tests/test_oracle_iisph.py): whole steps
 from identical states with Akinci2012 particles, Bender2019 volume maps
 (box wall and sphere, sampled again at the predicted positions every
 iteration) and Koschier2017 density maps, with XSPH and standard viscosity;
@@ -16,8 +17,12 @@ import pytest
 
 from oracle import oracle
 from splishsplash_amd import capi, scenes
-from splishsplash_amd.params import (BOUNDARY_BENDER2019, BOUNDARY_KOSCHIER2017, METHOD_PCISPH, VISCOSITY_STANDARD,
-                                     default_params)
+from splishsplash_amd.params import (BOUNDARY_BENDER2019, BOUNDARY_KOSCHIER2017, METHOD_IISPH, METHOD_PCISPH,
+                                     VISCOSITY_STANDARD, default_params)
+
+METHODS = [METHOD_PCISPH, METHOD_IISPH]
+# the pressure persists between IISPH steps (its warm start): synchronised too
+FIELDS = ("position", "velocity", "p / rho^2")
 
 pytestmark = pytest.mark.gpu
 
@@ -26,11 +31,11 @@ def rel(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def build(params, x, v=None, boundary_x=None, vm=None, spheres=()):
+def build(params, x, v=None, boundary_x=None, vm=None, spheres=(), method=METHOD_PCISPH):
     g = capi.TimeStepDFSPH(params)
     o = oracle.OracleDFSPH(params)
     for s in (g, o):
-        s.set_simulation_method(METHOD_PCISPH)
+        s.set_simulation_method(method)
         s.set_fluid(x, v)
         if boundary_x is not None:
             s.add_boundary_particles(boundary_x)
@@ -44,12 +49,13 @@ def build(params, x, v=None, boundary_x=None, vm=None, spheres=()):
     return g, o
 
 
-def stepwise(g, o, steps, fields=("position", "velocity"), pred=False):
+def stepwise(g, o, steps, fields=FIELDS, pred=False, states=False):
     for k in range(steps):
         for nm in fields:
             o.set(nm, g.get(nm).astype(np.float64))
         if pred:
             o.set("predicted position", g.get("predicted position").astype(np.float64))
+        if states:
             o.set("state", g.get("state").astype(np.float64))
         c = g.counters()
         o.set_time(c["time"], c["time_step_size"])
@@ -65,37 +71,40 @@ def stepwise(g, o, steps, fields=("position", "velocity"), pred=False):
     return g.counters()
 
 
-def test_pcisph_akinci_dam_break():
+@pytest.mark.parametrize("method", METHODS)
+def test_akinci_dam_break(method):
     sc = scenes.dam_break_small(per_side=16)
     prm = sc["params"]
     prm.xsph = 0.01
-    g, o = build(prm, sc["x"], sc["v"], boundary_x=sc["boundary_x"])
+    g, o = build(prm, sc["x"], sc["v"], boundary_x=sc["boundary_x"], method=method)
     g.step(30)  # let the column start to collapse
     c = stepwise(g, o, 5)
     assert c["iterations"] >= prm.min_iterations
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("bhm", [BOUNDARY_BENDER2019, BOUNDARY_KOSCHIER2017])
-def test_pcisph_maps(bhm):
+def test_maps(bhm, method):
     x = scenes.lattice_block((14, 14, 14), 0.05, (0.06, 0.06, 0.06), jitter=0.002, seed=5)
     prm = default_params(boundary_handling_method=bhm, xsph=0.01, xsph_boundary=0.02, cfl_factor=0.5,
                          time_step_size=0.002, viscosity_method=VISCOSITY_STANDARD, viscosity=0.01,
                          viscosity_boundary=0.01, max_error=0.05)
     lo, hi = (0.0, 0.0, 0.0), (1.2, 1.0, 1.0)
     vm = (lo, hi, scenes.volume_map_resolution(lo, hi, 0.1, 0.05))
-    g, o = build(prm, x, vm=vm, spheres=[((0.9, 0.2, 0.5), 0.15)])
+    g, o = build(prm, x, vm=vm, spheres=[((0.9, 0.2, 0.5), 0.15)], method=method)
     g.step(25)
     stepwise(g, o, 5)
 
 
-def test_pcisph_with_an_emitter():
+@pytest.mark.parametrize("method", METHODS)
+def test_with_an_emitter(method):
     x = scenes.lattice_block((6, 6, 6), 0.05, (-0.15, 0.0, -0.15), jitter=0.002, seed=3)
     prm = default_params(particle_radius=0.025, boundary_handling_method=-1, cfl_method=0, time_step_size=0.001)
-    g, o = build(prm, x)
+    g, o = build(prm, x, method=method)
     R = np.array([[math.cos(-1.2), -math.sin(-1.2), 0.0], [math.sin(-1.2), math.cos(-1.2), 0.0], [0.0, 0.0, 1.0]])
     for s in (g, o):
         s.add_emitter(5, 4, [0.0, 0.45, 0.0], R, 2.0)
-    stepwise(g, o, 60, pred=True)
+    stepwise(g, o, 60, pred=method == METHOD_PCISPH, states=True)
     assert g.emitter_counts()[0] == 3 * 20
     np.testing.assert_array_equal(g.get("state"), o.get("state").astype(np.uint32))
 
@@ -107,6 +116,9 @@ def test_method_switch_and_errors():
     g.add_boundary_particles(sc["boundary_x"])
     g.step(3)
     g.set_simulation_method(METHOD_PCISPH)  # between steps
+    g.step(3)
+    assert np.isfinite(g.get("position")).all() and g.counters()["iterations"] >= 2
+    g.set_simulation_method(METHOD_IISPH)
     g.step(3)
     assert np.isfinite(g.get("position")).all() and g.counters()["iterations"] >= 2
     with pytest.raises(capi.SPHError):

@@ -144,7 +144,7 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
 @pytest.mark.parametrize("change, key", [
     ({"RigidBodies": [{"geometryFile": "unit_box.obj", "isDynamic": True}]}, "dynamic"),
     ({"Emitters": [{"id": "Other", "width": 5}]}, "fluid id"),
-    ({"Configuration": {"simulationMethod": 3}}, "simulationMethod"),
+    ({"Configuration": {"simulationMethod": 0}}, "simulationMethod"),
     ({"Configuration": {"sim2D": True}}, "sim2D"),
     ({"Configuration": {"kernel": 0}}, "kernel"),
     ({"Configuration": {"boundaryHandlingMethod": 3}}, "boundaryHandlingMethod"),
