@@ -4,6 +4,7 @@
 // samples its fluid blocks as SimulatorBase::createFluidBlocks does
 // (Simulator/SimulatorBase.cpp:1418-1516), turns its static boxes and
 // spheres into the handle's boundaries and creates a DFSPH handle.
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -13,6 +14,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "common.hpp"
@@ -483,6 +485,55 @@ static SceneBody body_from_mesh(const std::vector<double> &V, const std::vector<
 // spacing gives the same boundary volumes up to the sampling noise.)
 static void sample_surface(const SceneBody &b, double r, std::vector<float> &out) {
     const double sp = 2.0 * r;
+    if (b.kind == 2) {
+        // any other closed mesh: each triangle on a barycentric lattice of
+        // at most 2r spacing, then a greedy thinning (hash grid of cell
+        // 0.7 * 2r) that keeps a point only if no kept point lies within
+        // 0.7 * 2r -- the points shared by neighbouring triangles collapse,
+        // the spacing stays between 1.4 r and 2r. (The reference samples
+        // with Poisson disks by default, StaticBoundarySimulator.cpp:96-124;
+        // Akinci's volumes adapt to the local sample density either way.)
+        const double dmin = 0.7 * sp, dmin2 = dmin * dmin;
+        std::unordered_map<uint64_t, std::vector<std::array<double, 3>>> grid;
+        auto key = [&](const double *p, int dx, int dy, int dz) {
+            const int64_t i = (int64_t)std::floor(p[0] / dmin) + dx, j = (int64_t)std::floor(p[1] / dmin) + dy,
+                          k = (int64_t)std::floor(p[2] / dmin) + dz;
+            return ((uint64_t)(i + (1 << 20)) << 42) | ((uint64_t)(j + (1 << 20)) << 21) | (uint64_t)(k + (1 << 20));
+        };
+        auto add = [&](const double *p) {
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        auto it = grid.find(key(p, dx, dy, dz));
+                        if (it == grid.end()) continue;
+                        for (const auto &q : it->second) {
+                            const double e0 = q[0] - p[0], e1 = q[1] - p[1], e2 = q[2] - p[2];
+                            if (e0 * e0 + e1 * e1 + e2 * e2 < dmin2) return;
+                        }
+                    }
+            grid[key(p, 0, 0, 0)].push_back({p[0], p[1], p[2]});
+            for (int c = 0; c < 3; c++) out.push_back((float)p[c]);
+        };
+        for (size_t t = 0; t + 2 < b.faces.size(); t += 3) {
+            const double *A = &b.verts[3 * b.faces[t]], *B = &b.verts[3 * b.faces[t + 1]],
+                         *C = &b.verts[3 * b.faces[t + 2]];
+            double e1[3], e2[3], e3[3];
+            for (int c = 0; c < 3; c++) {
+                e1[c] = B[c] - A[c];
+                e2[c] = C[c] - A[c];
+                e3[c] = C[c] - B[c];
+            }
+            auto len = [](const double *e) { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]); };
+            const int m = std::max(1, (int)std::ceil(std::max(len(e1), std::max(len(e2), len(e3))) / sp));
+            for (int i = 0; i <= m; i++)
+                for (int j = 0; j <= m - i; j++) {
+                    double p[3];
+                    for (int c = 0; c < 3; c++) p[c] = A[c] + (e1[c] * i + e2[c] * j) / m;
+                    add(p);
+                }
+        }
+        return;
+    }
     if (b.kind == 0) {
         int n[3];
         for (int c = 0; c < 3; c++) n[c] = std::max(1, (int)std::lround((b.hi[c] - b.lo[c]) / sp));
@@ -692,8 +743,6 @@ static Scene *read_scene(const std::string &path) {
             std::vector<unsigned> faces;
             const std::vector<double> verts = obj_vertices(mesh, &faces);
             SceneBody body = body_from_mesh(verts, faces, sc, tr, ax, angle, mesh);
-            if (body.kind == 2 && P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012)
-                unsupported("Akinci2012 sampling of the general mesh \"" + mesh + "\" (boxes and spheres only)");
             body.invert = flag(b, "mapInvert", false);
             body.thickness = num(b, "mapThickness", 0.0);
             double res[3] = {20, 20, 20};
@@ -704,10 +753,11 @@ static Scene *read_scene(const std::string &path) {
         }
     }
     // emitter housings after the scene's bodies, as the reference appends
-    // them to the boundary models; Akinci2012 has no mesh sampling here, so
-    // its emitters run without a housing
-    if (P.boundary_handling_method != SPH_BOUNDARY_AKINCI2012)
-        S->bodies.insert(S->bodies.end(), housings.begin(), housings.end());
+    // them to the boundary models
+    for (const SceneBody &hb : housings) {
+        if (P.boundary_handling_method == SPH_BOUNDARY_AKINCI2012) sample_surface(hb, r, S->bx);
+        S->bodies.push_back(hb);
+    }
     return S.release();
 }
 

@@ -148,3 +148,16 @@ def test_emitter_scene_matches_oracle(tmp_path):
             assert rel(g.get(nm), o.get(nm)) <= 1e-4, (k, nm)
     e, _ = g.emitter_counts()
     assert e >= 24 * 3 + 37  # the box emitter's layers and the circle emitter's first disc
+
+
+def test_akinci_emitter_scene_with_sampled_housings(tmp_path):
+    # Akinci2012: the wall box and the emitters' housings become boundary
+    # particles (the housings sampled from their meshes); the scene steps
+    # and emits
+    from test_scene import emitter_scene, write_scene
+    sc = capi.Scene(write_scene(tmp_path, emitter_scene(0)))
+    g = sc.create_dfsph()
+    g.step(60)
+    assert g.emitter_counts()[0] >= 24 * 2
+    c = g.counters()
+    assert c["flags"] == 0 and np.isfinite(g.get("position")).all()

@@ -153,7 +153,6 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
     ({"FluidBlocks": [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]},
                       {"id": "Other", "start": [1, 0, 0], "end": [1.5, 0.5, 0.5]}]}, "fluid id"),
     ({"RigidBodies": [{"geometryFile": "triangle.obj"}]}, "closed"),
-    ({"Configuration": {"boundaryHandlingMethod": 0}, "RigidBodies": [{"geometryFile": "tetra.obj"}]}, "Akinci2012"),
 ])
 def test_unsupported_content_is_rejected(tmp_path, change, key):
     (tmp_path / "tetra.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 3 2\nf 1 2 4\nf 1 4 3\nf 2 3 4\n")
@@ -243,9 +242,41 @@ def test_emitters_and_their_housings(tmp_path):
         hi = (R @ (np.array([0.5, 0.55, 0.55]) * size)) + t
         np.testing.assert_allclose(np.minimum(lo, hi), body["lo"], atol=1e-12)
         np.testing.assert_allclose(np.maximum(lo, hi), body["hi"], atol=1e-12)
-    # Akinci2012: no housing (no mesh sampling), emitters still read
+    # Akinci2012: the housings are sampled as boundary particles too
     sc0 = capi.Scene(write_scene(tmp_path, emitter_scene(0)))
-    assert sc0.nbodies == 1 and len(sc0.emitters()[0]) == 2
+    assert sc0.nbodies == 3 and len(sc0.emitters()[0]) == 2
+    walls = emitter_scene(0)
+    del walls["Emitters"]
+    assert sc0.nb > capi.Scene(write_scene(tmp_path, walls)).nb + 100
+
+
+def test_akinci_sampling_of_a_general_mesh(tmp_path):
+    # a rotated box (a general mesh): the boundary particles lie on its
+    # surface, no two closer than 0.7 * 2r, and every point of the surface
+    # has a particle within 2r
+    from oracle import oracle
+    r = 0.025
+    scene = {"Configuration": {"particleRadius": r, "boundaryHandlingMethod": 0},
+             "FluidBlocks": [{"start": [0, 0, 0], "end": [0.3, 0.3, 0.3]}],
+             "RigidBodies": [{"geometryFile": "unit_box.obj", "rotationAxis": [1, 1, 0], "rotationAngle": 0.6,
+                              "translation": [1, 1, 1], "scale": [0.5, 0.4, 0.3]}]}
+    sc = capi.Scene(write_scene(tmp_path, scene))
+    _, _, bx = sc.particles()
+    assert sc.body(0)["kind"] == "mesh" and len(bx) > 100
+    v, f = sc.body_mesh(0)
+    d = oracle.mesh_sdf(v, f, bx.astype(np.float64))
+    assert np.abs(d).max() < 1e-6
+    dist, _ = cKDTree(bx).query(bx, k=2)
+    assert dist[:, 1].min() >= 0.7 * 2 * r * (1 - 1e-6)
+    rng = np.random.default_rng(0)
+    tri = v[f]
+    pick = rng.integers(0, len(f), 2000)
+    a, b = rng.random((2, 2000, 1))
+    flip = (a + b) > 1
+    a, b = np.where(flip, 1 - a, a), np.where(flip, 1 - b, b)
+    pts = tri[pick, 0] + a * (tri[pick, 1] - tri[pick, 0]) + b * (tri[pick, 2] - tri[pick, 0])
+    near, _ = cKDTree(bx).query(pts)
+    assert near.max() <= 2 * r
 
 
 def test_pcisph_scene_parameters(tmp_path):
