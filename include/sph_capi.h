@@ -478,7 +478,7 @@ typedef struct sph_scene *sph_scene_t;
  * "RigidBodies" from OBJ meshes (Bender2019 / Koschier2017: one volume or
  * density map each with mapInvert / mapThickness / mapResolution, analytic
  * for an axis-aligned box or a sphere, a mesh map otherwise; Akinci2012:
- * box and sphere surfaces sampled at 2r), and "Emitters" (SimulatorBase::createEmitters,
+ * surfaces sampled at 2r, any closed mesh included), and "Emitters" (SimulatorBase::createEmitters,
  * SimulatorBase.cpp:1286-1384: width, height, translation, rotationAxis,
  * rotationAngle, velocity, type, emitStartTime, emitEndTime; the fluid
  * material's maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max;
