@@ -55,7 +55,8 @@ extern "C" {
 #define SPH_BOUNDARY_AKINCI2012  0
 #define SPH_BOUNDARY_KOSCHIER2017 1  /* density maps of the boundary objects */
 #define SPH_BOUNDARY_BENDER2019  2
-#define SPH_METHOD_PCISPH        1   /* Simulation simulationMethod values */
+#define SPH_METHOD_WCSPH         0   /* Simulation simulationMethod values */
+#define SPH_METHOD_PCISPH        1
 #define SPH_METHOD_IISPH         3
 #define SPH_METHOD_DFSPH         4
 #define SPH_VISCOSITY_NONE       0
@@ -314,6 +315,11 @@ int sph_dfsph_emit_particles(sph_dfsph_t s, const float *x, const float *v,
  * CFL, states and emitters are shared. Call before the first step or
  * between steps. */
 int sph_dfsph_set_simulation_method(sph_dfsph_t s, int method);
+/* SPH_METHOD_WCSPH runs TimeStepWCSPH::step (WCSPH/TimeStepWCSPH.cpp:74-199):
+ * the state equation p = stiffness ((rho / rho0)^exponent - 1) on the
+ * density clamped to rho0, the pressure accelerations and an explicit
+ * integration. TimeStepWCSPH's stiffness and exponent (defaults 50, 7). */
+int sph_dfsph_set_wcsph(sph_dfsph_t s, double stiffness, double exponent);
 
 /* Emitter system (SPlisHSPlasH/EmitterSystem.h, Emitter.h). An emitter
  * places a width x height grid of particles (type 0, box) or a disc of width
@@ -511,6 +517,10 @@ int sph_scene_emitter(sph_scene_t s, int k, sph_emitter_t *out);
 int sph_scene_emitter_settings(sph_scene_t s, int *n_emitters,
                                size_t *max_emitter_particles, int *reuse,
                                double box_min[3], double box_max[3]);
+/* The scene's simulationMethod and its "WCSPH" section's stiffness and
+ * exponent (defaults 50 and 7). */
+int sph_scene_simulation_method(sph_scene_t s, int *method,
+                                double *wcsph_stiffness, double *wcsph_exponent);
 /* A DFSPH handle with the scene's parameters, fluid, boundaries and
  * emitters. */
 int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out);

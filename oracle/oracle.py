@@ -54,6 +54,7 @@ def lib():
             "orc_emitter_counts": (C.c_int, [C.c_void_p, _szp, _szp]),
             "orc_set_simulation_method": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_pcisph_factor": (C.c_int, [C.c_void_p, _dp]),
+            "orc_set_wcsph": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
             "orc_init": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
             "orc_stage": (C.c_int, [C.c_void_p, C.c_int]),
@@ -297,6 +298,9 @@ class OracleDFSPH:
 
     def set_simulation_method(self, m):
         _chk(lib().orc_set_simulation_method(self.h, int(m)))
+
+    def set_wcsph(self, stiffness, exponent):
+        _chk(lib().orc_set_wcsph(self.h, float(stiffness), float(exponent)))
 
     def pcisph_factor(self):
         f = C.c_double(0.0)

@@ -2058,8 +2058,83 @@ void Sim::step_iisph() {
     time += h_step;
 }
 
+// TimeStepWCSPH::step (WCSPH/TimeStepWCSPH.cpp:67-131) and
+// computePressureAccels (:133-211).
+void Sim::step_wcsph() {
+    const double h = dt;  // the step's starting h integrates and advances the time
+    neighborhood_search();
+    volume_maps();
+    clear_accelerations();
+    densities();
+    viscosity_standard();
+    viscosity_weiler();
+    xsph();
+    const double density0 = P.density0;
+    for (size_t i = 0; i < n; i++) {
+        rho[i] = std::max(rho[i], density0);
+        p[i] = wcK * (std::pow(rho[i] / density0, wcGamma) - 1.0);
+    }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; i++) {
+        const double *xi = &x[3 * i];
+        double *ai = &ap[3 * i];
+        ai[0] = ai[1] = ai[2] = 0.0;
+        const double dpi = p[i] / (rho[i] * rho[i]);
+        for (uint32_t k = nf.off[i]; k < nf.off[i + 1]; k++) {
+            const size_t j = nf.idx[k];
+            const double dpj = p[j] / (rho[j] * rho[j]);
+            const double r[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+            double g[3];
+            K.gradW(r, g);
+            for (int c = 0; c < 3; c++) ai[c] -= density0 * V * (dpi + dpj) * g[c];
+        }
+        const double dpb = dpi + p[i] / (density0 * density0);
+        if (has_boundary_particles()) {
+            for (uint32_t k = nbd.off[i]; k < nbd.off[i + 1]; k++) {
+                const uint32_t j = nbd.idx[k];
+                const double d[3] = {xi[0] - bx[3 * j], xi[1] - bx[3 * j + 1], xi[2] - bx[3 * j + 2]};
+                double g[3];
+                K.gradW(d, g);
+                for (int c = 0; c < 3; c++) ai[c] -= density0 * bV[j] * dpb * g[c];
+            }
+        } else if (has_density_map()) {
+            for (size_t o = 0; o < vms.size(); o++)
+                if (vmV[o * n + i] != 0.0)
+                    for (int c = 0; c < 3; c++) ai[c] += density0 * dpb * vmG[3 * (o * n + i) + c];
+        } else if (has_volume_map()) {
+            for (size_t o = 0; o < vms.size(); o++) {
+                const double Vj = vmV[o * n + i];
+                if (Vj > 0.0) {
+                    const double *xj = &vmX[3 * (o * n + i)];
+                    const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                    double g[3];
+                    K.gradW(d, g);
+                    for (int c = 0; c < 3; c++) ai[c] -= density0 * Vj * dpb * g[c];
+                }
+            }
+        }
+    }
+    cfl();  // reads the non-pressure acceleration only
+    iterations = iterationsV = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (!active(i)) continue;
+        for (int c = 0; c < 3; c++) {
+            a[3 * i + c] += ap[3 * i + c];
+            v[3 * i + c] += a[3 * i + c] * h;
+            x[3 * i + c] += v[3 * i + c] * h;
+        }
+    }
+    emitter_system_step();
+    time += h;
+}
+
 // TimeStepDFSPH::step (TimeStepDFSPH.cpp:117-249).
 void Sim::step() {
+    if (method == SPH_METHOD_WCSPH) {
+        step_wcsph();
+        steps++;
+        return;
+    }
     if (method == SPH_METHOD_PCISPH) {
         step_pcisph();
         steps++;
@@ -2154,7 +2229,7 @@ bool field_of(Sim *s, const char *name, FieldRef &f) {
     else if (n == "density") f = {&s->rho, 1};
     else if (n == "factor") f = {&s->factor, 1};
     else if (n == "advected density") f = {&s->rho_adv, 1};
-    else if (n == "p / rho^2") f = {&s->p, 1};
+    else if (n == "p / rho^2" || n == "pressure") f = {&s->p, 1};
     else if (n == "p_v / rho^2") f = {&s->pv, 1};
     else if (n == "pressure acceleration") f = {&s->ap, 3};
     else if (n == "velocity difference") f = {&s->vdiff, 3};
@@ -2312,9 +2387,16 @@ int orc_pcisph_factor(void *h, double *out) {
     return SPH_OK;
 }
 int orc_set_simulation_method(void *h, int m) {
-    if (m != SPH_METHOD_DFSPH && m != SPH_METHOD_PCISPH && m != SPH_METHOD_IISPH)
-        return fail("simulation method: PCISPH (1), IISPH (3) or DFSPH (4)");
+    if (m != SPH_METHOD_DFSPH && m != SPH_METHOD_PCISPH && m != SPH_METHOD_IISPH && m != SPH_METHOD_WCSPH)
+        return fail("simulation method: WCSPH (0), PCISPH (1), IISPH (3) or DFSPH (4)");
     ((Sim *)h)->method = m;
+    return SPH_OK;
+}
+int orc_set_wcsph(void *h, double stiffness, double exponent) {
+    if (!(stiffness >= 1e-6) || !(exponent >= 1e-6) || !std::isfinite(stiffness) || !std::isfinite(exponent))
+        return fail("WCSPH: stiffness and exponent must be >= 1e-6");
+    ((Sim *)h)->wcK = stiffness;
+    ((Sim *)h)->wcGamma = exponent;
     return SPH_OK;
 }
 int orc_init(void *h) {

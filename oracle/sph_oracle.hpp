@@ -184,6 +184,9 @@ public:
     // IISPH (TimeStepIISPH, SimulationDataIISPH)
     std::vector<double> dii, dijpj, lastP;
     void step_iisph();
+    // WCSPH (TimeStepWCSPH)
+    double wcK = 50.0, wcGamma = 7.0;
+    void step_wcsph();
     void densities();
     void dfsph_factor();
     void divergence_setup();

@@ -88,9 +88,11 @@ _SIGS = {
     "sph_scene_emitter": (C.c_int, [_vp, C.c_int, C.c_void_p]),
     "sph_scene_emitter_settings": (C.c_int, [_vp, C.POINTER(C.c_int), _szp, C.POINTER(C.c_int), _dp, _dp]),
     "sph_scene_create_dfsph": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sph_scene_simulation_method": (C.c_int, [_vp, C.POINTER(C.c_int), _dp, _dp]),
     "sph_dfsph_emit_particles": (C.c_int, [_vp, _fp, _fp, C.c_size_t]),
     "sph_dfsph_add_emitter": (C.c_int, [_vp, C.c_void_p]),
     "sph_dfsph_set_simulation_method": (C.c_int, [_vp, C.c_int]),
+    "sph_dfsph_set_wcsph": (C.c_int, [_vp, C.c_double, C.c_double]),
     "sph_dfsph_set_emitter_reuse": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "sph_dfsph_set_max_emitter_particles": (C.c_int, [_vp, C.c_size_t]),
     "sph_dfsph_emitter_counts": (C.c_int, [_vp, _szp, _szp]),
@@ -411,6 +413,13 @@ class Scene:
                         "emit_start_time": e.emit_start_time, "emit_end_time": e.emit_end_time})
         return out, {"max_emitter_particles": me.value, "reuse": bool(ru.value), "box_min": lo, "box_max": hi}
 
+    def simulation_method(self):
+        """The scene's simulationMethod and, for WCSPH, its stiffness and
+        exponent: (method, stiffness, exponent)."""
+        m, k, g = C.c_int(), C.c_double(), C.c_double()
+        _chk(lib().sph_scene_simulation_method(self.h, C.byref(m), C.byref(k), C.byref(g)))
+        return m.value, k.value, g.value
+
     def create_dfsph(self):
         """A TimeStepDFSPH with the scene's parameters, fluid and boundaries."""
         h = _vp()
@@ -552,9 +561,15 @@ class TimeStepDFSPH:
         self.n += len(x)
 
     def set_simulation_method(self, method):
-        """Simulation::setSimulationMethod for this handle: METHOD_DFSPH (4)
-        or METHOD_PCISPH (1)."""
+        """Simulation::setSimulationMethod for this handle: METHOD_DFSPH (4),
+        METHOD_PCISPH (1), METHOD_IISPH (3) or METHOD_WCSPH (0)."""
         _chk(lib().sph_dfsph_set_simulation_method(self.h, int(method)))
+
+    def set_wcsph(self, stiffness=50.0, exponent=7.0):
+        """TimeStepWCSPH's stiffness and exponent of the state equation
+        p = stiffness ((rho / rho0)^exponent - 1) (TimeStepWCSPH.cpp:56-63,
+        106); both at least 1e-6."""
+        _chk(lib().sph_dfsph_set_wcsph(self.h, float(stiffness), float(exponent)))
 
     def add_emitter(self, width, height, pos, rotation, velocity, type=0, emit_start_time=0.0,
                     emit_end_time=float(np.finfo(np.float64).max)):

@@ -2560,6 +2560,10 @@ void DFSPH::step() {
         step_iisph();
         return;
     }
+    if (method_ == SPH_METHOD_WCSPH) {
+        step_wcsph();
+        return;
+    }
     neighborhood_search();
     volume_maps();
     const bool div = P.enable_divergence_solver != 0;
@@ -2731,7 +2735,7 @@ void DFSPH::get_field(const std::string &name, void *host, size_t bytes) {
     else if (name == "density") f1 = rho_.get();
     else if (name == "factor") f1 = alpha_.get();
     else if (name == "advected density") f1 = radv_.get();
-    else if (name == "p / rho^2") f1 = p_.get();
+    else if (name == "p / rho^2" || name == "pressure") f1 = p_.get();  // "pressure": WCSPH, PCISPH, IISPH
     else if (name == "p_v / rho^2") f1 = pv_.get();
     else if (name == "id") u1 = id_.get();
     else if (name == "state") u1 = pst_.get();
@@ -2786,7 +2790,7 @@ void DFSPH::set_field(const std::string &name, const void *host, size_t bytes) {
     else if (name == "density") f1 = rho_.get();
     else if (name == "factor") f1 = alpha_.get();
     else if (name == "advected density") f1 = radv_.get();
-    else if (name == "p / rho^2") f1 = p_.get();
+    else if (name == "p / rho^2" || name == "pressure") f1 = p_.get();  // "pressure": WCSPH, PCISPH, IISPH
     else if (name == "p_v / rho^2") f1 = pv_.get();
     else if (name != "state") throw Error(SPH_ERR_NOTFOUND, "unknown or read-only field '" + name + "'");
     SPH_REQUIRE(bytes == n * comps * 4, SPH_ERR_INVALID, "size mismatch for field '" + name + "'");

@@ -242,6 +242,15 @@ private:
     DBuf<float4> iid_, iidp_;
     DBuf<float> ilp_, iq_;
     void step_iisph();
+    // WCSPH (wcsph.hip): state equation stiffness and exponent
+    double wc_k_ = 50.0, wc_gamma_ = 7.0;
+    void step_wcsph();
+
+public:
+    // TimeStepWCSPH's stiffness and exponent (defaults 50 and 7).
+    void set_wcsph(double stiffness, double exponent);
+
+private:
     void stiff_to_w(bool pressure);
     template <bool PRESSURE>
     void iteration_any(bool force);

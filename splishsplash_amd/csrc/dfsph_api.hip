@@ -141,6 +141,13 @@ int sph_dfsph_set_simulation_method(sph_dfsph_t s, int method) {
     });
 }
 
+int sph_dfsph_set_wcsph(sph_dfsph_t s, double stiffness, double exponent) {
+    return guarded([&] {
+        DCHK(s);
+        s->s->set_wcsph(stiffness, exponent);
+    });
+}
+
 int sph_dfsph_add_emitter(sph_dfsph_t s, const sph_emitter_t *e) {
     return guarded([&] {
         DCHK(s);

@@ -260,8 +260,8 @@ double pcisph_factor(double r, double V) {
 }
 
 void DFSPH::set_method(int m) {
-    SPH_REQUIRE(m == SPH_METHOD_DFSPH || m == SPH_METHOD_PCISPH || m == SPH_METHOD_IISPH, SPH_ERR_INVALID,
-                "simulation method: only PCISPH (1), IISPH (3) and DFSPH (4) are implemented");
+    SPH_REQUIRE(m == SPH_METHOD_DFSPH || m == SPH_METHOD_PCISPH || m == SPH_METHOD_IISPH || m == SPH_METHOD_WCSPH,
+                SPH_ERR_INVALID, "simulation method: only WCSPH (0), PCISPH (1), IISPH (3) and DFSPH (4) are implemented");
     if (m == method_) return;
     method_ = m;
     if (m == SPH_METHOD_PCISPH && initialized_) {

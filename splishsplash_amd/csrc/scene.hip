@@ -221,6 +221,7 @@ struct Scene {
     std::vector<float> x, v, bx;  // fluid positions, velocities, Akinci particles
     std::vector<SceneBody> bodies;
     int method = SPH_METHOD_DFSPH;  // simulationMethod
+    double wc_k = 50.0, wc_gamma = 7.0;  // the "WCSPH" section's stiffness and exponent
     // emitters and the fluid's emitter settings (FluidParameterObject
     // maxEmitterParticles, emitterReuseParticles, emitterBoxMin/Max)
     std::vector<sph_emitter_t> emitters;
@@ -578,8 +579,9 @@ static Scene *read_scene(const std::string &path) {
     const JValue *cfgp = root.find("Configuration");
     const JValue &cfg = cfgp ? *cfgp : empty_obj;
     S->method = (int)num(cfg, "simulationMethod", 4);
-    if (S->method != SPH_METHOD_DFSPH && S->method != SPH_METHOD_PCISPH && S->method != SPH_METHOD_IISPH)
-        unsupported("\"simulationMethod\" other than PCISPH (1), IISPH (3) and DFSPH (4)");
+    if (S->method != SPH_METHOD_DFSPH && S->method != SPH_METHOD_PCISPH && S->method != SPH_METHOD_IISPH &&
+        S->method != SPH_METHOD_WCSPH)
+        unsupported("\"simulationMethod\" other than WCSPH (0), PCISPH (1), IISPH (3) and DFSPH (4)");
     if (flag(cfg, "sim2D", false)) unsupported("\"sim2D\"");
     for (const char *k : {"kernel", "gradKernel"})
         if (cfg.find(k) && num(cfg, k, 4) != 4) unsupported(std::string("\"") + k + "\" other than the precomputed cubic spline (4)");
@@ -606,6 +608,14 @@ static Scene *read_scene(const std::string &path) {
             P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
             P.max_error = num(*d, "maxError", P.max_error);
         }
+    } else if (S->method == SPH_METHOD_WCSPH) {
+        // TimeStepWCSPH's stiffness and exponent (TimeStepWCSPH.cpp:27-28, 56-63)
+        if (const JValue *d = cfg.find("WCSPH")) {
+            S->wc_k = num(*d, "stiffness", S->wc_k);
+            S->wc_gamma = num(*d, "exponent", S->wc_gamma);
+        }
+        SPH_REQUIRE(S->wc_k >= 1e-6 && S->wc_gamma >= 1e-6 && std::isfinite(S->wc_k) && std::isfinite(S->wc_gamma),
+                    SPH_ERR_INVALID, "scene: WCSPH stiffness and exponent must be >= 1e-6");
     } else if (const JValue *d = cfg.find("DFSPH")) {
         P.min_iterations = (int)num(*d, "minIterations", P.min_iterations);
         P.max_iterations = (int)num(*d, "maxIterations", P.max_iterations);
@@ -834,6 +844,15 @@ int sph_scene_emitter_settings(sph_scene_t s, int *n_emitters, size_t *max_emitt
     });
 }
 
+int sph_scene_simulation_method(sph_scene_t s, int *method, double *wcsph_stiffness, double *wcsph_exponent) {
+    return guarded([&] {
+        SPH_REQUIRE(s, SPH_ERR_INVALID, "null scene");
+        if (method) *method = s->s->method;
+        if (wcsph_stiffness) *wcsph_stiffness = s->s->wc_k;
+        if (wcsph_exponent) *wcsph_exponent = s->s->wc_gamma;
+    });
+}
+
 int sph_scene_body_mesh(sph_scene_t s, int k, double *vertices, size_t *n_vertices, unsigned *faces,
                         size_t *n_faces) {
     return guarded([&] {
@@ -880,6 +899,7 @@ int sph_scene_create_dfsph(sph_scene_t s, sph_dfsph_t *out) {
             }
         };
         chk(sph_dfsph_set_simulation_method(h, S.method));
+        chk(sph_dfsph_set_wcsph(h, S.wc_k, S.wc_gamma));
         chk(sph_dfsph_set_fluid(h, S.x.data(), S.v.data(), S.x.size() / 3));
         if (S.params.boundary_handling_method == SPH_BOUNDARY_AKINCI2012) {
             if (!S.bx.empty()) chk(sph_dfsph_add_boundary_particles(h, S.bx.data(), S.bx.size() / 3));

@@ -101,6 +101,7 @@ class Emitter(C.Structure):
 
 
 # Simulation simulationMethod values
+METHOD_WCSPH = 0
 METHOD_PCISPH = 1
 METHOD_IISPH = 3
 METHOD_DFSPH = 4
@@ -149,7 +150,7 @@ def default_params(**kw):
 
 FIELD_COMPONENTS = {
     "position": 3, "velocity": 3, "acceleration": 3, "density": 1,
-    "factor": 1, "advected density": 1, "p / rho^2": 1, "p_v / rho^2": 1,
+    "factor": 1, "advected density": 1, "p / rho^2": 1, "pressure": 1, "p_v / rho^2": 1,
     "pressure acceleration": 3, "velocity difference": 3,
     "boundary volume": 1, "boundary x": 3, "boundary density": 1, "boundary density gradient": 3,
     "fluid neighbor count": 1, "boundary neighbor count": 1, "id": 1, "state": 1, "predicted position": 3,

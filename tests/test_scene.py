@@ -144,7 +144,7 @@ def test_boxes_spheres_and_akinci_sampling(tmp_path):
 @pytest.mark.parametrize("change, key", [
     ({"RigidBodies": [{"geometryFile": "unit_box.obj", "isDynamic": True}]}, "dynamic"),
     ({"Emitters": [{"id": "Other", "width": 5}]}, "fluid id"),
-    ({"Configuration": {"simulationMethod": 0}}, "simulationMethod"),
+    ({"Configuration": {"simulationMethod": 2}}, "simulationMethod"),
     ({"Configuration": {"sim2D": True}}, "sim2D"),
     ({"Configuration": {"kernel": 0}}, "kernel"),
     ({"Configuration": {"boundaryHandlingMethod": 3}}, "boundaryHandlingMethod"),
@@ -285,6 +285,19 @@ def test_pcisph_scene_parameters(tmp_path):
     sc = capi.Scene(write_scene(tmp_path, scene))
     p = sc.params
     assert (p.min_iterations, p.max_iterations, p.max_error) == (3, 50, 0.05)  # TimeStepPCISPH defaults + section
+    assert sc.simulation_method()[0] == 1
+
+
+def test_wcsph_scene_parameters(tmp_path):
+    blocks = [{"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]
+    sc = capi.Scene(write_scene(tmp_path, {"Configuration": {"simulationMethod": 0}, "FluidBlocks": blocks}))
+    assert sc.simulation_method() == (0, 50.0, 7.0)  # TimeStepWCSPH's defaults
+    scene = {"Configuration": {"simulationMethod": 0, "WCSPH": {"stiffness": 50000, "exponent": 3}},
+             "FluidBlocks": blocks}
+    assert capi.Scene(write_scene(tmp_path, scene)).simulation_method() == (0, 50000.0, 3.0)
+    scene["Configuration"]["WCSPH"]["exponent"] = 0.0
+    with pytest.raises(capi.SPHError):
+        capi.Scene(write_scene(tmp_path, scene))
 
 
 def test_malformed_files_are_errors(tmp_path):
