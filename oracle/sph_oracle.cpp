@@ -2116,6 +2116,7 @@ void Sim::step_wcsph() {
     }
     cfl();  // reads the non-pressure acceleration only
     iterations = iterationsV = 0;
+    lastErr = lastErrV = 0.0;
     for (size_t i = 0; i < n; i++) {
         if (!active(i)) continue;
         for (int c = 0; c < 3; c++) {

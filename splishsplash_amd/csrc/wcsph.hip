@@ -81,6 +81,8 @@ __global__ __launch_bounds__(BLOCK) void k_wc_final(Dev D) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         C->t_prev = C->time;
         C->time += C->h_step;
+        C->it = C->it_v = 0;  // no solver: the counters read 0
+        C->avg_err = C->avg_err_v = 0.0;
     }
     if (i >= D.n) return;
     const bool act = D.state == nullptr || D.state[i] == 0u;
