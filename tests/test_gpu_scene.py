@@ -44,16 +44,19 @@ def test_scene_file_runs_like_the_hand_built_handle():
     assert a.get("position")[:, 1].mean() < x[:, 1].mean()  # the column collapses
 
 
-@pytest.mark.parametrize("bhm", [2, 1])
-def test_scene_file_step_matches_oracle(bhm, tmp_path):
-    # the packaged dam break as is (Bender2019 volume maps) and with
-    # Koschier2017 density maps for the same bodies
+@pytest.mark.parametrize("bhm, method", [(2, 4), (1, 4), (2, 0)])
+def test_scene_file_step_matches_oracle(bhm, method, tmp_path):
+    # the packaged dam break as is (Bender2019 volume maps), with
+    # Koschier2017 density maps for the same bodies, and run by WCSPH
     import json
     import shutil
     path = os.path.join(DATA, "dam_break.json")
-    if bhm != 2:
+    if bhm != 2 or method != 4:
         scene = json.load(open(path))
         scene["Configuration"]["boundaryHandlingMethod"] = bhm
+        scene["Configuration"]["simulationMethod"] = method
+        if method == 0:
+            scene["Configuration"]["WCSPH"] = {"stiffness": 20000.0, "exponent": 7.0}
         for f in os.listdir(DATA):
             if f.endswith(".obj"):
                 shutil.copy(os.path.join(DATA, f), tmp_path / f)
@@ -61,9 +64,13 @@ def test_scene_file_step_matches_oracle(bhm, tmp_path):
         json.dump(scene, open(path, "w"))
     sc = capi.Scene(path)
     assert sc.params.boundary_handling_method == bhm
+    m, k, gamma = sc.simulation_method()
+    assert m == method and (method != 0 or k == 20000.0)
     g = sc.create_dfsph()
     g.step(40)
     o = oracle.OracleDFSPH(sc.params)
+    o.set_simulation_method(m)
+    o.set_wcsph(k, gamma)
     o.set_fluid(g.get("position"), g.get("velocity"))
     for k in range(sc.nbodies):
         d = sc.body(k)
