@@ -476,11 +476,14 @@ typedef struct sph_scene *sph_scene_t;
  * for what this time step runs. Read: "Configuration" (particleRadius,
  * timeStepSize, gravitation, cflMethod, cflFactor, cflMin/MaxTimeStepSize,
  * boundaryHandlingMethod, simulationMethod 4 (DFSPH) with the "DFSPH"
- * section or 1 (PCISPH) with the "PCISPH" section), the fluid's "Materials"
+ * section, 1 (PCISPH) or 3 (IISPH) with their sections, or 0 (WCSPH) with
+ * its "WCSPH" section), the fluid's "Materials"
  * entry (density0, viscosityMethod 0/1/6 with its "Standard viscosity" or
  * "Weiler et al. 2018" section, the "XSPH" section), "FluidBlocks" sampled
  * as SimulatorBase::createFluidBlocks (Simulator/SimulatorBase.cpp:
- * 1418-1516) with their initial velocities (:1079-1093), and static
+ * 1418-1516) with their initial velocities (:1079-1093), "FluidModels"
+ * from .bgeo particle files after them (R (s x) + t, then the initial
+ * velocities; :1095-1260), and static
  * "RigidBodies" from OBJ meshes (Bender2019 / Koschier2017: one volume or
  * density map each with mapInvert / mapThickness / mapResolution, analytic
  * for an axis-aligned box or a sphere, a mesh map otherwise; Akinci2012:

@@ -5,6 +5,7 @@
 // (Simulator/SimulatorBase.cpp:1418-1516), turns its static boxes and
 // spheres into the handle's boundaries and creates a DFSPH handle.
 #include <array>
+#include <cctype>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -626,10 +627,8 @@ static Scene *read_scene(const std::string &path) {
     }
     const double r = P.particle_radius;
 
-    for (const char *k : {"FluidModels", "AnimationFields"}) {
-        const JValue *v = root.find(k);
-        if (v && v->kind == JValue::ARR && !v->arr.empty()) unsupported(std::string("\"") + k + "\"");
-    }
+    if (const JValue *v = root.find("AnimationFields"))
+        if (v->kind == JValue::ARR && !v->arr.empty()) unsupported("\"AnimationFields\"");
 
     // FluidBlocks (one fluid id: one fluid model)
     std::string fluid_id;
@@ -651,7 +650,66 @@ static Scene *read_scene(const std::string &path) {
             sample_block(r, mode, bmin, bmax, tr, sc, v0, w0, S->x, S->v);
         }
     }
-    if (S->x.empty()) throw Error(SPH_ERR_INVALID, "scene: no fluid particles (\"FluidBlocks\" missing or empty)");
+    // FluidModels from particle files, after the blocks (SimulatorBase.cpp:
+    // 1095-1260): x = R (s x_file) + t with s the first scale component
+    // (PartioReaderWriter::readParticles), then the velocities of the model's
+    // particles v = v0 + w0 x (x - centre of mass) (setInitialVelocity,
+    // SimulatorBase.cpp:1079-1093), which replaces any file velocity.
+    if (const JValue *models = root.find("FluidModels")) {
+        for (const JValue &m : models->arr) {
+            const std::string id = text(m, "id", "Fluid");
+            if (!fluid_id.empty() && id != fluid_id) unsupported("more than one fluid id (\"" + fluid_id + "\", \"" + id + "\")");
+            fluid_id = id;
+            std::string file = text(m, "particleFile", "");
+            if (file.empty()) throw Error(SPH_ERR_INVALID, "scene: a fluid model without \"particleFile\"");
+            std::string ext = file.size() > 5 ? file.substr(file.size() - 5) : "";
+            for (char &ch : ext) ch = (char)std::tolower((unsigned char)ch);
+            if (ext != ".bgeo") unsupported("\"FluidModels\" other than particle files (.bgeo): " + file);
+            if (file[0] != '/') file = dir_of(path) + "/" + file;
+            double tr[3] = {0, 0, 0}, sc[3] = {1, 1, 1}, ax[3] = {1, 0, 0}, v0[3] = {0, 0, 0}, w0[3] = {0, 0, 0},
+                   R[9];
+            vec3(m, "translation", tr);
+            vec3(m, "scale", sc);
+            vec3(m, "rotationAxis", ax);
+            vec3(m, "initialVelocity", v0);
+            vec3(m, "initialAngularVelocity", w0);
+            axis_angle(ax, num(m, "rotationAngle", 0.0), R);
+            sph_bgeo_t f = nullptr;
+            if (sph_bgeo_open(file.c_str(), &f) != SPH_OK)
+                throw Error(SPH_ERR_INVALID, "scene: cannot read the fluid model \"" + file + "\": " + sph_last_error());
+            size_t np = 0;
+            int na = 0;
+            std::vector<float> px;
+            const int rc = sph_bgeo_count(f, &np, &na);
+            if (rc == SPH_OK) px.resize(3 * np);
+            const int rc2 = rc == SPH_OK && np ? sph_bgeo_data(f, "position", px.data(), px.size() * sizeof(float)) : rc;
+            sph_bgeo_close(f);
+            if (rc2 != SPH_OK) throw Error(SPH_ERR_INVALID, "scene: fluid model \"" + file + "\": " + sph_last_error());
+            double com[3] = {0, 0, 0};
+            std::vector<double> xs(3 * np);
+            for (size_t k = 0; k < np; k++) {
+                double q[3];
+                for (int c = 0; c < 3; c++) q[c] = sc[0] * (double)px[3 * k + c];
+                for (int c = 0; c < 3; c++) {
+                    xs[3 * k + c] = (R[3 * c] * q[0] + R[3 * c + 1] * q[1]) + R[3 * c + 2] * q[2] + tr[c];
+                    com[c] += (double)(float)xs[3 * k + c];
+                }
+            }
+            for (int c = 0; c < 3; c++) com[c] /= np ? (double)np : 1.0;
+            for (size_t k = 0; k < np; k++) {
+                const double d[3] = {(double)(float)xs[3 * k] - com[0], (double)(float)xs[3 * k + 1] - com[1],
+                                     (double)(float)xs[3 * k + 2] - com[2]};
+                const double vv[3] = {v0[0] + (w0[1] * d[2] - w0[2] * d[1]), v0[1] + (w0[2] * d[0] - w0[0] * d[2]),
+                                      v0[2] + (w0[0] * d[1] - w0[1] * d[0])};
+                for (int c = 0; c < 3; c++) {
+                    S->x.push_back((float)xs[3 * k + c]);
+                    S->v.push_back((float)vv[c]);
+                }
+            }
+        }
+    }
+    if (S->x.empty())
+        throw Error(SPH_ERR_INVALID, "scene: no fluid particles (\"FluidBlocks\" and \"FluidModels\" missing or empty)");
     if (fluid_id.empty()) fluid_id = "Fluid";
 
     // Materials: the fluid's density, viscosity and XSPH sections

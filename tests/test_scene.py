@@ -318,3 +318,33 @@ def test_packaged_dam_break_scene():
     assert sc.n == 19 ** 3 and sc.nbodies == 2
     assert sc.params.viscosity_method == 1 and sc.params.xsph == 0.01
     assert math.isclose(sc.body(1)["radius"], 0.25, rel_tol=1e-9)
+
+
+def test_fluid_model_from_a_particle_file(tmp_path):
+    # a FluidModels entry after a FluidBlocks block: x = R (s x_file) + t,
+    # then v = v0 + w0 x (x - centre of mass) over the model's particles,
+    # replacing the file's velocities (SimulatorBase.cpp:1079-1093, 1095-1260)
+    rng = np.random.default_rng(2)
+    xf = rng.uniform(-0.1, 0.1, (50, 3)).astype(np.float32)
+    capi.write_bgeo(str(tmp_path / "drop.bgeo"), xf, {"velocity": rng.normal(size=(50, 3)).astype(np.float32)})
+    block = {"start": [0, 0, 0], "end": [0.3, 0.3, 0.3]}
+    only = capi.Scene(write_scene(tmp_path, {"FluidBlocks": [block]}))
+    xb, _, _ = only.particles()
+    model = {"particleFile": "drop.bgeo", "translation": [1.0, 0.5, 0.0], "rotationAxis": [0, 0, 1],
+             "rotationAngle": math.pi / 2, "scale": [2.0, 3.0, 4.0], "initialVelocity": [0.1, 0.0, 0.0],
+             "initialAngularVelocity": [0.0, 0.0, 1.0]}
+    sc = capi.Scene(write_scene(tmp_path, {"FluidBlocks": [block], "FluidModels": [model]}))
+    x, v, _ = sc.particles()
+    nb = len(xb)
+    assert sc.n == nb + 50
+    np.testing.assert_array_equal(x[:nb], xb)
+    R = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    want = (2.0 * xf.astype(np.float64)) @ R.T + [1.0, 0.5, 0.0]
+    np.testing.assert_allclose(x[nb:], want, atol=1e-6)
+    d = x[nb:].astype(np.float64) - x[nb:].astype(np.float64).mean(0)
+    np.testing.assert_allclose(v[nb:], [0.1, 0.0, 0.0] + np.cross([0.0, 0.0, 1.0], d), atol=1e-6)
+    np.testing.assert_array_equal(v[:nb], 0.0)
+    # a model from a mesh (volume sampling) or a missing file is an error
+    for bad in ({"particleFile": "unit_box.obj"}, {"particleFile": "missing.bgeo"}, {}):
+        with pytest.raises(capi.SPHError):
+            capi.Scene(write_scene(tmp_path, {"FluidBlocks": [block], "FluidModels": [bad]}))
